@@ -1,0 +1,101 @@
+"""Seeded synthetic porous media in the reference's input layout.
+
+The reference takes a segmented image, turns it into a signed-distance mask
+(positive in solid) and starts the non-wetting phase as a large sphere centred
+outside the x-low face (reference: src/applications/createMaskUBC.c:90-116,
+src/applications/createInitFromMask.c:80-104).  Here the image is replaced by
+a Boolean sphere pack (centres uniform in the fill box, overlap allowed,
+spheres added until the voxel porosity reaches the target) whose mask is the
+exact union signed distance max_l (r_l - |x - c_l|), clipped at -clip in the
+pore space.  Arrays are (nz, ny, nx) views of the column-major ghost box.
+"""
+import numpy as np
+
+
+def ghost_coords(grid):
+    """Physical x, y, z of every ghost-box voxel centre, as in
+    createIntersectionOfSpheres (reference: src/geometry/qss_initialization3d.c:136-141)."""
+    out = []
+    for a in range(3):
+        n = grid.grid_dims_ghostbox[a]
+        out.append(grid.x_lo_ghostbox[a] + grid.dx[a] * np.arange(n, dtype=np.float64))
+    return out
+
+
+def sphere_pack_mask(grid, radius, porosity, seed, clip=6.0, max_spheres=10_000_000):
+    """Signed distance to a Boolean sphere pack, > 0 in solid, float64.
+
+    radius: a number, or a (rmin, rmax) pair for radii uniform in [rmin, rmax].
+    Returns (mask, centres (m,3), radii (m,), achieved fill-box porosity)."""
+    rng = np.random.default_rng(seed)
+    xs, ys, zs = ghost_coords(grid)
+    dx = float(grid.dx[0])
+    N = [grid.grid_dims_ghostbox[a] for a in range(3)]
+    mask = np.full((N[2], N[1], N[0]), -clip, dtype=np.float64)
+    lo = [float(grid.x_lo[a]) for a in range(3)]
+    hi = [float(grid.x_hi[a]) for a in range(3)]
+    fb = (slice(grid.klo_fb, grid.khi_fb + 1), slice(grid.jlo_fb, grid.jhi_fb + 1),
+          slice(grid.ilo_fb, grid.ihi_fb + 1))
+    if np.isscalar(radius):
+        rmin = rmax = float(radius)
+    else:
+        rmin, rmax = float(radius[0]), float(radius[1])
+    vol_box = np.prod([hi[a] - lo[a] for a in range(3)])
+    er3 = (rmax ** 4 - rmin ** 4) / (4 * (rmax - rmin)) if rmax > rmin else rmin ** 3
+    expected = int(np.ceil(-np.log(porosity) * vol_box / (4.0 / 3.0 * np.pi * er3)))
+    centres, radii = [], []
+
+    def add(count):
+        for _ in range(count):
+            c = np.array([rng.uniform(lo[a], hi[a]) for a in range(3)])
+            r = rmin if rmax == rmin else rng.uniform(rmin, rmax)
+            centres.append(c)
+            radii.append(r)
+            reach = r + clip
+            sl = []
+            for a, axis in enumerate((xs, ys, zs)):
+                i0 = max(0, int(np.floor((c[a] - reach - axis[0]) / dx)))
+                i1 = min(N[a], int(np.ceil((c[a] + reach - axis[0]) / dx)) + 1)
+                sl.append((i0, i1))
+            if any(i1 <= i0 for i0, i1 in sl):
+                continue
+            X = xs[sl[0][0]:sl[0][1]] - c[0]
+            Y = ys[sl[1][0]:sl[1][1]] - c[1]
+            Z = zs[sl[2][0]:sl[2][1]] - c[2]
+            d = r - np.sqrt(Z[:, None, None] ** 2 + Y[None, :, None] ** 2 + X[None, None, :] ** 2)
+            view = mask[sl[2][0]:sl[2][1], sl[1][0]:sl[1][1], sl[0][0]:sl[0][1]]
+            np.maximum(view, d, out=view)
+
+    add(int(0.9 * expected))
+    step = max(1, expected // 50)
+    while True:
+        phi_now = float(np.mean(mask[fb] < 0))
+        if phi_now <= porosity or len(radii) >= max_spheres:
+            break
+        add(step)
+    return mask, np.array(centres), np.array(radii), phi_now
+
+
+def initial_phi(grid, mask):
+    """Non-wetting phase as a sphere of radius 0.7 L centred 0.49 L before the
+    x-low face, masked by the solid (reference:
+    src/applications/createInitFromMask.c:80-86,104)."""
+    xs, ys, zs = ghost_coords(grid)
+    L = float(grid.x_hi[0] - grid.x_lo[0])
+    r = 0.7 * L
+    cx = float(grid.x_lo[0]) - r * 0.7
+    cy = 0.5 * float(grid.x_lo[1] + grid.x_hi[1])
+    cz = 0.5 * float(grid.x_lo[2] + grid.x_hi[2])
+    d = np.sqrt((zs[:, None, None] - cz) ** 2 + (ys[None, :, None] - cy) ** 2
+                + (xs[None, None, :] - cx) ** 2) - r
+    return np.maximum(d, mask)
+
+
+def pack_case(dims, radius, porosity, seed, dtype=np.float64, clip=6.0):
+    """(grid, mask, phi0, info) for one seeded pack in run precision dtype."""
+    from .abi import make_grid
+    g = make_grid(dims, dtype=dtype)
+    mask, c, r, achieved = sphere_pack_mask(g, radius, porosity, seed, clip=clip)
+    phi0 = initial_phi(g, mask)
+    info = {"spheres": int(len(r)), "porosity": achieved, "seed": int(seed)}
+    return g, mask.astype(dtype), phi0.astype(dtype), info
