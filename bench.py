@@ -1,0 +1,231 @@
+#!/usr/bin/env python3
+"""Throughput of the LSMPQS level-set hot path on one MI355X.
+
+A step is one inner time step of constCurvModel3d on a 512^3 seeded sphere
+pack (radius U[6,12], porosity 0.36, seed 1002): HJ-WENO5 + TVD-RK3 with the
+contact-angle (theta = 30 deg) variable-coefficient path and the trapped-phase
+masks imposed, i.e. time-step probe, three fused RHS+RK stage kernels, ghost
+extrapolation and mask imposition.  The value is voxel-updates/s of the FP32
+library; the FP64 library is measured on the same input and reported beside
+it.  Inputs are device-resident before the timed region (set up by the
+library's own drain_model3d set-up on the GPU).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512] [--no-cpu]
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "lsmpqs-1.0_amd"), os.path.join(ROOT, "oracle")]
+
+from lsmpqs_amd import hiplib  # noqa: E402
+from lsmpqs_amd.abi import types_for, make_grid, default_options  # noqa: E402
+from lsmpqs_amd.inputs import sphere_pack_mask, initial_phi  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+THETA_DEG = 30.0
+
+
+def lean_arrays(dtype, mask, phi0):
+    """QSS_DataArrays record with only phi and mask (the rest NULL)."""
+    _, _, DataArrays = types_for(dtype)
+    rec = DataArrays()
+    keep = [np.ascontiguousarray(phi0, dtype=dtype), np.ascontiguousarray(mask, dtype=dtype)]
+    rec.phi = keep[0].ctypes.data
+    rec.mask = keep[1].ctypes.data
+    return rec, keep
+
+
+def options(dtype, a, b):
+    o = default_options(dtype)
+    o.order_space_accur = 5
+    o.order_time_accur = 3
+    o.theta = THETA_DEG * 3.14159 / 180
+    o.a = a
+    o.b = b
+    o.check_connectivity = 1
+    return o
+
+
+def gpu_leg(dtype, g64, mask, phi0, steps, warmup, a, b):
+    lib = hiplib.load(dtype)
+    g = make_grid(tuple(g64.grid_dims), dtype)
+    rec, keep = lean_arrays(dtype, mask, phi0)
+    o = options(dtype, a, b)
+    lib.lsmpqs_set_dt_slabs(16)
+    if lib.lsmpqs_model_prepare(C.byref(o), C.byref(rec), C.byref(g), 1) != 0:
+        raise hiplib.LsmpqsError(hiplib.last_error(lib))
+    t = hiplib.Timing()
+    if warmup:
+        if lib.lsmpqs_bench_inner_steps(C.byref(o), C.byref(rec), C.byref(g), warmup, 1, a, 0,
+                                        C.byref(t)) != 0:
+            raise hiplib.LsmpqsError(hiplib.last_error(lib))
+    t = hiplib.Timing()
+    w0 = time.perf_counter()
+    rc = lib.lsmpqs_bench_inner_steps(C.byref(o), C.byref(rec), C.byref(g), steps, 1, a, 0,
+                                      C.byref(t))
+    wall = time.perf_counter() - w0
+    if rc != 0:
+        raise hiplib.LsmpqsError(hiplib.last_error(lib))
+    # per-kernel durations in a separate pass (events around each stage launch)
+    tp = hiplib.Timing()
+    tp.profile = 1
+    if lib.lsmpqs_bench_inner_steps(C.byref(o), C.byref(rec), C.byref(g), max(3, steps // 2), 1, a,
+                                    0, C.byref(tp)) != 0:
+        raise hiplib.LsmpqsError(hiplib.last_error(lib))
+    lib.lsmpqs_release(C.byref(rec))
+    nvox = g64.grid_dims[0] * g64.grid_dims[1] * g64.grid_dims[2]
+    return dict(elapsed_ms=t.elapsed_ms, wall_s=wall, nvox=nvox, last_dt=t.last_dt,
+                stage_avg_ms=[tp.stage_ms[i] / max(1, tp.stage_launches[i]) for i in range(3)],
+                nstale=t.nstale)
+
+
+def stage_roofline(res, itemsize):
+    """Roofline of the final-stage kernel (stage 3 with imposition): its
+    algorithmic traffic per fill voxel is 6 fields (phi^(2), phi^n, mask,
+    mask_w, mask_nw read; phi^(n+1) written)."""
+    per_voxel = 6 * itemsize
+    t = res["stage_avg_ms"][2] * 1e-3
+    ach = per_voxel * res["nvox"] / t / 1e9
+    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "stage_kernel<WENO5,var,stage3,final>",
+            "alg_bytes_per_launch": per_voxel * res["nvox"]}
+
+
+def cpu_leg(n, steps=8):
+    """The reference build (oracle/_ref, OpenMP, FP64) on a bounded sample:
+    one outer step of constCurvModel3d at n^3 whose pseudo-time window holds
+    `steps` inner steps; the time is the reference's own inner-loop timer
+    (the "Level set time" line it writes to its log)."""
+    import tempfile
+    import oracle_py as op
+    from lsmpqs_amd.abi import Arrays
+    dt = np.float64
+    if not op.have_ref(dt):
+        return None
+    threads = int(os.environ.get("OMP_NUM_THREADS", "16") or 16)
+    op.set_omp_threads(threads)
+    lib = op.ref_lib(dt)
+    g = make_grid((n, n, n), dt)
+    mask, _, _, _ = sphere_pack_mask(g, (6, 12), 0.36, 1002)
+    A = Arrays(g, dt)
+    A["mask"] = mask
+    A["phi"] = initial_phi(g, mask)
+    o = options(dt, 0.02, 0.04)
+    gb, fb = op.gb6(g), op.fb6(g)
+    op.fcall(lib, "getcentralgrad3dorder2_", dt, A["mask_x"], A["mask_y"], A["mask_z"], *gb,
+             A["mask"], *gb, *fb, 1.0, 1.0, 1.0)
+    bm, um = op.Scalar(0.0, C.c_double), op.Scalar(0.0, C.c_double)
+    op.fcall(lib, "setvarcurvadv3d_", dt, A["mask"], A["mask_x"], A["mask_y"], A["mask_z"],
+             A["curvature_coeff"], A["external_velocity_x"], A["external_velocity_y"],
+             A["external_velocity_z"], bm, um, 0.04, *gb, *fb, 1.0)
+    op.fcall(lib, "setvarnorm3d_", dt, A["mask"], A["mask_x"], A["mask_y"], A["mask_z"],
+             A["normal_velocity"], 0.02, float(o.theta), *gb, *fb, 1.0)
+    o.b_max_over_dx, o.max_U_over_dx = bm.value, um.value
+    # first time step exactly as the reference forms it: H/dx at each slab's last voxel
+    nsl = g.khi_fb - g.klo_fb + 1
+    H = -1.0
+    for t in range(threads):
+        klo = g.klo_fb + nsl * t // threads
+        khi = min(g.klo_fb + nsl * (t + 1) // threads - 1, g.khi_fb)
+        if klo > khi:
+            continue
+        mh = op.Scalar(0.0, C.c_double)
+        rhs = np.zeros_like(A["phi"])
+        op.fcall(lib, "getrhs3d5th_", dt, A["phi"], A["normal_velocity"], A["curvature_coeff"],
+                 A["external_velocity_x"], A["external_velocity_y"], A["external_velocity_z"], mh,
+                 rhs, *gb, g.ilo_fb, g.ihi_fb, g.jlo_fb, g.jhi_fb, klo, khi, 1.0, 1.0, 1.0)
+        H = max(H, mh.value)
+    dt1 = 0.5 / (H + o.b_max_over_dx + o.max_U_over_dx)
+    o.check_connectivity = 0
+    o.tmax = 1e-30
+    o.tplot = dt1 * (steps - 0.5)
+    log = tempfile.NamedTemporaryFile(prefix="lsm_ref_", suffix=".log", delete=False).name
+    libc = C.CDLL("libc.so.6")
+    libc.fopen.restype = C.c_void_p
+    libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
+    libc.fclose.argtypes = [C.c_void_p]
+    fp = libc.fopen(log.encode(), b"w")
+    lib.constCurvModel3d.restype = C.c_double
+    lib.constCurvModel3d(C.byref(o), A.ptr(), C.byref(g), C.c_void_p(fp))
+    libc.fclose(C.c_void_p(fp))
+    secs = None
+    for line in open(log):
+        if "Level set time" in line:
+            secs = float(line.split("=")[1])
+    os.unlink(log)
+    if not secs:
+        return None
+    per_step = secs / steps
+    return {"value": round(n ** 3 / per_step, 1), "unit": "voxel-updates/s", "cores": threads,
+            "kind": "reference",
+            "sample": f"reference constCurvModel3d (OpenMP {threads} threads, FP64) at {n}^3, "
+                      f"WENO5+TVD-RK3 theta 30: {steps} inner steps of one outer step, timed by "
+                      f"the reference's own inner-loop timer ({per_step * 1e3:.1f} ms/step)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--cpu-n", type=int, default=128)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(z-slab sharding is described in DESIGN.md, not built)", file=sys.stderr)
+        sys.exit(2)
+    n = args.n
+    g64 = make_grid((n, n, n), np.float64)
+    t0 = time.perf_counter()
+    mask, _, radii, por = sphere_pack_mask(g64, (6, 12), 0.36, 1002)
+    phi0 = initial_phi(g64, mask)
+    print(f"# input: {n}^3 pack, {len(radii)} spheres, porosity {por:.4f}, "
+          f"{time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    a, b = 0.02, 0.04
+    r32 = gpu_leg(np.float32, g64, mask, phi0, args.steps, args.warmup, a, b)
+    print(f"# fp32 done: {r32['elapsed_ms'] / args.steps:.3f} ms/step", file=sys.stderr, flush=True)
+    r64 = gpu_leg(np.float64, g64, mask, phi0, args.steps, args.warmup, a, b)
+    print(f"# fp64 done: {r64['elapsed_ms'] / args.steps:.3f} ms/step", file=sys.stderr, flush=True)
+    del mask, phi0
+    v32 = r32["nvox"] * args.steps / (r32["elapsed_ms"] * 1e-3)
+    v64 = r64["nvox"] * args.steps / (r64["elapsed_ms"] * 1e-3)
+    cpu = None if args.no_cpu else cpu_leg(args.cpu_n)
+    line = {
+        "metric": "voxel-updates/s per RK3 step (WENO5, theta 30, trapped masks), FP32 vs FP64, 1 MI355X",
+        "value": round(v32, 1),
+        "unit": "voxel-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r32["elapsed_ms"] / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Boolean sphere pack, radius U[6,12], porosity 0.36, seed 1002)",
+        "config": {"workload": f"{n}^3 drainage step, WENO5 + TVD-RK3, theta 30 deg, dx 1, "
+                               f"16 dt slabs, trapping masks on", "grid": n, "scheme": "WENO5+TVD-RK3"},
+        "roofline": stage_roofline(r32, 4),
+        "cpu_baseline": cpu,
+        "fp64": {"value": round(v64, 1), "ms_per_step": round(r64["elapsed_ms"] / args.steps, 4),
+                 "roofline": stage_roofline(r64, 8)},
+        "fp32_over_fp64": round(v32 / v64, 3),
+        "stage_avg_ms": {"f32": [round(x, 4) for x in r32["stage_avg_ms"]],
+                         "f64": [round(x, 4) for x in r64["stage_avg_ms"]]},
+        "host_wall_s": {"f32": round(r32["wall_s"], 4), "f64": round(r64["wall_s"], 4)},
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

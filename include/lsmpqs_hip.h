@@ -1,0 +1,124 @@
+/*
+ * lsmpqs_hip.h -- the C ABI of the MI355X LSMPQS level-set library.
+ *
+ * liblsmpqs_hip_f32.so / liblsmpqs_hip_f64.so export the entry points below
+ * under the reference's own names, so C code written against the reference's
+ * headers links against this library unchanged (same QSSLIB_REAL).  Each
+ * declaration cites the reference interface it replaces.  All arrays are
+ * HOST arrays over the ghost box; the library stages them to the GPU, runs
+ * there, and writes back what the reference would have modified.  Scratch
+ * arrays of QSS_DataArrays (lse_rhs, scratch1/2, phi_next) are not written.
+ *
+ * Errors: the reference has no error codes on this path.  Here
+ * constCurvModel3d / constCurvModel3dNoVar return -2 on a device or input
+ * error (the reference's -1 keeps its meaning: saturation limit reached);
+ * void entry points leave their outputs untouched on error.  The message of
+ * the calling thread's last error is lsmpqs_hip_last_error().
+ *
+ * Entry points are not re-entrant for one Options or QSS_DataArrays record
+ * (the reference mutates both); separate records may be used from separate
+ * threads.
+ */
+#ifndef LSMPQS_HIP_H
+#define LSMPQS_HIP_H
+
+#include <stdio.h>
+#include "QSSLIB_config.h"
+#include "qss_grid.h"
+#include "qss_options.h"
+#include "qss_data_arrays.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- evolution (reference: src/level_set_evolution/constCurvModel3d.h:29-30) ----
+ * Outer/inner pseudo-time loop at fixed pressure.  Variable-coefficient path
+ * (reads normal_velocity, curvature_coeff, external_velocity_*; kernel order
+ * from order_space_accur: 2 ENO2, 3/4 ENO3, 5 WENO5).  Returns the pseudo
+ * time reached, -1 when the saturation limit is hit, -2 on error. */
+QSSLIB_REAL constCurvModel3d(Options *options, QSS_DataArrays *p, Grid *g, FILE *fp_out);
+/* Constant-coefficient path (a and options->b; ENO2 for order 2, ENO3 otherwise). */
+QSSLIB_REAL constCurvModel3dNoVar(Options *options, QSS_DataArrays *p, Grid *g, FILE *fp_out,
+                                  QSSLIB_REAL a);
+
+/* ---- model drivers (reference: src/level_set_evolution/drain_model3d.h:31,
+ *      imbibe_model3d.h:32) ----
+ * Curvature schedule a += b*dc up to amax (drain) / down to amin (imbibe).
+ * Writes data_step_N.gz in the working directory after each curvature step,
+ * as the reference does, unless LSMPQS_NO_STEP_FILES=1. */
+void drain_model3d(Options *options, QSS_DataArrays *p, Grid *g, FILE *fp_out);
+void imbibe_model3d(Options *options, QSS_DataArrays *p, Grid *g, FILE *fp_out);
+
+/* ---- trapping (reference: src/connectivity/connectivity.h:23-27) ---- */
+void trapComponents_mask(QSS_DataArrays *p, Grid *g, Options *o);
+/* labels of phi_bin (26-neighbourhood, each component labelled with its
+   largest linear index, 0 on background) into p->connectivity */
+void findConnectivity3d(QSS_DataArrays *p, Grid *g);
+void getMainInd_new(Options *o, QSS_DataArrays *p, Grid *g);
+
+/* ---- reinitialization (reference: src/reinitialization/reinitialize_top.h:28) ---- */
+void reinitializeSubcellFix3d(QSSLIB_REAL *data, Grid *g, Options *o);
+
+/* ---- setup and boundary helpers (reference: src/utilities/qss_general_util.h:28-37) ---- */
+void signedLinearExtrapolationBCqss(QSSLIB_REAL *phi, Grid *g, int bdry_location_idx);
+void initializeDisconnectedMasks(QSSLIB_REAL *data, Grid *g);
+void createReservoirInlet3d(QSS_DataArrays *p, Grid *g);
+void setSpaceDerivFunc(Grid *g, Options *options);
+void setThetaOverlap(QSS_DataArrays *p, Grid *g, Options *options);
+
+/* ---- records and files (reference: src/utilities/qss_options.h:213-272,
+ *      src/utilities/qss_data_arrays.h:84-180) ---- */
+Options *createOptionsDefault(void);
+void destroyOptions(Options *options);
+void writeDataArrayQSS(QSSLIB_REAL *data, Grid *grid, char *file_name, int zip_status);
+QSSLIB_REAL *readDataArrayQSS(int *grid_dims_ghostbox, char *file_name);
+
+/* ---- extensions with no counterpart in the reference ---- */
+
+/* message of the calling thread's last error ("" if none) */
+const char *lsmpqs_hip_last_error(void);
+
+/* Number of z-slabs whose last voxels set the time step.  The reference's
+   dt uses H/dx at the last fill voxel of each OpenMP thread's slab, so it
+   depends on the thread count; pass the OMP_NUM_THREADS a reference run used.
+   0 (default): LSMPQS_DT_SLABS, else OMP_NUM_THREADS, else the CPU count. */
+int lsmpqs_set_dt_slabs(int nslabs);
+
+/* Reference-behaviour switches, all set by default; returns the old value.
+   1: ENO3 x line reads phi(i+3,j,j); 2: WENO5 normal term with + sign;
+   4: |vel_n| < 1e-5 reuses the previous voxel's normal term. */
+unsigned lsmpqs_set_quirks(unsigned mask);
+
+/* free the device state kept for record p */
+void lsmpqs_release(QSS_DataArrays *p);
+
+/* Benchmark / test hook: run exactly nsteps inner time steps (probe, RK
+   stages, ghost extrapolation, mask imposition) on the device state of p.
+   upload != 0 first copies phi, mask, mask_w, mask_nw and (var_path) the
+   coefficient arrays from the host.  Times with HIP events on the library's
+   stream. */
+typedef struct {
+  int profile;                /* in: also time each stage kernel */
+  double elapsed_ms;          /* out: event-to-event time of the nsteps */
+  double stage_ms[3];         /* out: summed stage-kernel time, stages 1..3 */
+  long long stage_launches[3];
+  long long inner_steps;
+  double last_dt;
+  int nstale;                 /* voxels reusing a previous normal term */
+} lsmpqs_timing;
+int lsmpqs_bench_inner_steps(Options *o, QSS_DataArrays *p, Grid *g, int nsteps, int var_path,
+                             QSSLIB_REAL a, int upload, lsmpqs_timing *out);
+/* Run the set-up half of drain_model3d (drain != 0) or imbibe_model3d on
+   p->phi and p->mask, plus the coefficients and trapped masks of the first
+   curvature step (a = options->a), leaving the state on the device for
+   lsmpqs_bench_inner_steps(..., upload = 0, ...). */
+int lsmpqs_model_prepare(Options *o, QSS_DataArrays *p, Grid *g, int drain);
+/* copy one device field ("phi", "mask_w", "mask_nw") back to p */
+int lsmpqs_download(QSS_DataArrays *p, Grid *g, const char *field);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

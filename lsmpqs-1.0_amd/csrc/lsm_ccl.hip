@@ -1,0 +1,178 @@
+// lsm_ccl.hip -- connected-component labelling on the GPU and the
+// trapped-phase mask rebuild.
+//
+// Labelling is a concurrent union-find over the whole ghost box: every
+// foreground voxel starts as its own root; each voxel hooks itself to its
+// foreground neighbours (13 of the 26, the other 13 hook to it) by linking
+// the smaller root under the larger one with atomicCAS; a final pass
+// compresses every path.  Linking only ever points a root at a larger index,
+// so each component ends rooted at its largest linear index, which is exactly
+// the label the reference's sequential union-find produces
+// (reference: src/connectivity/connectivity.c:48-83,148-217).  Labels are
+// therefore bit-identical to the reference's, not just the same partition.
+#include "lsm_kernels.h"
+
+namespace lsm {
+
+__device__ __forceinline__ int uf_find(int* L, int x) {
+  volatile int* V = L;
+  int p = V[x];
+  while (p != x) {
+    int gp = V[p];
+    if (gp != p) V[x] = gp;  // path halving; gp is an ancestor of x
+    x = p;
+    p = gp;
+  }
+  return x;
+}
+
+__device__ __forceinline__ void uf_unite(int* L, int a, int b) {
+  while (true) {
+    a = uf_find(L, a);
+    b = uf_find(L, b);
+    if (a == b) return;
+    if (a > b) { int t = a; a = b; b = t; }
+    int old = atomicCAS(&L[a], a, b);
+    if (old == a) return;
+    a = old;
+  }
+}
+
+__global__ void ccl_init_kernel(const uint8_t* bin, int* L, int64_t n) {
+  for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n;
+       l += (int64_t)gridDim.x * blockDim.x)
+    L[l] = bin[l] ? (int)l : -1;
+}
+
+// conn = 6: face neighbours only; conn = 26: faces, edges and corners
+__global__ void ccl_merge_kernel(Box bx, const uint8_t* bin, int* L, int conn) {
+  int64_t n = bx.size();
+  for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n;
+       l += (int64_t)gridDim.x * blockDim.x) {
+    if (!bin[l]) continue;
+    int64_t t = l / bx.nx;
+    int i = (int)(l - t * bx.nx), j = (int)(t % bx.ny), k = (int)(t / bx.ny);
+    // forward half of the neighbourhood: (dk > 0) or (dk == 0 and dj > 0) or
+    // (dk == 0, dj == 0, di > 0)
+    for (int dk = 0; dk <= 1; ++dk)
+      for (int dj = (dk ? -1 : 0); dj <= 1; ++dj)
+        for (int di = ((dk || dj) ? -1 : 1); di <= 1; ++di) {
+          if (conn == 6 && (abs(di) + abs(dj) + abs(dk)) != 1) continue;
+          int i2 = i + di, j2 = j + dj, k2 = k + dk;
+          if (i2 < 0 || i2 >= bx.nx || j2 < 0 || j2 >= bx.ny || k2 >= bx.nz) continue;
+          int64_t m = bx.at(i2, j2, k2);
+          if (bin[m]) uf_unite(L, (int)l, (int)m);
+        }
+  }
+}
+
+__global__ void ccl_compress_kernel(int* L, int64_t n) {
+  for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n;
+       l += (int64_t)gridDim.x * blockDim.x) {
+    if (L[l] >= 0) L[l] = uf_find(L, (int)l);
+  }
+}
+
+static int grid_for(int64_t n) {
+  int64_t b = (n + 255) / 256;
+  return (int)(b > 16384 ? 16384 : (b < 1 ? 1 : b));
+}
+
+// labels: root (largest linear index of the component) on foreground, -1 on background
+hipError_t launch_label_components(const Box& bx, const uint8_t* bin, int* L, int conn,
+                                   hipStream_t st) {
+  int64_t n = bx.size();
+  hipLaunchKernelGGL(ccl_init_kernel, dim3(grid_for(n)), dim3(256), 0, st, bin, L, n);
+  hipLaunchKernelGGL(ccl_merge_kernel, dim3(grid_for(n)), dim3(256), 0, st, bx, bin, L, conn);
+  hipLaunchKernelGGL(ccl_compress_kernel, dim3(grid_for(n)), dim3(256), 0, st, L, n);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Trapped-phase masks (reference: src/connectivity/trapComponents.c:156-375).
+// The reference's connectivity value is the root label on foreground voxels
+// and 0 elsewhere; a voxel is "disconnected" when that value is neither the
+// value at the inlet/outlet index nor 0.
+
+__device__ __forceinline__ int conn_value(const int* L, int64_t idx) {
+  int v = L[idx];
+  return v >= 0 ? v : 0;
+}
+
+// phi = merge(phi, mask_nw); bin = phi < 0, plus the inlet plane forced to 1
+__global__ void trap_nw_bin_kernel(TrapArgs a) {
+  const Box bx = a.bx;
+  int64_t n = bx.size();
+  int ci = (int)(0.5 * (bx.ilo + bx.ihi)), ck = (int)(0.5 * (bx.klo + bx.khi));
+  for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n;
+       l += (int64_t)gridDim.x * blockDim.x) {
+    real mn = a.mask_nw[l];
+    real p = a.phi[l];
+    if (mn > real(0)) { p = -mn; a.phi[l] = p; }
+    uint8_t b = p < real(0) ? 1 : 0;
+    int64_t t = l / bx.nx;
+    int i = (int)(l - t * bx.nx), j = (int)(t % bx.ny), k = (int)(t / bx.ny);
+    if (a.center_inlet) {
+      if (i == ci && k == ck) b = 1;
+    } else if (i == bx.ilo - 1 && j >= bx.jlo && j <= bx.jhi && k >= bx.klo && k <= bx.khi) {
+      b = 1;
+    }
+    a.bin[l] = b;
+  }
+}
+
+// mask_nw from the non-wetting labels; s = max(-phi, mask, mask_nw);
+// bin = s < 0, plus the outlet plane(s) forced to 1
+__global__ void trap_w_bin_kernel(TrapArgs a) {
+  const Box bx = a.bx;
+  int64_t n = bx.size();
+  int main_val = conn_value(a.labels, a.nw_ind);
+  for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n;
+       l += (int64_t)gridDim.x * blockDim.x) {
+    int cv = conn_value(a.labels, l);
+    bool disc = !(cv == main_val || cv == 0);
+    real p = a.phi[l];
+    real mn = (disc && p < real(0)) ? -p : a.fill;
+    a.mask_nw[l] = mn;
+    real s = -p;
+    real m = a.mask[l];
+    s = (m > s) ? m : s;
+    s = (mn > s) ? mn : s;
+    a.s[l] = s;
+    uint8_t b = s < real(0) ? 1 : 0;
+    int64_t t = l / bx.nx;
+    int i = (int)(l - t * bx.nx), j = (int)(t % bx.ny), k = (int)(t / bx.ny);
+    if (a.center_inlet) {
+      if (k == bx.klo - 1 || k == bx.khi + 1 || i == bx.ilo - 1 || i == bx.ihi + 1) b = 1;
+    } else if (i == bx.ihi + 1 && j >= bx.jlo && j <= bx.jhi && k >= bx.klo && k <= bx.khi) {
+      b = 1;
+    }
+    a.bin[l] = b;
+  }
+}
+
+__global__ void trap_w_mask_kernel(TrapArgs a) {
+  int64_t n = a.bx.size();
+  int main_val = conn_value(a.labels, a.w_ind);
+  for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n;
+       l += (int64_t)gridDim.x * blockDim.x) {
+    int cv = conn_value(a.labels, l);
+    bool disc = !(cv == main_val || cv == 0);
+    real s = a.s[l];
+    a.mask_w[l] = (disc && s < real(0)) ? -s : a.fill;
+  }
+}
+
+hipError_t launch_trap_components(const TrapArgs& a, int conn, hipStream_t st) {
+  int64_t n = a.bx.size();
+  hipLaunchKernelGGL(trap_nw_bin_kernel, dim3(grid_for(n)), dim3(256), 0, st, a);
+  hipError_t e = launch_label_components(a.bx, a.bin, a.labels, conn, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(trap_w_bin_kernel, dim3(grid_for(n)), dim3(256), 0, st, a);
+  e = launch_label_components(a.bx, a.bin, a.labels, conn, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(trap_w_mask_kernel, dim3(grid_for(n)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+}  // namespace lsm

@@ -1,0 +1,182 @@
+// lsm_device.h -- per-voxel numerics of the LSMPQS 3D hot path, as device
+// functions templated on the storage/compute type T (float or double).
+//
+// Every expression keeps the reference's operation order and the library is
+// compiled without FMA contraction, so the double instantiation reproduces
+// the reference's FP64 results bit for bit wherever only + - * / sqrt are
+// involved.  Citations are to src/spatial_derivatives/qss_spatial_derivatives3d.f
+// unless stated otherwise.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lsm {
+
+enum Scheme { ENO2 = 2, ENO3 = 3, WENO5 = 5 };
+
+template <typename T> __device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
+template <typename T> __device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
+template <typename T> __device__ __forceinline__ T tabs(T a) { return fabs(a); }
+
+// Geometry of one ghost box and its fill box; passed by value to kernels.
+struct Box {
+  int nx, ny, nz;             // ghost-box extents
+  int ilo, ihi, jlo, jhi, klo, khi;  // fill box (inclusive)
+  __host__ __device__ int64_t at(int i, int j, int k) const {
+    return (int64_t)i + (int64_t)nx * ((int64_t)j + (int64_t)ny * k);
+  }
+  __host__ __device__ int64_t size() const { return (int64_t)nx * ny * nz; }
+  __host__ __device__ bool in_fill(int i, int j, int k) const {
+    return i >= ilo && i <= ihi && j >= jlo && j <= jhi && k >= klo && k <= khi;
+  }
+};
+
+// HJ-ENO2 one-sided derivatives from p[0..6] = phi(-3..+3) (:386-456)
+template <typename T>
+__device__ __forceinline__ void eno2(const T* p, T inv_d, T& plus, T& minus) {
+  const T half = T(0.5);
+  T D1 = p[3] - p[2];
+  T D1p = p[4] - p[3];
+  T D2 = p[4] + p[2] - T(2) * p[3];
+  T D2p = p[5] + p[3] - T(2) * p[4];
+  T D2m = p[1] + p[3] - T(2) * p[2];
+  plus = (tabs(D2) < tabs(D2p)) ? (D1p - half * D2) * inv_d : (D1p - half * D2p) * inv_d;
+  minus = (tabs(D2m) < tabs(D2)) ? (D1 + half * D2m) * inv_d : (D1 + half * D2) * inv_d;
+}
+
+// HJ-ENO3 (:672-745); the minus side is scaled by inv_minus, which the
+// reference sets to 1/dx on every axis (:823,900)
+template <typename T>
+__device__ __forceinline__ void eno3(const T* p, T inv_plus, T inv_minus, T& plus, T& minus) {
+  const T half = T(0.5), third = T(1.0 / 3.0), sixth = T(1.0 / 6.0);
+  T D1 = p[3] - p[2], D1p = p[4] - p[3], D1m = p[2] - p[1];
+  T D1mm = p[1] - p[0], D1pp = p[5] - p[4], D1ppp = p[6] - p[5];
+  T D2 = D1p - D1, D2p = D1pp - D1p, D2m = D1 - D1m;
+  T D2pp = D1ppp - D1pp, D2mm = D1m - D1mm;
+  T D3 = D2 - D2m, D3p = D2p - D2, D3pp = D2pp - D2p, D3m = D2m - D2mm;
+  T v = D1p;
+  if (tabs(D2) < tabs(D2p)) {
+    v = v - half * D2;
+    v = (tabs(D3) < tabs(D3p)) ? v - sixth * D3 : v - sixth * D3p;
+  } else {
+    v = v - half * D2p;
+    v = (tabs(D3p) < tabs(D3pp)) ? v + third * D3p : v + third * D3pp;
+  }
+  plus = v * inv_plus;
+  v = D1;
+  if (tabs(D2m) < tabs(D2)) {
+    v = v + half * D2m;
+    v = (tabs(D3m) < tabs(D3)) ? v + third * D3m : v + third * D3;
+  } else {
+    v = v + half * D2;
+    v = (tabs(D3) < tabs(D3p)) ? v - sixth * D3 : v - sixth * D3p;
+  }
+  minus = v * inv_minus;
+}
+
+// HJ-WENO5 combination with linear weights 0.1/0.6/0.3 (:1960-2035)
+template <typename T>
+__device__ __forceinline__ T weno5_combine(T v1, T v2, T v3, T v4, T v5) {
+  const T one_third = T(1.0 / 3.0), seven_sixths = T(7.0 / 6.0);
+  const T eleven_sixths = T(11.0 / 6.0), one_sixth = T(1.0 / 6.0);
+  const T five_sixths = T(5.0 / 6.0), thirteen_twelfths = T(13.0 / 12.0);
+  const T one_fourth = T(0.25), tiny = T(1.0e-35);
+  T eps = T(1e-6) * tmax(tmax(tmax(tmax(v1 * v1, v2 * v2), v3 * v3), v4 * v4), v5 * v5) + tiny;
+  T p1 = one_third * v1 - seven_sixths * v2 + eleven_sixths * v3;
+  T p2 = -one_sixth * v2 + five_sixths * v3 + one_third * v4;
+  T p3 = one_third * v3 + five_sixths * v4 - one_sixth * v5;
+  T t, S1, S2, S3;
+  t = v1 - T(2) * v2 + v3; S1 = thirteen_twelfths * (t * t);
+  t = v1 - T(4) * v2 + T(3) * v3; S1 = S1 + one_fourth * (t * t);
+  t = v2 - T(2) * v3 + v4; S2 = thirteen_twelfths * (t * t);
+  t = v2 - v4; S2 = S2 + one_fourth * (t * t);
+  t = v3 - T(2) * v4 + v5; S3 = thirteen_twelfths * (t * t);
+  t = T(3) * v3 - T(4) * v4 + v5; S3 = S3 + one_fourth * (t * t);
+  T a1, a2, a3;
+  t = S1 + eps; a1 = T(0.1) / (t * t);
+  t = S2 + eps; a2 = T(0.6) / (t * t);
+  t = S3 + eps; a3 = T(0.3) / (t * t);
+  T inv = T(1) / (a1 + a2 + a3);
+  a1 = a1 * inv; a2 = a2 * inv; a3 = a3 * inv;
+  return a1 * p1 + a2 * p2 + a3 * p3;
+}
+
+template <typename T>
+__device__ __forceinline__ void weno5(const T* p, T inv_d, T& plus, T& minus) {
+  T D1 = p[3] - p[2], D1p = p[4] - p[3], D1m = p[2] - p[1];
+  T D1pp = p[5] - p[4], D1ppp = p[6] - p[5], D1mm = p[1] - p[0];
+  plus = weno5_combine(D1ppp * inv_d, D1pp * inv_d, D1p * inv_d, D1 * inv_d, D1m * inv_d);
+  minus = weno5_combine(D1mm * inv_d, D1m * inv_d, D1 * inv_d, D1p * inv_d, D1pp * inv_d);
+}
+
+// Upwind derivative pair along one axis with the scheme's scalings.
+template <typename T, int S>
+__device__ __forceinline__ void axis_derivs(const T* p, T inv_axis, T inv_x, T& plus, T& minus) {
+  if (S == ENO2) eno2(p, inv_axis, plus, minus);
+  else if (S == ENO3) eno3(p, inv_axis, inv_x, plus, minus);
+  else weno5(p, inv_axis, plus, minus);
+}
+
+// Godunov |grad phi|^2 selected by the sign of vel_n (:460-480)
+template <typename T>
+__device__ __forceinline__ T godunov_sq(T vn, const T* d) {
+  T a, b, s;
+  if (vn > T(0)) {
+    a = tmax(d[1], T(0)); a = a * a; b = tmin(d[0], T(0)); b = b * b; s = tmax(a, b);
+    a = tmax(d[3], T(0)); a = a * a; b = tmin(d[2], T(0)); b = b * b; s = s + tmax(a, b);
+    a = tmax(d[5], T(0)); a = a * a; b = tmin(d[4], T(0)); b = b * b; s = s + tmax(a, b);
+  } else {
+    a = tmin(d[1], T(0)); a = a * a; b = tmax(d[0], T(0)); b = b * b; s = tmax(a, b);
+    a = tmin(d[3], T(0)); a = a * a; b = tmax(d[2], T(0)); b = b * b; s = s + tmax(a, b);
+    a = tmin(d[5], T(0)); a = a * a; b = tmax(d[4], T(0)); b = b * b; s = s + tmax(a, b);
+  }
+  return s;
+}
+
+// H/dx of the CFL bound (:488-513); d = {x+, x-, y+, y-, z+, z-}
+template <typename T>
+__device__ __forceinline__ T h_over_dx(T vn, const T* d, T dx, T dy, T dz) {
+  T mdx = tmax(tmax(dx, dy), dz);
+  T max_dx_sq = mdx * mdx;
+  T inv_dx = T(1.0) / dx, inv_dy = T(1.0) / dy, inv_dz = T(1.0) / dz;
+  T xc = tmax(tabs(d[0]), tabs(d[1]));
+  T yc = tmax(tabs(d[2]), tabs(d[3]));
+  T zc = tmax(tabs(d[4]), tabs(d[5]));
+  T ng = sqrt(xc * xc + yc * yc + zc * zc + max_dx_sq);
+  return tabs(vn) / ng * (xc * inv_dx + yc * inv_dy + zc * inv_dz);
+}
+
+// kappa |grad phi| from wide central differences (:518-556).  c = phi(i,j,k);
+// xm2..xp2 the x line, likewise y, z; cross = {(+1,+1),(-1,-1),(-1,+1),(+1,-1)}
+// in the xy, xz and yz planes.
+template <typename T>
+__device__ __forceinline__ T curvature(T c, T xm2, T xm1, T xp1, T xp2, T ym2, T ym1, T yp1, T yp2,
+                                       T zm2, T zm1, T zp1, T zp2, const T* xy, const T* xz,
+                                       const T* yz, T fx, T fy, T fz) {
+  T px = (xp1 - xm1) * fx;
+  T py = (yp1 - ym1) * fy;
+  T pz = (zp1 - zm1) * fz;
+  T pxx = (xp2 + xm2 - T(2) * c) * fx * fx;
+  T pzz = (zp2 + zm2 - T(2) * c) * fz * fz;
+  T pyy = (yp2 + ym2 - T(2) * c) * fy * fy;
+  T pxy = (xy[0] + xy[1] - xy[2] - xy[3]) * fx * fy;
+  T pxz = (xz[0] + xz[1] - xz[2] - xz[3]) * fx * fz;
+  T pyz = (yz[0] + yz[1] - yz[2] - yz[3]) * fy * fz;
+  T g2 = px * px + py * py + pz * pz;
+  if (g2 < T(1.e-5)) return T(0);
+  T curv = pxx * py * py + pyy * px * px - T(2) * pxy * px * py
+         + pxx * pz * pz + pzz * px * px - T(2) * pxz * px * pz
+         + pyy * pz * pz + pzz * py * py - T(2) * pyz * py * pz;
+  return curv / g2;
+}
+
+// Smoothed Heaviside (reference: src/utilities/qss_util3d.f:450-491)
+template <typename T>
+__device__ __forceinline__ T heaviside(T x, T eps) {
+  const T pi = T(3.14159265358979323846), one_over_pi = T(0.31830988618379);
+  auto basic = [](T v) { return v < T(0) ? T(0) : (v > T(0) ? T(1) : T(0.5)); };
+  T v1 = basic(x + eps), v2 = basic(eps - x), v3 = basic(x - eps);
+  return (T(0.5) + x / (T(2) * eps) + (T(0.5) * one_over_pi) * sin((pi * x) / eps)) * v1 * v2 + v3;
+}
+
+}  // namespace lsm

@@ -1,0 +1,781 @@
+// lsm_driver.hip -- the level-set evolution drivers on one GPU, behind the
+// reference's C entry points.
+//
+// Every entry point copies the host arrays it reads to the device, runs the
+// whole computation device-resident on one stream, and copies back what the
+// reference would have modified.  Control decisions that need scalars (the
+// inner pseudo-time loop, the outer stopping tests) read back one or two
+// values per step through pinned memory; the inner loop is pipelined so the
+// GPU never waits for the host (see inner_loop()).
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <algorithm>
+#include <stdexcept>
+#include <vector>
+#include "lsm_session.h"
+#include "lsmpqs_hip.h"
+
+namespace lsm {
+
+#define DCHECK(x)                                                                \
+  do {                                                                           \
+    hipError_t e_ = (x);                                                         \
+    if (e_ != hipSuccess)                                                        \
+      throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+static size_t bytes(const Session* s) { return (size_t)s->n * sizeof(real); }
+
+static void up(Session* s, real* d, const real* h) {
+  if (!h) throw std::runtime_error("required host array is NULL");
+  DCHECK(hipMemcpyAsync(d, h, bytes(s), hipMemcpyHostToDevice, s->st));
+}
+static void down(Session* s, real* h, const real* d) {
+  if (h) DCHECK(hipMemcpyAsync(h, d, bytes(s), hipMemcpyDeviceToHost, s->st));
+}
+static void sync(Session* s) { DCHECK(hipStreamSynchronize(s->st)); }
+
+// Which right-hand-side kernel the reference's dispatcher picks
+// (reference: src/utilities/qss_general_util.c:115-141).  Returns the scheme
+// order; *cst says whether the constant-coefficient variant is selected.
+static int rhs_scheme(const Options* o, bool var_path) {
+  switch (o->order_space_accur) {
+  case 2: return 2;
+  case 3: case 4: return 3;
+  case 5: return var_path ? 5 : 3;
+  default: return 3;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Reinitialization with the subcell fix (reference:
+// src/reinitialization/reinitialize_top.c:297-377).  `f` is replaced by the
+// result; A, B and S serve as next-iterate, phi0 copy and distance buffers.
+static void reinit_dev(Session* s, real*& f, const Grid* g, const Options* o) {
+  const Box& bx = s->bx;
+  real* phi0 = s->B;
+  real* dist0 = s->S;
+  DCHECK(hipMemcpyAsync(phi0, f, bytes(s), hipMemcpyDeviceToDevice, s->st));
+  DCHECK(launch_reinit_distance(bx, dist0, phi0, g->dx[0], g->dx[1], g->dx[2], s->st));
+  real t_r = 0, dt_r = real(0.5) * g->dx[0];
+  while (t_r < o->tmax_r) {
+    DCHECK(launch_reinit_step(bx, s->A, f, phi0, dist0, dt_r, g->dx[0], g->dx[1], g->dx[2], s->st));
+    BcArgs ba{};
+    ba.bx = bx; ba.src = s->A; ba.dst = s->A;
+    DCHECK(launch_bc(ba, false, s->st));
+    std::swap(f, s->A);
+    t_r = t_r + dt_r;
+  }
+}
+
+// Trapped-phase masks (reference: src/connectivity/trapComponents.c:156-375)
+static void trap_dev(Session* s, const Grid* g, const Options* o) {
+  ensure_labels(s);
+  TrapArgs ta{};
+  ta.bx = s->bx;
+  ta.phi = s->phi; ta.mask = s->mask; ta.mask_nw = s->mask_nw; ta.mask_w = s->mask_w;
+  ta.s = s->S; ta.bin = s->bin; ta.labels = s->labels;
+  ta.nw_ind = o->phi_nw_ind; ta.w_ind = o->phi_w_ind;
+  ta.center_inlet = o->center_inlet;
+  ta.fill = -20 * g->dx[0];
+  if (o->phi_nw_ind < 0 || o->phi_nw_ind >= s->n || o->phi_w_ind < 0 || o->phi_w_ind >= s->n)
+    throw std::runtime_error("phi_nw_ind / phi_w_ind outside the ghost box (call getMainInd_new)");
+  int conn = getenv("LSMPQS_CONNECTIVITY") ? atoi(getenv("LSMPQS_CONNECTIVITY")) : 26;
+  if (conn != 6) conn = 26;
+  DCHECK(launch_trap_components(ta, conn, s->st));
+}
+
+// Sorted list of fill voxels with |vel_n| < 1e-5 and, for each, the voxel
+// whose normal term the reference's loop reuses: the closest preceding voxel
+// of the same z-slab, in (k, j, i) loop order, with |vel_n| >= 1e-5.
+static void build_stale(Session* s, const Grid* g) {
+  s->nstale = 0;
+  if (!(quirks() & QUIRK_STALE_NORM)) return;
+  const Box& bx = s->bx;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    if (s->stale_cap == 0 || attempt == 1) {
+      int cap = attempt == 0 ? (1 << 16) : *s->h_count;
+      if (s->stale_list) { (void)hipFree(s->stale_list); (void)hipFree(s->stale_idx); (void)hipFree(s->stale_src); }
+      DCHECK(hipMalloc((void**)&s->stale_list, (size_t)cap * sizeof(int64_t)));
+      DCHECK(hipMalloc((void**)&s->stale_idx, (size_t)cap * sizeof(int64_t)));
+      DCHECK(hipMalloc((void**)&s->stale_src, (size_t)cap * sizeof(int64_t)));
+      s->stale_cap = cap;
+    }
+    DCHECK(hipMemsetAsync(s->count, 0, sizeof(int), s->st));
+    DCHECK(launch_collect_stale(bx, s->vn, s->stale_list, s->count, s->stale_cap, s->st));
+    DCHECK(hipMemcpyAsync(s->h_count, s->count, sizeof(int), hipMemcpyDeviceToHost, s->st));
+    sync(s);
+    if (*s->h_count <= s->stale_cap) break;
+  }
+  int cnt = *s->h_count;
+  if (cnt == 0) return;
+  std::vector<int64_t> idx(cnt), src(cnt);
+  DCHECK(hipMemcpyAsync(idx.data(), s->stale_list, (size_t)cnt * sizeof(int64_t),
+                        hipMemcpyDeviceToHost, s->st));
+  sync(s);
+  std::sort(idx.begin(), idx.end());
+  int T = dt_slabs(g);
+  int nsl = bx.khi - bx.klo + 1;
+  int64_t plane = (int64_t)bx.nx * bx.ny;
+  for (int e = 0; e < cnt; ++e) {
+    int64_t v = idx[e];
+    int k = (int)(v / plane);
+    int64_t r = v - k * plane;
+    int j = (int)(r / bx.nx), i = (int)(r - (int64_t)j * bx.nx);
+    // first plane of the slab containing k
+    int t = (int)(((int64_t)(k - bx.klo + 1) * T - 1) / nsl);
+    while (t > 0 && bx.klo + (int)((int64_t)nsl * t / T) > k) --t;
+    while (t + 1 < T && bx.klo + (int)((int64_t)nsl * (t + 1) / T) <= k) ++t;
+    int slab_klo = bx.klo + (int)((int64_t)nsl * t / T);
+    int64_t prev;
+    if (i > bx.ilo) prev = bx.at(i - 1, j, k);
+    else if (j > bx.jlo) prev = bx.at(bx.ihi, j - 1, k);
+    else if (k > slab_klo) prev = bx.at(bx.ihi, bx.jhi, k - 1);
+    else prev = -1;
+    if (prev >= 0 && e > 0 && idx[e - 1] == prev) src[e] = src[e - 1];
+    else src[e] = prev;
+  }
+  DCHECK(hipMemcpyAsync(s->stale_idx, idx.data(), (size_t)cnt * sizeof(int64_t),
+                        hipMemcpyHostToDevice, s->st));
+  DCHECK(hipMemcpyAsync(s->stale_src, src.data(), (size_t)cnt * sizeof(int64_t),
+                        hipMemcpyHostToDevice, s->st));
+  sync(s);
+  s->nstale = cnt;
+}
+
+// ---------------------------------------------------------------------------
+// One inner time step: dt probe, the RK stages (each a fused RHS+stage
+// kernel followed by the ghost extrapolation), mask imposition fused into the
+// last stage.  Returns nothing; dt lands in s->dt (device) and s->h_dt
+// (pinned, valid after ev_dt).
+struct StepCfg {
+  int scheme;
+  bool var;
+  int order;
+  real a_const, b_const;
+  real norm_sign;
+  real bmax, umax, overlap;
+  int conn, slabs, quirk_eno3;
+  real dx, dy, dz;
+};
+
+static void stage_timed(Session* s, const StageArgs& a, const StepCfg& c, int stage, bool fin) {
+  if (s->profile) DCHECK(hipEventRecord(s->stage_ev[2 * stage], s->st));
+  DCHECK(launch_stage(a, c.scheme, c.var, stage, fin, s->st));
+  if (s->profile) {
+    DCHECK(hipEventRecord(s->stage_ev[2 * stage + 1], s->st));
+  }
+}
+
+static void inner_step(Session* s, const StepCfg& c) {
+  const Box& bx = s->bx;
+  ProbeArgs pa{};
+  pa.bx = bx; pa.phi = s->phi; pa.vn = c.var ? s->vn : nullptr; pa.a_const = c.a_const;
+  pa.dx = c.dx; pa.dy = c.dy; pa.dz = c.dz; pa.eno3_quirk = c.quirk_eno3; pa.nslabs = c.slabs;
+  pa.bmax_over_dx = c.bmax; pa.umax_over_dx = c.umax; pa.dt_out = s->dt;
+  DCHECK(launch_probe(pa, c.scheme, s->st));
+  DCHECK(hipMemcpyAsync(s->h_dt, s->dt, sizeof(real), hipMemcpyDeviceToHost, s->st));
+  DCHECK(hipEventRecord(s->ev_dt, s->st));
+
+  StageArgs sa{};
+  sa.bx = bx; sa.dt = s->dt;
+  sa.vn = s->vn; sa.b = s->b; sa.vx = s->vx; sa.vy = s->vy; sa.vz = s->vz;
+  sa.a_const = c.a_const; sa.b_const = c.b_const;
+  sa.dx = c.dx; sa.dy = c.dy; sa.dz = c.dz;
+  sa.norm_sign = c.norm_sign; sa.eno3_quirk = c.quirk_eno3;
+  sa.mask = s->mask; sa.mask_w = s->mask_w; sa.mask_nw = s->mask_nw;
+  sa.overlap = c.overlap; sa.conn = c.conn;
+  sa.stale_idx = s->stale_idx; sa.stale_src = s->stale_src;
+  sa.nstale = (c.var ? s->nstale : 0);
+  BcArgs bc{};
+  bc.bx = bx;
+
+  if (c.order <= 1) {
+    // phi_new = max(phi + dt L(phi), masks) into B; face layers into A
+    sa.in = s->phi; sa.un = s->phi; sa.out = s->B; sa.pre = s->A;
+    stage_timed(s, sa, c, 1, true);
+    bc.src = s->A; bc.dst = s->B; bc.mask = s->mask; bc.mask_w = s->mask_w; bc.mask_nw = s->mask_nw;
+    bc.overlap = c.overlap; bc.conn = c.conn;
+    DCHECK(launch_bc(bc, true, s->st));
+    std::swap(s->phi, s->B);
+    return;
+  }
+  sa.in = s->phi; sa.un = s->phi; sa.out = s->A; sa.pre = nullptr;
+  stage_timed(s, sa, c, 1, false);
+  bc.src = s->A; bc.dst = s->A;
+  DCHECK(launch_bc(bc, false, s->st));
+  if (c.order == 2) {
+    sa.in = s->A; sa.un = s->phi; sa.out = s->phi; sa.pre = s->B;
+    stage_timed(s, sa, c, 2, true);
+    bc.src = s->B;
+  } else {
+    sa.in = s->A; sa.un = s->phi; sa.out = s->B; sa.pre = nullptr;
+    stage_timed(s, sa, c, 2, false);
+    bc.src = s->B; bc.dst = s->B;
+    DCHECK(launch_bc(bc, false, s->st));
+    sa.in = s->B; sa.un = s->phi; sa.out = s->phi; sa.pre = s->A;
+    stage_timed(s, sa, c, 3, true);
+    bc.src = s->A;
+  }
+  bc.dst = s->phi; bc.mask = s->mask; bc.mask_w = s->mask_w; bc.mask_nw = s->mask_nw;
+  bc.overlap = c.overlap; bc.conn = c.conn;
+  DCHECK(launch_bc(bc, true, s->st));
+}
+
+static StepCfg make_cfg(Session* s, const Options* o, const Grid* g, bool var, real a_const) {
+  StepCfg c{};
+  c.scheme = rhs_scheme(o, var);
+  c.var = var;
+  c.order = o->order_time_accur < 1 ? 1 : (o->order_time_accur > 3 ? 3 : o->order_time_accur);
+  c.a_const = a_const; c.b_const = o->b;
+  c.norm_sign = (var && c.scheme == 5 && (quirks() & QUIRK_WENO5_SIGN)) ? real(1) : real(-1);
+  c.bmax = o->b_max_over_dx; c.umax = o->max_U_over_dx; c.overlap = o->overlap;
+  c.conn = o->check_connectivity ? 1 : 0;
+  c.slabs = dt_slabs(g);
+  c.quirk_eno3 = (quirks() & QUIRK_ENO3_READ) ? 1 : 0;
+  c.dx = g->dx[0]; c.dy = g->dx[1]; c.dz = g->dx[2];
+  (void)s;
+  return c;
+}
+
+static void profile_collect(Session* s, int order) {
+  if (!s->profile) return;
+  sync(s);
+  for (int st = 1; st <= (order < 1 ? 1 : (order > 3 ? 3 : order)); ++st) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, s->stage_ev[2 * st], s->stage_ev[2 * st + 1]) == hipSuccess) {
+      s->tm.stage_ms[st] += ms;
+      s->tm.stage_launches[st] += 1;
+    }
+  }
+}
+
+// The pseudo-time loop `while (dt_sub < tplot)` (constCurvModel3d.c:127-244).
+// dt of step n is known right after its probe kernel, so the host waits on
+// that event while the GPU is still running step n's stages, then enqueues
+// step n+1 (or stops).  Returns dt_sub; *last_dt gets the final dt.
+static real inner_loop(Session* s, const StepCfg& c, real tplot, real* last_dt) {
+  real dt_sub = 0, dt = 0;
+  while (dt_sub < tplot) {
+    inner_step(s, c);
+    DCHECK(hipEventSynchronize(s->ev_dt));
+    dt = *s->h_dt;
+    if (!(dt > real(0)) || !isfinite((double)dt))
+      throw std::runtime_error("non-positive or non-finite time step");
+    profile_collect(s, c.order);
+    dt_sub += dt;
+    s->tm.inner_steps++;
+  }
+  if (last_dt) *last_dt = dt;
+  return dt_sub;
+}
+
+// outer reductions: out[0] = volume of {f < 0}, out[1] = local max-norm diff vs f2
+static void outer_reduce(Session* s, const real* f, const real* f2, real zone, real eps, real dV) {
+  DCHECK(launch_outer_reduce(s->bx, f, f2, zone, eps, dV, s->part, s->red, s->st));
+  DCHECK(hipMemcpyAsync(s->h_red, s->red, 2 * sizeof(double), hipMemcpyDeviceToHost, s->st));
+  sync(s);
+}
+
+// Outer loop of constCurvModel3d / constCurvModel3dNoVar
+// (reference: src/level_set_evolution/constCurvModel3d.c:50-334, 336-610).
+static real ccm_dev(Session* s, Options* o, const Grid* g, FILE* fp, bool var, real a_const) {
+  const Box& bx = s->bx;
+  real vsmall = g->dx[0] * g->dx[1] * g->dx[2];
+  real eps = o->eps_coefficient * g->dx[0];
+  real dV = g->dx[0] * g->dx[1] * g->dx[2];
+  double max_abs_err = 100, vol_phi, vol_pore, satn, satn_old;
+  real t = 0;
+  StepCfg c = make_cfg(s, o, g, var, a_const);
+  if (s->profile && s->stage_ev.empty()) {
+    s->stage_ev.resize(8);
+    for (auto& e : s->stage_ev) DCHECK(hipEventCreate(&e));
+  }
+  outer_reduce(s, s->mask, nullptr, 0, eps, dV);
+  vol_pore = s->h_red[0];
+  outer_reduce(s, s->phi, nullptr, 0, eps, dV);
+  vol_phi = s->h_red[0];
+  satn = vol_phi / vol_pore;
+  (void)bx;
+  while ((t < o->tmax) && (max_abs_err > (double)o->eps_stop)) {
+    s->tm.outer_steps++;
+    DCHECK(hipMemcpyAsync(s->phi_prev, s->phi, bytes(s), hipMemcpyDeviceToDevice, s->st));
+    DCHECK(hipEventRecord(s->ev_a, s->st));
+    real last_dt = 0;
+    real dt_sub = inner_loop(s, c, o->tplot, &last_dt);
+    DCHECK(hipEventRecord(s->ev_b, s->st));
+    sync(s);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, s->ev_a, s->ev_b);
+    s->tm.inner_ms += ms;
+    if (fp) fprintf(fp, "1 GPU: Level set time = %lf\n", ms * 1e-3);
+    t += dt_sub;
+    if (var) DCHECK(launch_impose(s->phi, s->mask, s->phi, 0, s->n, s->st));
+    if (o->check_connectivity) {
+      trap_dev(s, g, o);
+      reinit_dev(s, s->mask_w, g, o);
+      reinit_dev(s, s->mask_nw, g, o);
+      if (var && o->conserve_imbibe) {
+        DCHECK(launch_trap_velocity(s->bx, s->mask_nw, s->vn, s->b, s->vx, s->vy, s->vz,
+                                    g->dx[0], s->st));
+        build_stale(s, g);
+      }
+      DCHECK(launch_merge(s->phi, s->mask_nw, s->n, s->st));
+    }
+    if (fp) fprintf(fp, "Reinitializing....");
+    reinit_dev(s, s->phi, g, o);
+    if (fp) fprintf(fp, "Reinitialized\n");
+    outer_reduce(s, s->phi, s->phi_prev, o->err_check_zone, eps, dV);
+    vol_phi = s->h_red[0];
+    max_abs_err = s->h_red[1];
+    satn_old = satn;
+    satn = vol_phi / vol_pore;
+    if ((satn > (double)o->vol_frac_max) || (vol_phi <= (double)vsmall)) return -1;
+    if (fp) {
+      fprintf(fp, "t = %4.1f\t", (double)t);
+      fprintf(fp, "max_abs_err = %4.3f,\t", max_abs_err);
+      fprintf(fp, "nw phase satn pct = %4.3f\n", satn * 100);
+      fflush(fp);
+    }
+    if (o->use_satn_stop) {
+      if (fabsf((float)(100 * (satn - satn_old))) < 0.001) break;
+    }
+  }
+  return t;
+}
+
+// ---------------------------------------------------------------------------
+// Model drivers (reference: src/level_set_evolution/drain_model3d.c:49-165,
+// imbibe_model3d.c:49-161)
+
+static void set_theta_overlap(Options* o, const Grid* g) {
+  real theta = o->theta;
+  real rad = (real)(3.14159 / 180);
+  if (theta <= 20 * rad) o->overlap = 0;
+  else if (theta <= 40 * rad) o->overlap = (real)(0.3 * g->dx[0]);
+  else if (theta <= 50 * rad) o->overlap = (real)(0.5 * g->dx[0]);
+  else o->overlap = (real)(1 * g->dx[0]);
+}
+
+static void main_ind(Options* o, const Grid* g) {
+  int nx = g->grid_dims_ghostbox[0], nxy = nx * g->grid_dims_ghostbox[1];
+  int xi = o->center_inlet ? (int)(0.5 * (g->ilo_fb + g->ihi_fb)) : g->ilo_fb - 1;
+  int yi = (int)(0.5 * (g->jlo_fb + g->jhi_fb));
+  int zi = (int)(0.5 * (g->klo_fb + g->khi_fb));
+  o->phi_nw_ind = xi + yi * nx + zi * nxy;
+  o->phi_w_ind = (g->ihi_fb + 1) + yi * nx + zi * nxy;
+}
+
+static void disconnected_init(Session* s, real* f, const Grid* g) {
+  real cx = (real)(0.5 * (g->x_lo[0] + g->x_hi[0]));
+  real cy = (real)(0.5 * (g->x_lo[1] + g->x_hi[1]));
+  real cz = (real)(0.5 * (g->x_lo[2] + g->x_hi[2]));
+  real r = 2 * (g->x_hi[0] - g->x_lo[0]);
+  DCHECK(launch_sphere(f, s->bx, g->x_lo_ghostbox, g->dx, cx, cy, cz, r, -1, s->st));
+}
+
+static void reservoir_box(Session* s, const Grid* g) {
+  real corner[3] = {g->x_lo_ghostbox[0], g->x_lo_ghostbox[1], g->x_lo_ghostbox[2]};
+  real side[3] = {3 * g->dx[0], g->x_hi_ghostbox[1] - g->x_lo_ghostbox[1],
+                  g->x_hi_ghostbox[2] - g->x_lo_ghostbox[2]};
+  DCHECK(launch_box(s->phi_extra, s->bx, g->x_lo_ghostbox, g->dx, corner, side, -1, s->st));
+}
+
+static bool step_files_enabled() {
+  const char* e = getenv("LSMPQS_NO_STEP_FILES");
+  return !(e && *e && atoi(e) != 0);
+}
+
+// Setup before the curvature loop (drain_model3d.c:56-100): trapped masks as
+// one huge sphere, reservoir box, mask reinitialization (drainage only), mask
+// gradient, overlap, inlet imposition, inlet/outlet indices.
+static void model_setup(Session* s, Options* o, QSS_DataArrays* p, Grid* g, bool drain, bool var) {
+  ensure_grad(s);
+  if (var) ensure_var(s);
+  up(s, s->phi, p->phi);
+  up(s, s->mask, p->mask);
+  disconnected_init(s, s->mask_w, g);
+  disconnected_init(s, s->mask_nw, g);
+  reservoir_box(s, g);
+  if (drain && o->mask_reinit) reinit_dev(s, s->mask, g, o);
+  DCHECK(launch_central_grad(s->bx, s->gx, s->gy, s->gz, s->mask, g->dx[0], g->dx[1], g->dx[2],
+                             s->st));
+  set_theta_overlap(o, g);
+  if (o->reservoir_inlet) {
+    DCHECK(launch_min(s->phi, s->phi_extra, s->n, s->st));
+    DCHECK(launch_impose(s->phi, s->mask, s->phi, 0, s->n, s->st));
+  }
+  main_ind(o, g);
+  if (p->mask_disconn_init) down(s, p->mask_disconn_init, s->mask_nw);
+  sync(s);
+}
+
+// Coefficients of one curvature step (drain_model3d.c:108-131): the
+// time-step bounds, and on the variable path b, V and a = a0 from the mask.
+static void model_coeffs(Session* s, Options* o, const Grid* g, bool var, bool drain, real a0) {
+  if (var) {
+    DCHECK(launch_var_curv_adv(s->bx, s->b, s->vx, s->vy, s->vz, s->mask, s->gx, s->gy, s->gz,
+                               o->b, g->dx[0], s->st));
+    real inv = real(1.0) / g->dx[0];
+    DCHECK(launch_coeff_reduce(s->bx, s->b, s->vx, s->vy, s->vz, inv, s->part, s->red, s->st));
+    DCHECK(hipMemcpyAsync(s->h_red, s->red, 2 * sizeof(double), hipMemcpyDeviceToHost, s->st));
+    sync(s);
+    real bmax = (real)s->h_red[0], umax = (real)s->h_red[1];
+    o->max_U_over_dx = umax;
+    o->b_max_over_dx = 2 * bmax * (inv * inv + inv * inv + inv * inv);
+  } else {
+    real ix = 1 / g->dx[0], iy = 1 / g->dx[1], iz = 1 / g->dx[2];
+    o->b_max_over_dx = 2 * o->b * (ix * ix + iy * iy + iz * iz);
+    o->max_U_over_dx = 0;
+  }
+  if (!drain) o->conserve_imbibe = 1;
+  if (var) {
+    real ct = (real)cos((double)o->theta);
+    DCHECK(launch_var_norm(s->bx, s->vn, s->mask, s->gx, s->gy, s->gz, a0, ct, g->dx[0], s->st));
+    build_stale(s, g);
+  }
+}
+
+static void model_dev(Options* o, QSS_DataArrays* p, Grid* g, FILE* fp, bool drain) {
+  Session* s = session_for(p, g);
+  s->tm = Timers();
+  bool var = (o->theta > 0) || o->use_var_theta;
+  model_setup(s, o, p, g, drain, var);
+  real a0 = o->a, da = o->b * o->dc;
+  int step = o->init_step + 1;
+  while (drain ? (a0 < o->amax) : (a0 > o->amin)) {
+    printf("\nStep %d: a = %f\n", step, (double)a0);
+    if (fp) fprintf(fp, "\nStep %d: a = %f\n", step, (double)a0);
+    model_coeffs(s, o, g, var, drain, a0);
+    if (o->check_connectivity) trap_dev(s, g, o);
+    real t_step = ccm_dev(s, o, g, fp, var, a0);
+    if (t_step == -1) break;
+    if (drain) a0 += da; else a0 -= da;
+    if (o->reservoir_inlet) DCHECK(launch_min(s->phi, s->phi_extra, s->n, s->st));
+    DCHECK(launch_impose(s->phi, s->mask, s->phi, 0, s->n, s->st));
+    if (step_files_enabled()) {
+      char fname[256];
+      snprintf(fname, sizeof fname, "data_step_%d", step);
+      down(s, p->phi, s->phi);
+      sync(s);
+      writeDataArrayQSS(p->phi, g, fname, 1);
+    }
+    step += 1;
+  }
+  down(s, p->phi, s->phi);
+  down(s, p->mask, s->mask);
+  down(s, p->mask_w, s->mask_w);
+  down(s, p->mask_nw, s->mask_nw);
+  down(s, p->phi_prev, s->phi_prev);
+  down(s, p->phi_extra, s->phi_extra);
+  down(s, p->mask_x, s->gx);
+  down(s, p->mask_y, s->gy);
+  down(s, p->mask_z, s->gz);
+  if (var) {
+    down(s, p->normal_velocity, s->vn);
+    down(s, p->curvature_coeff, s->b);
+    down(s, p->external_velocity_x, s->vx);
+    down(s, p->external_velocity_y, s->vy);
+    down(s, p->external_velocity_z, s->vz);
+  }
+  sync(s);
+}
+
+}  // namespace lsm
+
+using namespace lsm;
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+
+extern "C" {
+
+const char* lsmpqs_hip_last_error(void) { return lsm::last_error(); }
+
+int lsmpqs_set_dt_slabs(int n) {
+  lsm::set_slabs(n);
+  return n;
+}
+
+unsigned lsmpqs_set_quirks(unsigned q) {
+  unsigned old = lsm::quirks();
+  lsm::set_quirks(q);
+  return old;
+}
+
+void lsmpqs_release(QSS_DataArrays* p) { lsm::session_release(p); }
+
+#define API_TRY try {
+#define API_CATCH(ret)                      \
+  }                                         \
+  catch (const std::exception& e) {         \
+    lsm::set_error(e.what());               \
+    fprintf(stderr, "lsmpqs_hip: %s\n", e.what()); \
+    return ret;                             \
+  }
+
+QSSLIB_REAL constCurvModel3d(Options* o, QSS_DataArrays* p, Grid* g, FILE* fp) {
+  API_TRY
+  lsm::set_error("");
+  Session* s = session_for(p, g);
+  s->tm = Timers();
+  ensure_var(s);
+  up(s, s->phi, p->phi);
+  up(s, s->mask, p->mask);
+  if (o->check_connectivity) {
+    up(s, s->mask_w, p->mask_w);
+    up(s, s->mask_nw, p->mask_nw);
+  }
+  up(s, s->vn, p->normal_velocity);
+  up(s, s->b, p->curvature_coeff);
+  up(s, s->vx, p->external_velocity_x);
+  up(s, s->vy, p->external_velocity_y);
+  up(s, s->vz, p->external_velocity_z);
+  sync(s);
+  build_stale(s, g);
+  real t = ccm_dev(s, o, g, fp, true, 0);
+  down(s, p->phi, s->phi);
+  down(s, p->phi_prev, s->phi_prev);
+  if (o->check_connectivity) {
+    down(s, p->mask_w, s->mask_w);
+    down(s, p->mask_nw, s->mask_nw);
+    if (o->conserve_imbibe) {
+      down(s, p->normal_velocity, s->vn);
+      down(s, p->curvature_coeff, s->b);
+      down(s, p->external_velocity_x, s->vx);
+      down(s, p->external_velocity_y, s->vy);
+      down(s, p->external_velocity_z, s->vz);
+    }
+  }
+  sync(s);
+  return t;
+  API_CATCH(-2)
+}
+
+QSSLIB_REAL constCurvModel3dNoVar(Options* o, QSS_DataArrays* p, Grid* g, FILE* fp,
+                                  QSSLIB_REAL a) {
+  API_TRY
+  lsm::set_error("");
+  Session* s = session_for(p, g);
+  s->tm = Timers();
+  up(s, s->phi, p->phi);
+  up(s, s->mask, p->mask);
+  if (o->check_connectivity) {
+    up(s, s->mask_w, p->mask_w);
+    up(s, s->mask_nw, p->mask_nw);
+  }
+  real t = ccm_dev(s, o, g, fp, false, a);
+  down(s, p->phi, s->phi);
+  down(s, p->phi_prev, s->phi_prev);
+  if (o->check_connectivity) {
+    down(s, p->mask_w, s->mask_w);
+    down(s, p->mask_nw, s->mask_nw);
+  }
+  sync(s);
+  return t;
+  API_CATCH(-2)
+}
+
+void drain_model3d(Options* o, QSS_DataArrays* p, Grid* g, FILE* fp) {
+  API_TRY
+  lsm::set_error("");
+  model_dev(o, p, g, fp, true);
+  API_CATCH()
+}
+
+void imbibe_model3d(Options* o, QSS_DataArrays* p, Grid* g, FILE* fp) {
+  API_TRY
+  lsm::set_error("");
+  model_dev(o, p, g, fp, false);
+  API_CATCH()
+}
+
+void trapComponents_mask(QSS_DataArrays* p, Grid* g, Options* o) {
+  API_TRY
+  lsm::set_error("");
+  Session* s = session_for(p, g);
+  up(s, s->phi, p->phi);
+  up(s, s->mask, p->mask);
+  up(s, s->mask_nw, p->mask_nw);
+  trap_dev(s, g, o);
+  down(s, p->phi, s->phi);
+  down(s, p->mask_nw, s->mask_nw);
+  down(s, p->mask_w, s->mask_w);
+  if (p->scratch1) {
+    DCHECK(hipMemsetAsync(s->S, 0, bytes(s), s->st));
+    down(s, p->scratch1, s->S);
+  }
+  std::vector<int> lab;
+  if (p->connectivity || p->phi_bin) {
+    lab.resize((size_t)s->n);
+    DCHECK(hipMemcpyAsync(lab.data(), s->labels, (size_t)s->n * sizeof(int), hipMemcpyDeviceToHost,
+                          s->st));
+    if (p->phi_bin)
+      DCHECK(hipMemcpyAsync(p->phi_bin, s->bin, (size_t)s->n, hipMemcpyDeviceToHost, s->st));
+  }
+  sync(s);
+  if (p->connectivity)
+    for (int64_t l = 0; l < s->n; ++l) p->connectivity[l] = lab[l] >= 0 ? lab[l] : 0;
+  API_CATCH()
+}
+
+void findConnectivity3d(QSS_DataArrays* p, Grid* g) {
+  API_TRY
+  lsm::set_error("");
+  Session* s = session_for(p, g);
+  ensure_labels(s);
+  DCHECK(hipMemcpyAsync(s->bin, p->phi_bin, (size_t)s->n, hipMemcpyHostToDevice, s->st));
+  int conn = getenv("LSMPQS_CONNECTIVITY") ? atoi(getenv("LSMPQS_CONNECTIVITY")) : 26;
+  DCHECK(launch_label_components(s->bx, s->bin, s->labels, conn == 6 ? 6 : 26, s->st));
+  std::vector<int> lab((size_t)s->n);
+  DCHECK(hipMemcpyAsync(lab.data(), s->labels, (size_t)s->n * sizeof(int), hipMemcpyDeviceToHost,
+                        s->st));
+  sync(s);
+  for (int64_t l = 0; l < s->n; ++l) p->connectivity[l] = lab[l] >= 0 ? lab[l] : 0;
+  API_CATCH()
+}
+
+void reinitializeSubcellFix3d(QSSLIB_REAL* data, Grid* g, Options* o) {
+  API_TRY
+  lsm::set_error("");
+  // a private session keyed to the data pointer
+  Session* s = session_for((const QSS_DataArrays*)data, g);
+  up(s, s->D, data);
+  reinit_dev(s, s->D, g, o);
+  down(s, data, s->D);
+  sync(s);
+  API_CATCH()
+}
+
+void signedLinearExtrapolationBCqss(QSSLIB_REAL* phi, Grid* g, int bdry_location_idx) {
+  API_TRY
+  lsm::set_error("");
+  if (bdry_location_idx != 9) return;  // the reference's 3D dispatcher handles only 9
+  Session* s = session_for((const QSS_DataArrays*)phi, g);
+  up(s, s->D, phi);
+  BcArgs ba{};
+  ba.bx = s->bx; ba.src = s->D; ba.dst = s->D;
+  DCHECK(launch_bc(ba, false, s->st));
+  down(s, phi, s->D);
+  sync(s);
+  API_CATCH()
+}
+
+void getMainInd_new(Options* o, QSS_DataArrays* p, Grid* g) {
+  (void)p;
+  main_ind(o, g);
+}
+
+void setThetaOverlap(QSS_DataArrays* p, Grid* g, Options* o) {
+  (void)p;
+  if (g->num_dims == 3) set_theta_overlap(o, g);
+}
+
+void setSpaceDerivFunc(Grid* g, Options* o) {
+  // kernels are chosen from order_space_accur and theta at run time; the
+  // slot is cleared so no stale pointer into another library survives
+  (void)g;
+  o->space_deriv_func = nullptr;
+}
+
+void createReservoirInlet3d(QSS_DataArrays* p, Grid* g) {
+  API_TRY
+  Session* s = session_for(p, g);
+  ensure_grad(s);
+  reservoir_box(s, g);
+  down(s, p->phi_extra, s->phi_extra);
+  sync(s);
+  API_CATCH()
+}
+
+void initializeDisconnectedMasks(QSSLIB_REAL* data, Grid* g) {
+  API_TRY
+  Session* s = session_for((const QSS_DataArrays*)data, g);
+  disconnected_init(s, s->D, g);
+  down(s, data, s->D);
+  sync(s);
+  API_CATCH()
+}
+
+int lsmpqs_bench_inner_steps(Options* o, QSS_DataArrays* p, Grid* g, int nsteps, int var_path,
+                             QSSLIB_REAL a, int upload, lsmpqs_timing* out) {
+  API_TRY
+  lsm::set_error("");
+  Session* s = session_for(p, g);
+  bool var = var_path != 0;
+  if (upload) {
+    if (var) ensure_var(s);
+    up(s, s->phi, p->phi);
+    up(s, s->mask, p->mask);
+    up(s, s->mask_w, p->mask_w);
+    up(s, s->mask_nw, p->mask_nw);
+    if (var) {
+      up(s, s->vn, p->normal_velocity);
+      up(s, s->b, p->curvature_coeff);
+      up(s, s->vx, p->external_velocity_x);
+      up(s, s->vy, p->external_velocity_y);
+      up(s, s->vz, p->external_velocity_z);
+    }
+    sync(s);
+    if (var) build_stale(s, g);
+  }
+  s->profile = out && out->profile;
+  if (s->profile && s->stage_ev.empty()) {
+    s->stage_ev.resize(8);
+    for (auto& e : s->stage_ev) DCHECK(hipEventCreate(&e));
+  }
+  s->tm = Timers();
+  StepCfg c = make_cfg(s, o, g, var, a);
+  DCHECK(hipEventRecord(s->ev_a, s->st));
+  for (int n = 0; n < nsteps; ++n) {
+    inner_step(s, c);
+    profile_collect(s, c.order);
+  }
+  DCHECK(hipEventRecord(s->ev_b, s->st));
+  sync(s);
+  float ms = 0;
+  DCHECK(hipEventElapsedTime(&ms, s->ev_a, s->ev_b));
+  if (out) {
+    out->elapsed_ms = ms;
+    for (int st = 0; st < 3; ++st) {
+      out->stage_ms[st] = s->tm.stage_ms[st + 1];
+      out->stage_launches[st] = s->tm.stage_launches[st + 1];
+    }
+    out->inner_steps = nsteps;
+    out->last_dt = (double)*s->h_dt;
+    out->nstale = s->nstale;
+  }
+  s->profile = false;
+  return 0;
+  API_CATCH(-1)
+}
+
+int lsmpqs_model_prepare(Options* o, QSS_DataArrays* p, Grid* g, int drain) {
+  API_TRY
+  lsm::set_error("");
+  Session* s = session_for(p, g);
+  bool var = (o->theta > 0) || o->use_var_theta;
+  model_setup(s, o, p, g, drain != 0, var);
+  model_coeffs(s, o, g, var, drain != 0, o->a);
+  if (o->check_connectivity) trap_dev(s, g, o);
+  sync(s);
+  return 0;
+  API_CATCH(-1)
+}
+
+int lsmpqs_download(QSS_DataArrays* p, Grid* g, const char* field) {
+  API_TRY
+  Session* s = session_for(p, g);
+  if (!strcmp(field, "phi")) down(s, p->phi, s->phi);
+  else if (!strcmp(field, "mask_w")) down(s, p->mask_w, s->mask_w);
+  else if (!strcmp(field, "mask_nw")) down(s, p->mask_nw, s->mask_nw);
+  else return -1;
+  sync(s);
+  return 0;
+  API_CATCH(-1)
+}
+
+}  // extern "C"

@@ -1,0 +1,132 @@
+// lsm_kernels.h -- host-side launchers of the gfx950 kernels.  One build of
+// the library serves one QSSLIB_REAL; every launcher enqueues on `st` and
+// returns the HIP status of the launch.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "QSSLIB_config.h"
+#include "lsm_device.h"
+
+namespace lsm {
+
+typedef QSSLIB_REAL real;
+
+// Inputs of one fused RHS + Runge-Kutta stage over the fill box.
+struct StageArgs {
+  Box bx;
+  const real* in;        // field the right-hand side is taken of (ghosts valid)
+  const real* un;        // u^n, read pointwise by stages 2 and 3
+  real* out;             // stage result (fill box)
+  real* pre;             // final stage: pre-imposition values of the two layers next to each face
+  const real* dt;        // device scalar time step
+  const real *vn, *b, *vx, *vy, *vz;   // variable coefficients (var path)
+  real a_const, b_const;               // constant coefficients (const path)
+  real dx, dy, dz;
+  real norm_sign;        // -1, or +1 for the reference's WENO5 normal term
+  int eno3_quirk;        // read phi(i+3,j,j) in the ENO3 x line, as the reference does
+  const real *mask, *mask_w, *mask_nw; // final stage: phi = max(phi, mask - overlap, mask_w, mask_nw)
+  real overlap;
+  int conn;              // trapped-phase masks active
+  const int64_t* stale_idx;  // sorted voxels with |vel_n| < 1e-5 (var path)
+  const int64_t* stale_src;  // voxel whose normal term each of them reuses, or -1
+  int nstale;
+};
+
+// scheme: 2, 3 or 5; var: variable coefficients; stage: 1, 2 or 3;
+// final: apply the mask imposition and write the face layers to `pre`.
+hipError_t launch_stage(const StageArgs& a, int scheme, bool var, int stage, bool final_stage,
+                        hipStream_t st);
+
+// Time step from H/dx at the last fill voxel of each of `nslabs` z-slabs:
+// dt = 0.5 / (max H + b_max_over_dx + max_U_over_dx), written to *dt_out.
+struct ProbeArgs {
+  Box bx;
+  const real* phi;
+  const real* vn;        // NULL on the const path
+  real a_const;
+  real dx, dy, dz;
+  int eno3_quirk;
+  int nslabs;
+  real bmax_over_dx, umax_over_dx;
+  real* dt_out;
+};
+hipError_t launch_probe(const ProbeArgs& a, int scheme, hipStream_t st);
+
+// Signed linear extrapolation of every ghost voxel from the fill box, composed
+// in the reference's face order x-, x+, y-, y+, z-, z+.  With final_mask set,
+// reads `src` fill values, writes dst ghosts = max(BC, mask - overlap [, mask_w, mask_nw]).
+struct BcArgs {
+  Box bx;
+  const real* src;
+  real* dst;
+  const real *mask, *mask_w, *mask_nw;
+  real overlap;
+  int conn;
+};
+hipError_t launch_bc(const BcArgs& a, bool final_mask, hipStream_t st);
+
+// Elementwise over the whole ghost box.
+hipError_t launch_impose(real* dst, const real* m, const real* src, real overlap, int64_t n,
+                         hipStream_t st);                     // dst = max(src, m - overlap)
+hipError_t launch_merge(real* phi, const real* m, int64_t n, hipStream_t st);  // phi = m>0 ? -m : phi
+hipError_t launch_min(real* phi, const real* e, int64_t n, hipStream_t st);    // phi = min(phi, e)
+hipError_t launch_fill(real* dst, real v, int64_t n, hipStream_t st);
+
+// FP64 reductions over the fill box.  out[0] = smoothed volume of {f1 < 0}
+// (width eps), out[1] = max |f1 - f2| over |f1| < zone (-1 if none; skipped
+// when f2 is NULL).  partials needs reduce_partials() doubles.
+int reduce_partials();
+hipError_t launch_outer_reduce(const Box& bx, const real* f1, const real* f2, real zone, real eps,
+                               real dV, double* partials, double* out, hipStream_t st);
+// max of a fill-box field (the reference's b_max / U_max scans); writes out[0]
+hipError_t launch_coeff_reduce(const Box& bx, const real* b, const real* vx, const real* vy,
+                               const real* vz, real inv_dx, double* partials, double* out,
+                               hipStream_t st);
+
+// Subcell-fix reinitialization pieces.
+hipError_t launch_reinit_distance(const Box& bx, real* dist0, const real* phi0, real dx, real dy,
+                                  real dz, hipStream_t st);
+hipError_t launch_reinit_step(const Box& bx, real* next, const real* data, const real* phi0,
+                              const real* dist0, real dt_r, real dx, real dy, real dz,
+                              hipStream_t st);
+
+// Coefficient fields of the variable-coefficient path.
+hipError_t launch_central_grad(const Box& bx, real* gx, real* gy, real* gz, const real* f,
+                               real dx, real dy, real dz, hipStream_t st);
+hipError_t launch_var_norm(const Box& bx, real* vn, const real* mask, const real* gx,
+                           const real* gy, const real* gz, real a0, real cos_theta, real dx,
+                           hipStream_t st);
+hipError_t launch_var_curv_adv(const Box& bx, real* b, real* vx, real* vy, real* vz,
+                               const real* mask, const real* gx, const real* gy, const real* gz,
+                               real const_b, real dx, hipStream_t st);
+hipError_t launch_trap_velocity(const Box& bx, const real* phi_w, real* vn, real* b, real* vx,
+                                real* vy, real* vz, real dx, hipStream_t st);
+// voxels with |vn| < 1e-5: appends fill-box indices to list (capacity cap), counts in *count
+hipError_t launch_collect_stale(const Box& bx, const real* vn, int64_t* list, int* count, int cap,
+                                hipStream_t st);
+
+// Analytic initialisers over the ghost box.
+hipError_t launch_sphere(real* phi, const Box& bx, const real* xlo, const real* dx, real cx,
+                         real cy, real cz, real r, int inside_flag, hipStream_t st);
+hipError_t launch_box(real* phi, const Box& bx, const real* xlo, const real* dx, const real* corner,
+                      const real* side, int inside_flag, hipStream_t st);
+
+// Connected components (26-neighbourhood) and trapped-phase masks.
+struct TrapArgs {
+  Box bx;
+  real* phi;
+  const real* mask;
+  real* mask_nw;
+  real* mask_w;
+  real* s;               // wetting-phase field max(-phi, mask, mask_nw)
+  uint8_t* bin;
+  int* labels;
+  int nw_ind, w_ind;
+  int center_inlet;
+  real fill;             // -20 dx
+};
+hipError_t launch_trap_components(const TrapArgs& a, int connectivity, hipStream_t st);
+hipError_t launch_label_components(const Box& bx, const uint8_t* bin, int* labels,
+                                   int connectivity, hipStream_t st);
+
+}  // namespace lsm

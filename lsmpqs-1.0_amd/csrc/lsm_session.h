@@ -1,0 +1,73 @@
+// lsm_session.h -- device-resident state of one QSS_DataArrays record.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include "qss_data_arrays.h"
+#include "qss_grid.h"
+#include "qss_options.h"
+#include "lsm_kernels.h"
+
+namespace lsm {
+
+// Reference-behaviour switches (all on by default: results match the reference).
+enum Quirk : unsigned {
+  QUIRK_ENO3_READ = 1u,    // ENO3 x line reads phi(i+3,j,j)
+  QUIRK_WENO5_SIGN = 2u,   // WENO5 normal term enters with + sign
+  QUIRK_STALE_NORM = 4u,   // |vel_n| < 1e-5 reuses the previous voxel's normal term
+};
+
+struct Timers {
+  double stage_ms[4] = {0, 0, 0, 0};  // summed stage-kernel time per stage (1..3), [0] unused
+  int64_t stage_launches[4] = {0, 0, 0, 0};
+  double inner_ms = 0;                 // wall time of inner steps (event to event)
+  int64_t inner_steps = 0;
+  int64_t outer_steps = 0;
+};
+
+struct Session {
+  const QSS_DataArrays* key = nullptr;
+  Box bx{};
+  int64_t n = 0;
+  hipStream_t st = nullptr;
+  // device fields
+  real *phi = nullptr, *A = nullptr, *B = nullptr, *S = nullptr, *D = nullptr, *phi_prev = nullptr;
+  real *mask = nullptr, *mask_w = nullptr, *mask_nw = nullptr, *phi_extra = nullptr;
+  real *vn = nullptr, *b = nullptr, *vx = nullptr, *vy = nullptr, *vz = nullptr;
+  real *gx = nullptr, *gy = nullptr, *gz = nullptr;
+  int* labels = nullptr;
+  uint8_t* bin = nullptr;
+  double *part = nullptr, *red = nullptr;
+  real* dt = nullptr;
+  int* count = nullptr;
+  int64_t *stale_idx = nullptr, *stale_src = nullptr, *stale_list = nullptr;
+  int stale_cap = 0, nstale = 0;
+  // pinned host staging
+  real* h_dt = nullptr;
+  double* h_red = nullptr;
+  int* h_count = nullptr;
+  hipEvent_t ev_dt = nullptr, ev_a = nullptr, ev_b = nullptr;
+  std::vector<hipEvent_t> stage_ev;  // timing pairs when profiling is on
+  bool profile = false;
+  Timers tm;
+};
+
+// error state of the calling thread
+void set_error(const std::string& msg);
+const char* last_error();
+
+Session* session_for(const QSS_DataArrays* p, const Grid* g);  // creates or resizes
+void session_release(const QSS_DataArrays* p);
+Box box_of(const Grid* g);
+
+unsigned quirks();
+void set_quirks(unsigned q);
+void set_slabs(int s);
+int dt_slabs(const Grid* g);
+
+void ensure_var(Session* s);     // vn, b, vx, vy, vz
+void ensure_grad(Session* s);    // mask gradient and reservoir box
+void ensure_labels(Session* s);  // component labels and binary field
+
+}  // namespace lsm

@@ -1,0 +1,359 @@
+// lsm_stage.hip -- the hot kernels of one level-set time step on gfx950:
+// the fused right-hand side + TVD Runge-Kutta stage, the time-step probe and
+// the composed signed-linear-extrapolation boundary condition.
+//
+// Stage kernel layout: a 64 x 8 thread block owns a 64 x 8 column of fill
+// voxels and marches it along z.  Three z-planes (k-1, k, k+1) of the input
+// field, each with a 3-voxel halo in x and y, live in LDS; the z line of
+// every voxel (k-3 .. k+3) lives in registers.  Each input plane is read from
+// HBM once per block column with x-fastest, 64-lane coalesced loads.  The
+// right-hand side never touches HBM: it is combined with u^n and dt into the
+// stage result in registers, and the last stage also applies the mask
+// imposition, so a full step writes only the stage buffers and phi.
+#include <algorithm>
+#include "lsm_kernels.h"
+
+namespace lsm {
+
+constexpr int TX = 64;
+constexpr int TY = 8;
+constexpr int PX = TX + 6;
+constexpr int PY = TY + 6;
+
+// phi(i+3,j,j): the reference's ENO3 x line uses the k index j
+// (qss_spatial_derivatives3d.f:677,1454); outside the array it reads 0.
+__device__ __forceinline__ real eno3_quirk_read(const real* f, const Box& bx, int i, int j) {
+  int64_t a = (int64_t)(i + 3) + (int64_t)bx.nx * ((int64_t)j + (int64_t)bx.ny * j);
+  return (a >= 0 && a < bx.size()) ? f[a] : real(0);
+}
+
+// The six one-sided derivatives at (i,j,k) read straight from global memory
+// (used off the fast path: the time-step probe and stale normal terms).
+template <int S>
+__device__ void derivs_global(const real* f, const Box& bx, int i, int j, int k, real dx, real dy,
+                              real dz, int quirk, real* d) {
+  real lx[7], ly[7], lz[7];
+#pragma unroll
+  for (int o = -3; o <= 3; ++o) {
+    bool need = !(S == ENO2 && (o == -3 || o == 3));
+    lx[o + 3] = need ? f[bx.at(i + o, j, k)] : real(0);
+    ly[o + 3] = need ? f[bx.at(i, j + o, k)] : real(0);
+    lz[o + 3] = need ? f[bx.at(i, j, k + o)] : real(0);
+  }
+  if (S == ENO3 && quirk) lx[6] = eno3_quirk_read(f, bx, i, j);
+  real inv_dx = real(1.0) / dx, inv_dy = real(1.0) / dy, inv_dz = real(1.0) / dz;
+  axis_derivs<real, S>(lx, inv_dx, inv_dx, d[0], d[1]);
+  axis_derivs<real, S>(ly, inv_dy, inv_dx, d[2], d[3]);
+  axis_derivs<real, S>(lz, inv_dz, inv_dx, d[4], d[5]);
+}
+
+// Normal term a voxel with |vel_n| < 1e-5 inherits: the reference leaves
+// temp_norm unassigned there and so reuses the value of the last voxel of the
+// same slab, in loop order, that did assign it
+// (qss_spatial_derivatives3d.f:459-486).
+template <int S>
+__device__ real stale_norm(const StageArgs& a, int64_t idx) {
+  int lo = 0, hi = a.nstale - 1;
+  while (lo <= hi) {
+    int mid = (lo + hi) >> 1;
+    int64_t v = a.stale_idx[mid];
+    if (v == idx) {
+      int64_t src = a.stale_src[mid];
+      if (src < 0) return real(0);
+      int k = (int)(src / ((int64_t)a.bx.nx * a.bx.ny));
+      int64_t r = src - (int64_t)k * a.bx.nx * a.bx.ny;
+      int j = (int)(r / a.bx.nx), i = (int)(r - (int64_t)j * a.bx.nx);
+      real d[6];
+      derivs_global<S>(a.in, a.bx, i, j, k, a.dx, a.dy, a.dz, a.eno3_quirk, d);
+      real vs = a.vn[src];
+      return a.norm_sign * vs * sqrt(godunov_sq(vs, d));
+    }
+    if (v < idx) lo = mid + 1; else hi = mid - 1;
+  }
+  return real(0);
+}
+
+template <int S, bool VAR, int STAGE, bool FINAL>
+__global__ __launch_bounds__(TX * TY) void stage_kernel(StageArgs a) {
+  __shared__ real pl[3][PY][PX];
+  const Box bx = a.bx;
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
+  const int i0 = bx.ilo + blockIdx.x * TX, j0 = bx.jlo + blockIdx.y * TY;
+  const int i = i0 + tx, j = j0 + ty;
+  const int zc = (bx.khi - bx.klo + gridDim.z) / gridDim.z;
+  const int k0 = bx.klo + blockIdx.z * zc;
+  const int k1 = min(k0 + zc - 1, bx.khi);
+  if (k0 > k1) return;
+  const bool active = (i <= bx.ihi) && (j <= bx.jhi);
+  const real* __restrict__ in = a.in;
+
+  auto load_plane = [&](int slot, int k) {
+    for (int e = tid; e < PX * PY; e += TX * TY) {
+      int r = e / PX, c = e - r * PX;
+      int gi = i0 - 3 + c, gj = j0 - 3 + r;
+      real v = real(0);
+      if (gi >= 0 && gi < bx.nx && gj >= 0 && gj < bx.ny) v = in[bx.at(gi, gj, k)];
+      pl[slot][r][c] = v;
+    }
+  };
+
+  const real inv_dx = real(1.0) / a.dx, inv_dy = real(1.0) / a.dy, inv_dz = real(1.0) / a.dz;
+  const real fx = real(0.5) / a.dx, fy = real(0.5) / a.dy, fz = real(0.5) / a.dz;
+  const real dt = *a.dt;
+
+  real w[7];
+  load_plane(0, k0 - 1);
+  load_plane(1, k0);
+  if (active) {
+#pragma unroll
+    for (int o = 0; o < 6; ++o) w[o] = in[bx.at(i, j, k0 - 3 + o)];
+  }
+  int sm = 0, sc = 1, sp = 2;
+  for (int k = k0; k <= k1; ++k) {
+    __syncthreads();
+    load_plane(sp, k + 1);
+    if (active) w[6] = in[bx.at(i, j, k + 3)];
+    __syncthreads();
+    if (active) {
+      const int64_t idx = bx.at(i, j, k);
+      real lx[7], ly[7];
+#pragma unroll
+      for (int o = 0; o < 7; ++o) {
+        lx[o] = pl[sc][ty + 3][tx + o];
+        ly[o] = pl[sc][ty + o][tx + 3];
+      }
+      if (S == ENO3 && a.eno3_quirk) lx[6] = eno3_quirk_read(in, bx, i, j);
+      real d[6];
+      axis_derivs<real, S>(lx, inv_dx, inv_dx, d[0], d[1]);
+      axis_derivs<real, S>(ly, inv_dy, inv_dx, d[2], d[3]);
+      axis_derivs<real, S>(w, inv_dz, inv_dx, d[4], d[5]);
+
+      const real vn = VAR ? a.vn[idx] : a.a_const;
+      real temp_norm;
+      if (fabs(vn) >= real(1.e-5)) {
+        temp_norm = a.norm_sign * vn * sqrt(godunov_sq(vn, d));
+      } else {
+        temp_norm = (VAR && a.nstale > 0) ? stale_norm<S>(a, idx) : real(0);
+      }
+      real xy[4] = {pl[sc][ty + 4][tx + 4], pl[sc][ty + 2][tx + 2], pl[sc][ty + 4][tx + 2],
+                    pl[sc][ty + 2][tx + 4]};
+      real xz[4] = {pl[sp][ty + 3][tx + 4], pl[sm][ty + 3][tx + 2], pl[sp][ty + 3][tx + 2],
+                    pl[sm][ty + 3][tx + 4]};
+      real yz[4] = {pl[sp][ty + 4][tx + 3], pl[sm][ty + 2][tx + 3], pl[sp][ty + 2][tx + 3],
+                    pl[sm][ty + 4][tx + 3]};
+      real curv = curvature(lx[3], lx[1], lx[2], lx[4], lx[5], ly[1], ly[2], ly[4], ly[5], w[1],
+                            w[2], w[4], w[5], xy, xz, yz, fx, fy, fz);
+      real rhs;
+      if (VAR) {
+        real vxv = a.vx[idx], vyv = a.vy[idx], vzv = a.vz[idx];
+        real ax = vxv >= real(0) ? d[1] : d[0];
+        real ay = vyv >= real(0) ? d[3] : d[2];
+        real az = vzv >= real(0) ? d[5] : d[4];
+        real temp_vel = -(vxv * ax + vyv * ay + vzv * az);
+        rhs = temp_norm + a.b[idx] * curv + temp_vel;
+      } else {
+        rhs = temp_norm + a.b_const * curv;
+      }
+      real val;
+      if (STAGE == 1) {
+        val = lx[3] + dt * rhs;
+      } else if (STAGE == 2) {
+        val = real(0.75) * a.un[idx] + real(0.25) * (lx[3] + dt * rhs);
+      } else {
+        val = real(1.0 / 3.0) * a.un[idx] + real(2.0 / 3.0) * (lx[3] + dt * rhs);
+      }
+      if (FINAL) {
+        bool face = (i <= bx.ilo + 1) || (i >= bx.ihi - 1) || (j <= bx.jlo + 1) ||
+                    (j >= bx.jhi - 1) || (k <= bx.klo + 1) || (k >= bx.khi - 1);
+        if (face) a.pre[idx] = val;
+        real m = a.mask[idx] - a.overlap;
+        val = (m > val) ? m : val;
+        if (a.conn) {
+          real mw = a.mask_w[idx];
+          val = (mw > val) ? mw : val;
+          real mn = a.mask_nw[idx];
+          val = (mn > val) ? mn : val;
+        }
+      }
+      a.out[idx] = val;
+    }
+#pragma unroll
+    for (int o = 0; o < 6; ++o) w[o] = w[o + 1];
+    int t = sm; sm = sc; sc = sp; sp = t;
+  }
+}
+
+template <int S, bool VAR, int STAGE>
+static hipError_t launch_sf(const StageArgs& a, bool fin, dim3 g, dim3 b, hipStream_t st) {
+  if (fin) hipLaunchKernelGGL((stage_kernel<S, VAR, STAGE, true>), g, b, 0, st, a);
+  else hipLaunchKernelGGL((stage_kernel<S, VAR, STAGE, false>), g, b, 0, st, a);
+  return hipGetLastError();
+}
+
+template <int S, bool VAR>
+static hipError_t launch_s(const StageArgs& a, int stage, bool fin, dim3 g, dim3 b, hipStream_t st) {
+  switch (stage) {
+  case 1: return launch_sf<S, VAR, 1>(a, fin, g, b, st);
+  case 2: return launch_sf<S, VAR, 2>(a, fin, g, b, st);
+  default: return launch_sf<S, VAR, 3>(a, fin, g, b, st);
+  }
+}
+
+hipError_t launch_stage(const StageArgs& a, int scheme, bool var, int stage, bool fin,
+                        hipStream_t st) {
+  const Box& bx = a.bx;
+  int nxf = bx.ihi - bx.ilo + 1, nyf = bx.jhi - bx.jlo + 1, nzf = bx.khi - bx.klo + 1;
+  int gx = (nxf + TX - 1) / TX, gy = (nyf + TY - 1) / TY;
+  // enough z-chunks to give every CU several blocks, each chunk >= 8 planes
+  int cols = gx * gy;
+  int gz = (2048 + cols - 1) / cols;
+  gz = max(1, min(gz, (nzf + 7) / 8));
+  dim3 grid(gx, gy, gz), block(TX, TY, 1);
+  switch (scheme) {
+  case 2: return var ? launch_s<ENO2, true>(a, stage, fin, grid, block, st)
+                     : launch_s<ENO2, false>(a, stage, fin, grid, block, st);
+  case 3: return var ? launch_s<ENO3, true>(a, stage, fin, grid, block, st)
+                     : launch_s<ENO3, false>(a, stage, fin, grid, block, st);
+  case 5: return var ? launch_s<WENO5, true>(a, stage, fin, grid, block, st)
+                     : launch_s<WENO5, false>(a, stage, fin, grid, block, st);
+  default: return hipErrorInvalidValue;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Time-step probe: H/dx at (ihi_fb, jhi_fb, last plane of each slab), the
+// voxels whose value the reference's per-thread max_H_over_dX ends up holding
+// (qss_spatial_derivatives3d.f:490; constCurvModel3d.c:105-108,139-149).
+template <int S>
+__global__ void probe_kernel(ProbeArgs a) {
+  __shared__ real hs[1024];
+  const Box bx = a.bx;
+  int t = threadIdx.x;
+  real h = real(-1);
+  int nsl = bx.khi - bx.klo + 1;
+  if (t < a.nslabs) {
+    int klo = bx.klo + (int)((int64_t)nsl * t / a.nslabs);
+    int khi = bx.klo + (int)((int64_t)nsl * (t + 1) / a.nslabs) - 1;
+    if (khi > bx.khi) khi = bx.khi;
+    if (klo <= khi) {
+      real d[6];
+      derivs_global<S>(a.phi, bx, bx.ihi, bx.jhi, khi, a.dx, a.dy, a.dz, a.eno3_quirk, d);
+      real vn = a.vn ? a.vn[bx.at(bx.ihi, bx.jhi, khi)] : a.a_const;
+      h = h_over_dx(vn, d, a.dx, a.dy, a.dz);
+    }
+  }
+  hs[t] = h;
+  __syncthreads();
+  if (t == 0) {
+    real cur = real(-1);
+    for (int s = 0; s < a.nslabs; ++s)
+      if (hs[s] > cur) cur = hs[s];
+    *a.dt_out = real(0.5) / (cur + a.bmax_over_dx + a.umax_over_dx);
+  }
+}
+
+hipError_t launch_probe(const ProbeArgs& a, int scheme, hipStream_t st) {
+  if (a.nslabs < 1 || a.nslabs > 1024) return hipErrorInvalidValue;
+  int nt = ((a.nslabs + 63) / 64) * 64;
+  switch (scheme) {
+  case 2: hipLaunchKernelGGL(probe_kernel<ENO2>, dim3(1), dim3(nt), 0, st, a); break;
+  case 3: hipLaunchKernelGGL(probe_kernel<ENO3>, dim3(1), dim3(nt), 0, st, a); break;
+  case 5: hipLaunchKernelGGL(probe_kernel<WENO5>, dim3(1), dim3(nt), 0, st, a); break;
+  default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Boundary condition.  The reference extrapolates face by face (x-, x+, y-,
+// y+, z-, z+), each pass covering the whole extent of the other two axes
+// (qss_spatial_derivatives3d.f:3055-3191, qss_general_util.c:630-698).  The
+// value a ghost voxel ends with therefore depends only on fill-box values:
+// z ghosts extrapolate the (x,y)-completed planes klo/klo+1, y ghosts the
+// x-completed rows jlo/jlo+1, x ghosts the fill values ilo/ilo+1.  Each
+// ghost voxel evaluates that composition directly, so one launch with no
+// ordering replaces the six passes.
+__device__ __forceinline__ real extrap(real c, real c2, int dist) {
+  real s = copysign(real(1), c);
+  real slope = s * fabs(c - c2);
+  return c + slope * real(dist);
+}
+
+__device__ __forceinline__ real bc_x(const real* f, const Box& bx, int i, int j, int k) {
+  if (i < bx.ilo) return extrap(f[bx.at(bx.ilo, j, k)], f[bx.at(bx.ilo + 1, j, k)], bx.ilo - i);
+  if (i > bx.ihi) return extrap(f[bx.at(bx.ihi, j, k)], f[bx.at(bx.ihi - 1, j, k)], i - bx.ihi);
+  return f[bx.at(i, j, k)];
+}
+__device__ __forceinline__ real bc_y(const real* f, const Box& bx, int i, int j, int k) {
+  if (j < bx.jlo) return extrap(bc_x(f, bx, i, bx.jlo, k), bc_x(f, bx, i, bx.jlo + 1, k), bx.jlo - j);
+  if (j > bx.jhi) return extrap(bc_x(f, bx, i, bx.jhi, k), bc_x(f, bx, i, bx.jhi - 1, k), j - bx.jhi);
+  return bc_x(f, bx, i, j, k);
+}
+__device__ __forceinline__ real bc_z(const real* f, const Box& bx, int i, int j, int k) {
+  if (k < bx.klo) return extrap(bc_y(f, bx, i, j, bx.klo), bc_y(f, bx, i, j, bx.klo + 1), bx.klo - k);
+  if (k > bx.khi) return extrap(bc_y(f, bx, i, j, bx.khi), bc_y(f, bx, i, j, bx.khi - 1), k - bx.khi);
+  return bc_y(f, bx, i, j, k);
+}
+
+template <bool FINAL>
+__global__ void bc_kernel(BcArgs a, int64_t nA, int64_t nB, int64_t nC) {
+  const Box bx = a.bx;
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  int nxf = bx.ihi - bx.ilo + 1, nyf = bx.jhi - bx.jlo + 1;
+  int gxn = bx.nx - nxf, gyn = bx.ny - nyf;
+  for (; g < nA + nB + nC; g += stride) {
+    int i, j, k;
+    if (g < nA) {
+      int64_t plane = (int64_t)bx.nx * bx.ny;
+      int kk = (int)(g / plane);
+      int64_t r = g - kk * plane;
+      j = (int)(r / bx.nx);
+      i = (int)(r - (int64_t)j * bx.nx);
+      k = kk < bx.klo ? kk : bx.khi + 1 + (kk - bx.klo);
+    } else if (g < nA + nB) {
+      int64_t h = g - nA;
+      i = (int)(h % bx.nx);
+      int64_t t = h / bx.nx;
+      int jj = (int)(t % gyn);
+      k = bx.klo + (int)(t / gyn);
+      j = jj < bx.jlo ? jj : bx.jhi + 1 + (jj - bx.jlo);
+    } else {
+      int64_t h = g - nA - nB;
+      int ii = (int)(h % gxn);
+      int64_t t = h / gxn;
+      j = bx.jlo + (int)(t % nyf);
+      k = bx.klo + (int)(t / nyf);
+      i = ii < bx.ilo ? ii : bx.ihi + 1 + (ii - bx.ilo);
+    }
+    real v = bc_z(a.src, bx, i, j, k);
+    int64_t idx = bx.at(i, j, k);
+    if (FINAL) {
+      real m = a.mask[idx] - a.overlap;
+      v = (m > v) ? m : v;
+      if (a.conn) {
+        real mw = a.mask_w[idx];
+        v = (mw > v) ? mw : v;
+        real mn = a.mask_nw[idx];
+        v = (mn > v) ? mn : v;
+      }
+    }
+    a.dst[idx] = v;
+  }
+}
+
+hipError_t launch_bc(const BcArgs& a, bool final_mask, hipStream_t st) {
+  const Box& bx = a.bx;
+  int nxf = bx.ihi - bx.ilo + 1, nyf = bx.jhi - bx.jlo + 1, nzf = bx.khi - bx.klo + 1;
+  int64_t nA = (int64_t)(bx.nz - nzf) * bx.nx * bx.ny;
+  int64_t nB = (int64_t)nzf * (bx.ny - nyf) * bx.nx;
+  int64_t nC = (int64_t)nzf * nyf * (bx.nx - nxf);
+  int64_t tot = nA + nB + nC;
+  int blocks = (int)std::min<int64_t>((tot + 255) / 256, 8192);
+  if (blocks < 1) blocks = 1;
+  if (final_mask) hipLaunchKernelGGL(bc_kernel<true>, dim3(blocks), dim3(256), 0, st, a, nA, nB, nC);
+  else hipLaunchKernelGGL(bc_kernel<false>, dim3(blocks), dim3(256), 0, st, a, nA, nB, nC);
+  return hipGetLastError();
+}
+
+}  // namespace lsm

@@ -1,0 +1,519 @@
+// lsm_util.hip -- per-outer-step kernels: elementwise mask algebra, FP64
+// reductions (saturation volume, steady-state measure, time-step bounds),
+// subcell-fix reinitialization, the variable-coefficient fields and the
+// analytic initialisers.  All are HBM-streaming kernels: grid-stride loops,
+// one voxel per lane per iteration, x-fastest so every wave reads 256/512
+// contiguous bytes.
+#include <float.h>
+#include <algorithm>
+#include "lsm_kernels.h"
+
+namespace lsm {
+
+static int stream_blocks(int64_t n) {
+  int64_t b = (n + 255) / 256;
+  if (b > 16384) b = 16384;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+// ---------------------------------------------------------------------------
+// Elementwise mask algebra (reference: src/utilities/qss_macros.h:79,192,239;
+// src/utilities/qss_util3d.f:130-175)
+
+__global__ void impose_kernel(real* dst, const real* m, const real* src, real ov, int64_t n) {
+  for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n;
+       l += (int64_t)gridDim.x * blockDim.x) {
+    real v = m[l] - ov, s = src[l];
+    dst[l] = (v > s) ? v : s;
+  }
+}
+hipError_t launch_impose(real* dst, const real* m, const real* src, real ov, int64_t n,
+                         hipStream_t st) {
+  hipLaunchKernelGGL(impose_kernel, dim3(stream_blocks(n)), dim3(256), 0, st, dst, m, src, ov, n);
+  return hipGetLastError();
+}
+
+__global__ void merge_kernel(real* phi, const real* m, int64_t n) {
+  for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n;
+       l += (int64_t)gridDim.x * blockDim.x) {
+    real v = m[l];
+    if (v > real(0)) phi[l] = -v;
+  }
+}
+hipError_t launch_merge(real* phi, const real* m, int64_t n, hipStream_t st) {
+  hipLaunchKernelGGL(merge_kernel, dim3(stream_blocks(n)), dim3(256), 0, st, phi, m, n);
+  return hipGetLastError();
+}
+
+__global__ void min_kernel(real* phi, const real* e, int64_t n) {
+  for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n;
+       l += (int64_t)gridDim.x * blockDim.x) {
+    real a = phi[l], b = e[l];
+    phi[l] = (a < b) ? a : b;
+  }
+}
+hipError_t launch_min(real* phi, const real* e, int64_t n, hipStream_t st) {
+  hipLaunchKernelGGL(min_kernel, dim3(stream_blocks(n)), dim3(256), 0, st, phi, e, n);
+  return hipGetLastError();
+}
+
+__global__ void fill_kernel(real* dst, real v, int64_t n) {
+  for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n;
+       l += (int64_t)gridDim.x * blockDim.x)
+    dst[l] = v;
+}
+hipError_t launch_fill(real* dst, real v, int64_t n, hipStream_t st) {
+  hipLaunchKernelGGL(fill_kernel, dim3(stream_blocks(n)), dim3(256), 0, st, dst, v, n);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Fill-box iteration helper: flat fill index -> (i,j,k)
+struct FillIter {
+  Box bx;
+  int nxf, nyf;
+  int64_t n;
+  __host__ __device__ FillIter(const Box& b) : bx(b) {
+    nxf = b.ihi - b.ilo + 1;
+    nyf = b.jhi - b.jlo + 1;
+    n = (int64_t)nxf * nyf * (b.khi - b.klo + 1);
+  }
+  __device__ void ijk(int64_t f, int& i, int& j, int& k) const {
+    int64_t t = f / nxf;
+    i = bx.ilo + (int)(f - t * nxf);
+    j = bx.jlo + (int)(t % nyf);
+    k = bx.klo + (int)(t / nyf);
+  }
+};
+
+constexpr int RED_BLOCKS = 2048;
+int reduce_partials() { return 2 * RED_BLOCKS; }
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  return v;
+}
+
+// Smoothed volume of {f1 < 0} (reference: src/utilities/qss_util3d.f:288-358)
+// accumulated in FP64, and the local max-norm difference
+// (reference: src/utilities/qss_util3d.f:65-125).
+__global__ __launch_bounds__(256) void outer_reduce_kernel(Box bx, const real* f1, const real* f2,
+                                                           real zone, real eps, real dV,
+                                                           double* part) {
+  const real pi = real(3.14159265358979323846), one_over_pi = real(0.31830988618379);
+  FillIter it(bx);
+  double vol = 0.0;
+  real mx = real(-1);
+  for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < it.n;
+       f += (int64_t)gridDim.x * blockDim.x) {
+    int i, j, k;
+    it.ijk(f, i, j, k);
+    int64_t idx = bx.at(i, j, k);
+    real p = f1[idx];
+    real q = p / eps;
+    if (p < -eps) vol += (double)dV;
+    else if (p < eps) vol += (double)(real(0.5) * (real(1) - q - one_over_pi * sin(pi * q)) * dV);
+    if (f2 && fabs(p) < zone) {
+      real d = fabs(p - f2[idx]);
+      if (d > mx) mx = d;
+    }
+  }
+  __shared__ double s_sum[4], s_max[4];
+  double ws = wave_sum(vol), wm = wave_max((double)mx);
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) { s_sum[wid] = ws; s_max[wid] = wm; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0, m = -1.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) { a += s_sum[w]; m = fmax(m, s_max[w]); }
+    part[blockIdx.x] = a;
+    part[RED_BLOCKS + blockIdx.x] = m;
+  }
+}
+
+__global__ __launch_bounds__(256) void finish_reduce_kernel(const double* part, int nb, double* out,
+                                                            int mode) {
+  // mode 0: out[0] = sum(part[0..nb)), out[1] = max(part[RB..RB+nb))
+  // mode 1: out[0] = max(part[0..nb)), out[1] = max(part[RB..RB+nb))
+  double a = mode == 0 ? 0.0 : -DBL_MAX, m = -DBL_MAX;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    a = mode == 0 ? a + part[b] : fmax(a, part[b]);
+    m = fmax(m, part[RED_BLOCKS + b]);
+  }
+  a = mode == 0 ? wave_sum(a) : wave_max(a);
+  m = wave_max(m);
+  __shared__ double sa[4], sm[4];
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) { sa[wid] = a; sm[wid] = m; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double x = sa[0], y = sm[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+      x = mode == 0 ? x + sa[w] : fmax(x, sa[w]);
+      y = fmax(y, sm[w]);
+    }
+    out[0] = x;
+    out[1] = y;
+  }
+}
+
+hipError_t launch_outer_reduce(const Box& bx, const real* f1, const real* f2, real zone, real eps,
+                               real dV, double* part, double* out, hipStream_t st) {
+  FillIter it(bx);
+  int nb = (int)std::min<int64_t>((it.n + 255) / 256, RED_BLOCKS);
+  if (nb < 1) nb = 1;
+  hipLaunchKernelGGL(outer_reduce_kernel, dim3(nb), dim3(256), 0, st, bx, f1, f2, zone, eps, dV,
+                     part);
+  hipLaunchKernelGGL(finish_reduce_kernel, dim3(1), dim3(256), 0, st, part, nb, out, 0);
+  return hipGetLastError();
+}
+
+// b_max and U_max of setVarCurvAdv3d (qss_spatial_derivatives3d.f:261-296)
+__global__ __launch_bounds__(256) void coeff_reduce_kernel(Box bx, const real* b, const real* vx,
+                                                           const real* vy, const real* vz,
+                                                           real inv, double* part) {
+  FillIter it(bx);
+  real bm = real(-1), um = real(-1);
+  for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < it.n;
+       f += (int64_t)gridDim.x * blockDim.x) {
+    int i, j, k;
+    it.ijk(f, i, j, k);
+    int64_t idx = bx.at(i, j, k);
+    real bv = b[idx];
+    if (bv > bm) bm = bv;
+    real u = fabs(vx[idx]) * inv + fabs(vy[idx]) * inv + fabs(vz[idx]) * inv;
+    if (u > um) um = u;
+  }
+  __shared__ double s0[4], s1[4];
+  double a = wave_max((double)bm), m = wave_max((double)um);
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) { s0[wid] = a; s1[wid] = m; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double x = s0[0], y = s1[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) { x = fmax(x, s0[w]); y = fmax(y, s1[w]); }
+    part[blockIdx.x] = x;
+    part[RED_BLOCKS + blockIdx.x] = y;
+  }
+}
+
+hipError_t launch_coeff_reduce(const Box& bx, const real* b, const real* vx, const real* vy,
+                               const real* vz, real inv, double* part, double* out,
+                               hipStream_t st) {
+  FillIter it(bx);
+  int nb = (int)std::min<int64_t>((it.n + 255) / 256, RED_BLOCKS);
+  if (nb < 1) nb = 1;
+  hipLaunchKernelGGL(coeff_reduce_kernel, dim3(nb), dim3(256), 0, st, bx, b, vx, vy, vz, inv, part);
+  hipLaunchKernelGGL(finish_reduce_kernel, dim3(1), dim3(256), 0, st, part, nb, out, 1);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Subcell-fix reinitialization (reference:
+// src/reinitialization/qss_reinitialization3d.f:476-746)
+
+__device__ __forceinline__ bool sign_change(real a, real c) {
+  return (a > real(0) && c <= real(0)) || (a <= real(0) && c > real(0));
+}
+
+__global__ void reinit_distance_kernel(Box bx, real* dist0, const real* phi0, real dx, real dy,
+                                       real dz) {
+  FillIter it(bx);
+  real flag = real(-1000.0) * tmax(tmax(dx, dy), dz);
+  for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < it.n;
+       f += (int64_t)gridDim.x * blockDim.x) {
+    int i, j, k;
+    it.ijk(f, i, j, k);
+    int64_t idx = bx.at(i, j, k);
+    int64_t sy = bx.nx, sz = (int64_t)bx.nx * bx.ny;
+    real c = phi0[idx];
+    real xp = phi0[idx + 1], xm = phi0[idx - 1];
+    real yp = phi0[idx + sy], ym = phi0[idx - sy];
+    real zp = phi0[idx + sz], zm = phi0[idx - sz];
+    bool near = sign_change(xp, c) || sign_change(xm, c) || sign_change(yp, c) ||
+                sign_change(ym, c) || sign_change(zp, c) || sign_change(zm, c);
+    if (!near) {
+      dist0[idx] = flag;
+    } else {
+      real delta = tmax(tmax(tmax(fabs(xp - xm) / real(2), fabs(xp - c)), fabs(c - xm)), real(1.e-5));
+      delta = tmax(tmax(tmax(fabs(yp - ym) / real(2), fabs(yp - c)), fabs(c - ym)), delta);
+      delta = tmax(tmax(tmax(fabs(zp - zm) / real(2), fabs(zp - c)), fabs(c - zm)), delta);
+      dist0[idx] = c / delta;
+    }
+  }
+}
+hipError_t launch_reinit_distance(const Box& bx, real* dist0, const real* phi0, real dx, real dy,
+                                  real dz, hipStream_t st) {
+  FillIter it(bx);
+  hipLaunchKernelGGL(reinit_distance_kernel, dim3(stream_blocks(it.n)), dim3(256), 0, st, bx,
+                     dist0, phi0, dx, dy, dz);
+  return hipGetLastError();
+}
+
+// one forward-Euler step of the reinitialization equation (RK1 fused with
+// the first-order subcell-fix right-hand side)
+__global__ void reinit_step_kernel(Box bx, real* next, const real* data, const real* phi0,
+                                   const real* dist0, real dt_r, real dx, real dy, real dz) {
+  FillIter it(bx);
+  real max_dx = tmax(tmax(dx, dy), dz);
+  real flag = real(-1000.0) * max_dx;
+  real inv_dx = real(1.0) / dx, inv_dy = real(1.0) / dy, inv_dz = real(1.0) / dz;
+  int64_t sy = bx.nx, sz = (int64_t)bx.nx * bx.ny;
+  for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < it.n;
+       f += (int64_t)gridDim.x * blockDim.x) {
+    int i, j, k;
+    it.ijk(f, i, j, k);
+    int64_t idx = bx.at(i, j, k);
+    real c = data[idx];
+    real gp[3], gm[3];
+    gp[0] = (data[idx + 1] - c) * inv_dx;
+    gm[0] = (c - data[idx - 1]) * inv_dx;
+    gp[1] = (data[idx + sy] - c) * inv_dy;
+    gm[1] = (c - data[idx - sy]) * inv_dy;
+    gp[2] = (data[idx + sz] - c) * inv_dz;
+    gm[2] = (c - data[idx - sz]) * inv_dz;
+    real p0 = phi0[idx];
+    real sgn0 = fabs(p0) > real(1.e-5) ? copysign(real(1), p0) : real(0);
+    real d0 = dist0[idx];
+    real rhs;
+    if (d0 == flag) {
+      real st[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        real a, b;
+        if (c > real(0)) { a = tmax(-gp[d], real(0)); b = tmax(gm[d], real(0)); }
+        else { a = tmax(gp[d], real(0)); b = tmax(-gm[d], real(0)); }
+        st[d] = tmax(a, b);
+      }
+      if (fabs(c) > real(1.e-5)) {
+        real n2 = st[0] * st[0] + st[1] * st[1] + st[2] * st[2];
+        rhs = sgn0 - sgn0 * sqrt(n2);
+      } else {
+        rhs = real(0);
+      }
+    } else {
+      rhs = d0 - sgn0 * fabs(c) / max_dx;
+    }
+    next[idx] = c + dt_r * rhs;
+  }
+}
+hipError_t launch_reinit_step(const Box& bx, real* next, const real* data, const real* phi0,
+                              const real* dist0, real dt_r, real dx, real dy, real dz,
+                              hipStream_t st) {
+  FillIter it(bx);
+  hipLaunchKernelGGL(reinit_step_kernel, dim3(stream_blocks(it.n)), dim3(256), 0, st, bx, next,
+                     data, phi0, dist0, dt_r, dx, dy, dz);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Coefficient fields (reference: qss_spatial_derivatives3d.f:13-87,178-298,2987-3048)
+
+__global__ void central_grad_kernel(Box bx, real* gx, real* gy, real* gz, const real* f, real fx,
+                                    real fy, real fz) {
+  FillIter it(bx);
+  int64_t sy = bx.nx, sz = (int64_t)bx.nx * bx.ny;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < it.n;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    int i, j, k;
+    it.ijk(q, i, j, k);
+    int64_t idx = bx.at(i, j, k);
+    gx[idx] = (f[idx + 1] - f[idx - 1]) * fx;
+    gy[idx] = (f[idx + sy] - f[idx - sy]) * fy;
+    gz[idx] = (f[idx + sz] - f[idx - sz]) * fz;
+  }
+}
+hipError_t launch_central_grad(const Box& bx, real* gx, real* gy, real* gz, const real* f,
+                               real dx, real dy, real dz, hipStream_t st) {
+  FillIter it(bx);
+  hipLaunchKernelGGL(central_grad_kernel, dim3(stream_blocks(it.n)), dim3(256), 0, st, bx, gx, gy,
+                     gz, f, real(0.5) / dx, real(0.5) / dy, real(0.5) / dz);
+  return hipGetLastError();
+}
+
+__device__ __forceinline__ void coeff_common(real m, real mx, real my, real mz, real dx, real& h1,
+                                             real& h2, real& ag, real& s) {
+  real eps_var = real(1.5) * dx;
+  h1 = heaviside(-m, eps_var);
+  h2 = heaviside(m, eps_var);
+  ag = mx * mx + my * my + mz * mz;
+  ag = sqrt(ag);
+  real x = m * m + ag * dx * ag * dx;
+  x = sqrt(x);
+  s = m / x;
+}
+
+__global__ void var_norm_kernel(Box bx, real* vn, const real* mask, const real* gx, const real* gy,
+                                const real* gz, real a0, real cos_theta, real dx) {
+  FillIter it(bx);
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < it.n;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    int i, j, k;
+    it.ijk(q, i, j, k);
+    int64_t idx = bx.at(i, j, k);
+    real h1, h2, ag, s;
+    coeff_common(mask[idx], gx[idx], gy[idx], gz[idx], dx, h1, h2, ag, s);
+    vn[idx] = h1 * a0 - s * h2 * real(1) * cos_theta * ag;
+  }
+}
+hipError_t launch_var_norm(const Box& bx, real* vn, const real* mask, const real* gx,
+                           const real* gy, const real* gz, real a0, real cos_theta, real dx,
+                           hipStream_t st) {
+  FillIter it(bx);
+  hipLaunchKernelGGL(var_norm_kernel, dim3(stream_blocks(it.n)), dim3(256), 0, st, bx, vn, mask,
+                     gx, gy, gz, a0, cos_theta, dx);
+  return hipGetLastError();
+}
+
+__global__ void var_curv_adv_kernel(Box bx, real* b, real* vx, real* vy, real* vz,
+                                    const real* mask, const real* gx, const real* gy,
+                                    const real* gz, real const_b, real dx) {
+  FillIter it(bx);
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < it.n;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    int i, j, k;
+    it.ijk(q, i, j, k);
+    int64_t idx = bx.at(i, j, k);
+    real h1, h2, ag, s;
+    real mxv = gx[idx], myv = gy[idx], mzv = gz[idx];
+    coeff_common(mask[idx], mxv, myv, mzv, dx, h1, h2, ag, s);
+    b[idx] = h1 * const_b;
+    vx[idx] = s * real(1) * mxv * h2;
+    vy[idx] = s * real(1) * myv * h2;
+    vz[idx] = s * real(1) * mzv * h2;
+  }
+}
+hipError_t launch_var_curv_adv(const Box& bx, real* b, real* vx, real* vy, real* vz,
+                               const real* mask, const real* gx, const real* gy, const real* gz,
+                               real const_b, real dx, hipStream_t st) {
+  FillIter it(bx);
+  hipLaunchKernelGGL(var_curv_adv_kernel, dim3(stream_blocks(it.n)), dim3(256), 0, st, bx, b, vx,
+                     vy, vz, mask, gx, gy, gz, const_b, dx);
+  return hipGetLastError();
+}
+
+// reference: src/utilities/qss_util3d.f:499-581
+__global__ void trap_velocity_kernel(Box bx, const real* phi_w, real* vn, real* b, real* vx,
+                                     real* vy, real* vz, real eps_var) {
+  FillIter it(bx);
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < it.n;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    int i, j, k;
+    it.ijk(q, i, j, k);
+    int64_t idx = bx.at(i, j, k);
+    if (phi_w[idx] > eps_var) {
+      vn[idx] = real(0); b[idx] = real(0);
+      vx[idx] = real(0); vy[idx] = real(0); vz[idx] = real(0);
+    }
+  }
+}
+hipError_t launch_trap_velocity(const Box& bx, const real* phi_w, real* vn, real* b, real* vx,
+                                real* vy, real* vz, real dx, hipStream_t st) {
+  FillIter it(bx);
+  hipLaunchKernelGGL(trap_velocity_kernel, dim3(stream_blocks(it.n)), dim3(256), 0, st, bx, phi_w,
+                     vn, b, vx, vy, vz, real(1.5) * dx);
+  return hipGetLastError();
+}
+
+__global__ void collect_stale_kernel(Box bx, const real* vn, int64_t* list, int* count, int cap) {
+  FillIter it(bx);
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < it.n;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    int i, j, k;
+    it.ijk(q, i, j, k);
+    int64_t idx = bx.at(i, j, k);
+    if (fabs(vn[idx]) < real(1.e-5)) {
+      int pos = atomicAdd(count, 1);
+      if (pos < cap) list[pos] = idx;
+    }
+  }
+}
+hipError_t launch_collect_stale(const Box& bx, const real* vn, int64_t* list, int* count, int cap,
+                                hipStream_t st) {
+  FillIter it(bx);
+  hipLaunchKernelGGL(collect_stale_kernel, dim3(stream_blocks(it.n)), dim3(256), 0, st, bx, vn,
+                     list, count, cap);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Analytic signed distances (reference: src/geometry/qss_initialization3d.c:34-166,440-522).
+// The reference's C computes the sphere distance as sqrt() of the
+// QSSLIB_REAL sum in double, minus the radius in double, then stores it.
+struct Coords {
+  real xlo[3], dx[3];
+};
+
+__global__ void sphere_kernel(real* phi, Box bx, Coords c, real cx, real cy, real cz, real r,
+                              int inside) {
+  int64_t n = bx.size();
+  for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n;
+       l += (int64_t)gridDim.x * blockDim.x) {
+    int64_t t = l / bx.nx;
+    int i = (int)(l - t * bx.nx), j = (int)(t % bx.ny), k = (int)(t / bx.ny);
+    real x = c.xlo[0] + c.dx[0] * i;
+    real y = c.xlo[1] + c.dx[1] * j;
+    real z = c.xlo[2] + c.dx[2] * k;
+    real s = (x - cx) * (x - cx) + (y - cy) * (y - cy) + (z - cz) * (z - cz);
+    real d = (real)(sqrt((double)s) - (double)r);
+    if (inside >= 0) d = -d;
+    real m = -FLT_MAX;
+    if (d > m) m = d;
+    phi[l] = m;
+  }
+}
+hipError_t launch_sphere(real* phi, const Box& bx, const real* xlo, const real* dx, real cx,
+                         real cy, real cz, real r, int inside, hipStream_t st) {
+  Coords c;
+  for (int a = 0; a < 3; ++a) { c.xlo[a] = xlo[a]; c.dx[a] = dx[a]; }
+  hipLaunchKernelGGL(sphere_kernel, dim3(stream_blocks(bx.size())), dim3(256), 0, st, phi, bx, c,
+                     cx, cy, cz, r, inside);
+  return hipGetLastError();
+}
+
+__global__ void box_kernel(real* phi, Box bx, Coords c, Coords corner_side, int inside) {
+  int64_t n = bx.size();
+  for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n;
+       l += (int64_t)gridDim.x * blockDim.x) {
+    int64_t t = l / bx.nx;
+    int i = (int)(l - t * bx.nx), j = (int)(t % bx.ny), k = (int)(t / bx.ny);
+    real p[3] = {c.xlo[0] + c.dx[0] * i, c.xlo[1] + c.dx[1] * j, c.xlo[2] + c.dx[2] * k};
+    real m = -FLT_MAX;
+    for (int l6 = 0; l6 < 6; ++l6) {
+      int ax = l6 % 3;
+      real nrm = l6 < 3 ? real(-1) : real(1);
+      real pt = l6 < 3 ? corner_side.xlo[ax] : corner_side.xlo[ax] + corner_side.dx[ax];
+      real nv[3] = {real(0), real(0), real(0)};
+      nv[ax] = nrm;
+      real pts[3] = {l6 < 3 ? corner_side.xlo[0] : corner_side.xlo[0] + corner_side.dx[0],
+                     l6 < 3 ? corner_side.xlo[1] : corner_side.xlo[1] + corner_side.dx[1],
+                     l6 < 3 ? corner_side.xlo[2] : corner_side.xlo[2] + corner_side.dx[2]};
+      (void)pt;
+      real dot = (p[0] - pts[0]) * nv[0] + (p[1] - pts[1]) * nv[1] + (p[2] - pts[2]) * nv[2];
+      real nn = nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2];
+      nn = (real)sqrt((double)nn);
+      real d = dot / nn;
+      if (d > m) m = d;
+    }
+    if (inside >= 1) m = -m;
+    phi[l] = m;
+  }
+}
+hipError_t launch_box(real* phi, const Box& bx, const real* xlo, const real* dx,
+                      const real* corner, const real* side, int inside, hipStream_t st) {
+  Coords c, cs;
+  for (int a = 0; a < 3; ++a) {
+    c.xlo[a] = xlo[a]; c.dx[a] = dx[a];
+    cs.xlo[a] = corner[a]; cs.dx[a] = side[a];
+  }
+  hipLaunchKernelGGL(box_kernel, dim3(stream_blocks(bx.size())), dim3(256), 0, st, phi, bx, c, cs,
+                     inside);
+  return hipGetLastError();
+}
+
+}  // namespace lsm

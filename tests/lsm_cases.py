@@ -1,0 +1,109 @@
+"""Shared builders for the parity tests: seeded sphere-pack cases and runners
+that drive the oracle, the reference build and the HIP library through the
+same ctypes records."""
+import ctypes as C
+import os
+import tempfile
+
+import numpy as np
+
+from lsmpqs_amd.abi import Arrays, default_options
+from lsmpqs_amd.inputs import pack_case
+import oracle_py as op
+
+_CASES = {}
+_libc = C.CDLL("libc.so.6")
+_libc.fopen.restype = C.c_void_p
+_libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
+DEVNULL = C.c_void_p(_libc.fopen(b"/dev/null", b"w"))
+
+
+def case(n, dtype=np.float64, radius=4.0, porosity=0.4, seed=1001):
+    """(grid, mask, phi0) of a seeded n^3 pack (cached)."""
+    dims = (n, n, n) if np.isscalar(n) else tuple(n)
+    key = (dims, np.dtype(dtype).str, radius, porosity, seed)
+    if key not in _CASES:
+        g, mask, phi0, _ = pack_case(dims, radius, porosity, seed, dtype)
+        _CASES[key] = (g, mask, phi0)
+    return _CASES[key]
+
+
+def options(dtype, space=2, time=2, theta_deg=0.0, a=0.02, b=0.04, dc=0.25, steps=2,
+            tmax=20.0, conn=1):
+    o = default_options(dtype)
+    o.order_space_accur = space
+    o.order_time_accur = time
+    o.theta = theta_deg * 3.14159 / 180
+    o.a = a
+    o.b = b
+    o.dc = dc
+    o.amax = a + b * dc * (steps - 1) + 1e-3 * b * dc
+    o.amin = a - b * dc * (steps - 1) - 1e-3 * b * dc
+    o.tmax = tmax
+    o.check_connectivity = conn
+    return o
+
+
+def arrays(g, mask, phi0, dtype):
+    A = Arrays(g, dtype)
+    A["mask"] = mask
+    A["phi"] = phi0
+    return A
+
+
+def run_model(kind, dtype, n, opts, T=4, drain=True, imbibe_after=False, **case_kw):
+    """Run drain_model3d (and optionally imbibe_model3d on its result) on the
+    oracle ('oracle'), the reference build ('ref') or the HIP library ('hip')."""
+    g, mask, phi0 = case(n, dtype, **case_kw)
+    A = arrays(g, mask, phi0, dtype)
+    o = options(dtype, **opts)
+    cwd = os.getcwd()
+    if kind == "oracle":
+        lib = op.oracle_lib(dtype)
+        f_d, f_i = lib.orc_drain_model3d, lib.orc_imbibe_model3d
+        extra = (C.c_int(T),)
+    elif kind == "ref":
+        lib = op.ref_lib(dtype)
+        op.set_omp_threads(T)
+        f_d, f_i = lib.drain_model3d, lib.imbibe_model3d
+        extra = ()
+        os.chdir(tempfile.mkdtemp(prefix="lsmref_"))
+    else:
+        from lsmpqs_amd import hiplib
+        lib = hiplib.load(dtype)
+        lib.lsmpqs_set_dt_slabs(T)
+        os.environ["LSMPQS_NO_STEP_FILES"] = "1"
+        f_d, f_i = lib.drain_model3d, lib.imbibe_model3d
+        extra = ()
+    try:
+        if drain:
+            f_d(C.byref(o), A.ptr(), C.byref(g), DEVNULL, *extra)
+        if imbibe_after:
+            o.a = o.a + o.b * o.dc
+            f_i(C.byref(o), A.ptr(), C.byref(g), DEVNULL, *extra)
+        if kind == "hip":
+            from lsmpqs_amd import hiplib
+            hiplib.check(lib)
+            lib.lsmpqs_release(A.ptr())
+    finally:
+        os.chdir(cwd)
+    return A, o, g
+
+
+def fill(g):
+    return (slice(g.klo_fb, g.khi_fb + 1), slice(g.jlo_fb, g.jhi_fb + 1),
+            slice(g.ilo_fb, g.ihi_fb + 1))
+
+
+def maxdiff(a, b):
+    return float(np.max(np.abs(a.astype(np.float64) - b.astype(np.float64))))
+
+
+def saturation(phi, mask, g, eps=1.5):
+    """Non-wetting saturation with the reference's smoothed Heaviside, FP64."""
+    def vol(f):
+        f = f[fill(g)].astype(np.float64)
+        q = f / eps
+        v = np.where(f < -eps, 1.0, np.where(f < eps, 0.5 * (1 - q - 0.31830988618379 * np.sin(np.pi * q)), 0.0))
+        return v.sum()
+    return vol(phi) / vol(mask)

@@ -1,0 +1,147 @@
+"""GPU parity: the HIP library (through its C ABI) against the oracle on the
+same seeded inputs.  FP64 paths must agree to 1e-10 max-abs on phi and the
+trapped masks (bit-exact wherever no transcendental function is involved);
+labels and trapped partitions bit-exact."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from lsm_cases import case, arrays, options, run_model, maxdiff, saturation, DEVNULL, fill
+from lsmpqs_amd import hiplib
+import oracle_py as op
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-10
+
+DRAIN = [
+    ("eno2_rk2_theta0", dict(space=2, time=2, theta_deg=0)),
+    ("eno2_rk1_theta0", dict(space=2, time=1, theta_deg=0)),
+    ("eno3c_rk3_theta0", dict(space=5, time=3, theta_deg=0)),
+    ("weno5_rk3_theta30", dict(space=5, time=3, theta_deg=30)),
+    ("eno3_rk3_theta45", dict(space=3, time=3, theta_deg=45)),
+    ("eno2_rk2_theta60", dict(space=2, time=2, theta_deg=60)),
+]
+
+
+@pytest.mark.parametrize("name,opts", DRAIN, ids=[d[0] for d in DRAIN])
+def test_drain_fp64_matches_oracle(name, opts):
+    print(f"case drain {name} 32^3 fp64", flush=True)
+    Ao, _, g = run_model("oracle", np.float64, 32, opts)
+    Ah, _, _ = run_model("hip", np.float64, 32, opts)
+    for f in ("phi", "mask_w", "mask_nw", "mask"):
+        d = maxdiff(Ao[f], Ah[f])
+        print(f"  {f}: max|diff| = {d:.3e} exact={np.array_equal(Ao[f], Ah[f])}")
+        assert d <= TOL64, f"{name} {f} differs by {d}"
+    assert (Ao["phi"] < 0).sum() == (Ah["phi"] < 0).sum()
+
+
+def test_imbibe_after_drain_fp64():
+    opts = dict(space=5, time=3, theta_deg=30)
+    print("case drain+imbibe weno5 theta30 32^3 fp64", flush=True)
+    Ao, _, _ = run_model("oracle", np.float64, 32, opts, imbibe_after=True)
+    Ah, _, _ = run_model("hip", np.float64, 32, opts, imbibe_after=True)
+    for f in ("phi", "mask_w", "mask_nw", "normal_velocity"):
+        d = maxdiff(Ao[f], Ah[f])
+        print(f"  {f}: {d:.3e}")
+        assert d <= TOL64
+
+
+@pytest.mark.parametrize("T", [1, 3, 7])
+def test_dt_slabs_follow_thread_count(T):
+    opts = dict(space=2, time=2, theta_deg=0)
+    print(f"case slabs T={T}", flush=True)
+    Ao, _, _ = run_model("oracle", np.float64, 24, opts, T=T)
+    Ah, _, _ = run_model("hip", np.float64, 24, opts, T=T)
+    assert maxdiff(Ao["phi"], Ah["phi"]) <= TOL64
+
+
+def test_noncubic_grid():
+    opts = dict(space=2, time=3, theta_deg=30)
+    print("case noncubic (40,28,20)", flush=True)
+    Ao, _, _ = run_model("oracle", np.float64, (40, 28, 20), opts)
+    Ah, _, _ = run_model("hip", np.float64, (40, 28, 20), opts)
+    for f in ("phi", "mask_w", "mask_nw"):
+        assert maxdiff(Ao[f], Ah[f]) <= TOL64
+
+
+def _random_bin(shape, p, seed):
+    rng = np.random.default_rng(seed)
+    return (rng.random(shape) < p).astype(np.uint8)
+
+
+@pytest.mark.parametrize("p", [0.2, 0.35, 0.6])
+def test_labels_bit_exact(p):
+    print(f"case labels p={p}", flush=True)
+    g, mask, phi0 = case(20)
+    lib = hiplib.load(np.float64)
+    res = []
+    for kind in ("oracle", "hip"):
+        A = arrays(g, mask, phi0, np.float64)
+        A["phi_bin"] = _random_bin(A.shape, p, 7)
+        if kind == "oracle":
+            op.oracle_lib(np.float64).orc_findConnectivity3d(A.ptr(), C.byref(g))
+        else:
+            lib.findConnectivity3d(A.ptr(), C.byref(g))
+            hiplib.check(lib)
+            lib.lsmpqs_release(A.ptr())
+        res.append(A["connectivity"].copy())
+    assert np.array_equal(res[0], res[1])
+
+
+def test_reinit_and_bc_bit_exact():
+    print("case reinit+bc", flush=True)
+    g, mask, phi0 = case(32)
+    lib = hiplib.load(np.float64)
+    orc = op.oracle_lib(np.float64)
+    o = options(np.float64)
+    a = mask.copy()
+    b = mask.copy()
+    orc.orc_reinitializeSubcellFix3d(a.ctypes.data_as(C.c_void_p), C.byref(g), C.byref(o))
+    lib.reinitializeSubcellFix3d(b.ctypes.data_as(C.c_void_p), C.byref(g), C.byref(o))
+    hiplib.check(lib)
+    assert np.array_equal(a, b)
+    rng = np.random.default_rng(3)
+    a = rng.normal(size=mask.shape)
+    b = a.copy()
+    orc.orc_signedLinearExtrapolationBCqss(a.ctypes.data_as(C.c_void_p), C.byref(g), 9)
+    lib.signedLinearExtrapolationBCqss(b.ctypes.data_as(C.c_void_p), C.byref(g), 9)
+    hiplib.check(lib)
+    assert np.array_equal(a, b)
+
+
+def test_trap_components_bit_exact():
+    print("case trap", flush=True)
+    # a drained state with trapped blobs: run the oracle drain, then trap once on both
+    Ao, o, g = run_model("oracle", np.float64, 32, dict(space=2, time=2, theta_deg=0))
+    lib = hiplib.load(np.float64)
+    outs = []
+    for kind in ("oracle", "hip"):
+        A = arrays(g, Ao["mask"], Ao["phi"], np.float64)
+        A["mask_nw"] = Ao["mask_nw"]
+        A["mask_w"] = Ao["mask_w"]
+        if kind == "oracle":
+            op.oracle_lib(np.float64).orc_trapComponents_mask(A.ptr(), C.byref(g), C.byref(o))
+        else:
+            lib.trapComponents_mask(A.ptr(), C.byref(g), C.byref(o))
+            hiplib.check(lib)
+            lib.lsmpqs_release(A.ptr())
+        outs.append(A)
+    for f in ("phi", "mask_nw", "mask_w", "connectivity", "phi_bin"):
+        assert np.array_equal(outs[0][f], outs[1][f]), f
+
+
+def test_fp32_path_against_fp64_oracle():
+    opts = dict(space=5, time=3, theta_deg=30)
+    print("case fp32 weno5 vs fp64 oracle", flush=True)
+    Ao, _, g = run_model("oracle", np.float64, 32, opts)
+    Ah, _, g32 = run_model("hip", np.float32, 32, opts)
+    s64 = saturation(Ao["phi"], Ao["mask"], g)
+    s32 = saturation(Ah["phi"], Ah["mask"], g32)
+    band = np.abs(Ao["phi"][fill(g)]) < 3.0
+    linf = float(np.max(np.abs(Ao["phi"][fill(g)][band] - Ah["phi"][fill(g32)][band].astype(np.float64))))
+    mism = int(np.sum((Ao["phi"][fill(g)] < 0) != (Ah["phi"][fill(g32)] < 0)))
+    print(f"  saturation fp64 {s64:.8f} fp32 {s32:.8f} |d|={abs(s64 - s32):.2e}; band Linf {linf:.2e}; "
+          f"interface mismatch {mism}")
+    assert abs(s64 - s32) < 1e-3
