@@ -145,6 +145,7 @@ def cpu_leg(n, steps=8):
         H = max(H, mh.value)
     dt1 = 0.5 / (H + o.b_max_over_dx + o.max_U_over_dx)
     o.check_connectivity = 0
+    lib.setSpaceDerivFunc(C.byref(g), C.byref(o))
     o.tmax = 1e-30
     o.tplot = dt1 * (steps - 0.5)
     log = tempfile.NamedTemporaryFile(prefix="lsm_ref_", suffix=".log", delete=False).name
