@@ -87,7 +87,10 @@ int lsmpqs_set_dt_slabs(int nslabs);
 
 /* Reference-behaviour switches, all set by default; returns the old value.
    1: ENO3 x line reads phi(i+3,j,j); 2: WENO5 normal term with + sign;
-   4: |vel_n| < 1e-5 reuses the previous voxel's normal term. */
+   4: |vel_n| < 1e-5 reuses the previous voxel's normal term.
+   8 (not set by default): the FP32 library evaluates every expression in the
+   reference's order with IEEE divisions, instead of its fast WENO5 (one
+   reciprocal per reconstruction, shared z faces) and FMA-contracted curvature. */
 unsigned lsmpqs_set_quirks(unsigned mask);
 
 /* free the device state kept for record p */

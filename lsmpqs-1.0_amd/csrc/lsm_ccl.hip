@@ -1,10 +1,10 @@
 // lsm_ccl.hip -- connected-component labelling on the GPU and the
 // trapped-phase mask rebuild.
 //
-// Labelling is a concurrent union-find over the whole ghost box: every
-// foreground voxel starts as its own root; each voxel hooks itself to its
-// foreground neighbours (13 of the 26, the other 13 hook to it) by linking
-// the smaller root under the larger one with atomicCAS; a final pass
+// Labelling is a concurrent union-find in two levels: inside 16x8x4 tiles in
+// LDS, then across tile boundaries in global memory.  Each voxel hooks itself
+// to its foreground neighbours (13 of the 26, the other 13 hook to it) by
+// linking the smaller root under the larger one with atomicCAS; a final pass
 // compresses every path.  Linking only ever points a root at a larger index,
 // so each component ends rooted at its largest linear index, which is exactly
 // the label the reference's sequential union-find produces
@@ -38,28 +38,81 @@ __device__ __forceinline__ void uf_unite(int* L, int a, int b) {
   }
 }
 
-__global__ void ccl_init_kernel(const uint8_t* bin, int* L, int64_t n) {
-  for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n;
-       l += (int64_t)gridDim.x * blockDim.x)
-    L[l] = bin[l] ? (int)l : -1;
+// Pass 1: union-find inside one 16 x 8 x 4 tile in LDS.  Local indices are
+// ordered like global linear indices (x fastest, then y, then z), so the
+// local root -- the largest local index of the tile component -- is also its
+// largest global index.  Every voxel leaves pointing straight at that root.
+constexpr int CX = 16, CY = 8, CZ = 4, CN = CX * CY * CZ;
+
+__device__ __forceinline__ int lds_find(volatile int* p, int x) {
+  int q = p[x];
+  while (q != x) { x = q; q = p[x]; }
+  return x;
 }
 
-// conn = 6: face neighbours only; conn = 26: faces, edges and corners
-__global__ void ccl_merge_kernel(Box bx, const uint8_t* bin, int* L, int conn) {
+__global__ __launch_bounds__(CN) void ccl_local_kernel(Box bx, const uint8_t* bin, int* L, int conn) {
+  __shared__ int p[CN];
+  int tx = threadIdx.x, ty = threadIdx.y, tz = threadIdx.z;
+  int l = tx + CX * (ty + CY * tz);
+  int i = blockIdx.x * CX + tx, j = blockIdx.y * CY + ty, k = blockIdx.z * CZ + tz;
+  bool inside = i < bx.nx && j < bx.ny && k < bx.nz;
+  int64_t g = inside ? bx.at(i, j, k) : 0;
+  bool fg = inside && bin[g];
+  p[l] = fg ? l : -1;
+  __syncthreads();
+  if (fg) {
+    for (int dk = 0; dk <= 1; ++dk)
+      for (int dj = (dk ? -1 : 0); dj <= 1; ++dj)
+        for (int di = ((dk || dj) ? -1 : 1); di <= 1; ++di) {
+          if (conn == 6 && (abs(di) + abs(dj) + abs(dk)) != 1) continue;
+          int x2 = tx + di, y2 = ty + dj, z2 = tz + dk;
+          if (x2 < 0 || x2 >= CX || y2 < 0 || y2 >= CY || z2 >= CZ) continue;
+          int m = x2 + CX * (y2 + CY * z2);
+          if (p[m] < 0) continue;
+          int a = l, b = m;
+          while (true) {
+            a = lds_find(p, a);
+            b = lds_find(p, b);
+            if (a == b) break;
+            if (a > b) { int t = a; a = b; b = t; }
+            int old = atomicCAS(&p[a], a, b);
+            if (old == a) break;
+            a = old;
+          }
+        }
+  }
+  __syncthreads();
+  if (inside) {
+    if (fg) {
+      int r = lds_find(p, l);
+      int rx = r % CX, ry = (r / CX) % CY, rz = r / (CX * CY);
+      L[g] = (int)bx.at(blockIdx.x * CX + rx, blockIdx.y * CY + ry, blockIdx.z * CZ + rz);
+    } else {
+      L[g] = -1;
+    }
+  }
+}
+
+// Pass 2: unite across tile boundaries only (the forward half of the
+// neighbourhood, as in pass 1).
+__global__ void ccl_border_kernel(Box bx, const uint8_t* bin, int* L, int conn) {
   int64_t n = bx.size();
   for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n;
        l += (int64_t)gridDim.x * blockDim.x) {
     if (!bin[l]) continue;
     int64_t t = l / bx.nx;
     int i = (int)(l - t * bx.nx), j = (int)(t % bx.ny), k = (int)(t / bx.ny);
-    // forward half of the neighbourhood: (dk > 0) or (dk == 0 and dj > 0) or
-    // (dk == 0, dj == 0, di > 0)
+    int ti = i / CX, tj = j / CY, tk = k / CZ;
+    bool edge = (i % CX == 0) || (i % CX == CX - 1) || (j % CY == 0) || (j % CY == CY - 1) ||
+                (k % CZ == CZ - 1);
+    if (!edge) continue;
     for (int dk = 0; dk <= 1; ++dk)
       for (int dj = (dk ? -1 : 0); dj <= 1; ++dj)
         for (int di = ((dk || dj) ? -1 : 1); di <= 1; ++di) {
           if (conn == 6 && (abs(di) + abs(dj) + abs(dk)) != 1) continue;
           int i2 = i + di, j2 = j + dj, k2 = k + dk;
           if (i2 < 0 || i2 >= bx.nx || j2 < 0 || j2 >= bx.ny || k2 >= bx.nz) continue;
+          if (i2 / CX == ti && j2 / CY == tj && k2 / CZ == tk) continue;
           int64_t m = bx.at(i2, j2, k2);
           if (bin[m]) uf_unite(L, (int)l, (int)m);
         }
@@ -82,8 +135,9 @@ static int grid_for(int64_t n) {
 hipError_t launch_label_components(const Box& bx, const uint8_t* bin, int* L, int conn,
                                    hipStream_t st) {
   int64_t n = bx.size();
-  hipLaunchKernelGGL(ccl_init_kernel, dim3(grid_for(n)), dim3(256), 0, st, bin, L, n);
-  hipLaunchKernelGGL(ccl_merge_kernel, dim3(grid_for(n)), dim3(256), 0, st, bx, bin, L, conn);
+  dim3 tiles((bx.nx + CX - 1) / CX, (bx.ny + CY - 1) / CY, (bx.nz + CZ - 1) / CZ);
+  hipLaunchKernelGGL(ccl_local_kernel, tiles, dim3(CX, CY, CZ), 0, st, bx, bin, L, conn);
+  hipLaunchKernelGGL(ccl_border_kernel, dim3(grid_for(n)), dim3(256), 0, st, bx, bin, L, conn);
   hipLaunchKernelGGL(ccl_compress_kernel, dim3(grid_for(n)), dim3(256), 0, st, L, n);
   return hipGetLastError();
 }

@@ -14,9 +14,9 @@ namespace lsm {
 
 enum Scheme { ENO2 = 2, ENO3 = 3, WENO5 = 5 };
 
-template <typename T> __device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
-template <typename T> __device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
-template <typename T> __device__ __forceinline__ T tabs(T a) { return fabs(a); }
+template <typename T> __host__ __device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
+template <typename T> __host__ __device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
+template <typename T> __host__ __device__ __forceinline__ T tabs(T a) { return fabs(a); }
 
 // Geometry of one ghost box and its fill box; passed by value to kernels.
 struct Box {
@@ -107,6 +107,138 @@ __device__ __forceinline__ void weno5(const T* p, T inv_d, T& plus, T& minus) {
   T D1pp = p[5] - p[4], D1ppp = p[6] - p[5], D1mm = p[1] - p[0];
   plus = weno5_combine(D1ppp * inv_d, D1pp * inv_d, D1p * inv_d, D1 * inv_d, D1m * inv_d);
   minus = weno5_combine(D1mm * inv_d, D1m * inv_d, D1 * inv_d, D1p * inv_d, D1pp * inv_d);
+}
+
+// ---------------------------------------------------------------------------
+// Fast single-precision HJ-WENO5.  Same scheme and weights, rearranged for
+// throughput: with q_k = (S_k + eps)^2 the normalised weights are
+// w_k = c_k prod_{m != k} q_m / sum, so one hardware reciprocal replaces the
+// four IEEE divisions, and FMA contraction is allowed.  The weights depend
+// only on ratios, so the smoothness indicators are formed from the
+// differences divided by their largest magnitude m, with eps divided by m^2
+// (1e-6 + 1e-35/m^2): every q_k then lies in [1e-12, 2e3] and no product
+// underflows into the denormal range, where the hardware reciprocal returns
+// infinity.  Rounding differs from the reference's expression order by a few
+// ulp.  Below m = 1e-18 the reconstruction is the middle candidate (the
+// reference's float build forms 0/0 when all differences vanish).
+template <typename T>
+struct Weno5Face {
+  T minus_right;  // phi^-(i+1)
+  T plus_left;    // phi^+(i)
+};
+
+// v_rcp_f32 (1 ulp, denormal inputs read as zero); the host version, used
+// only by host-side tests of these formulas, models the same flush
+__host__ __device__ __forceinline__ float fast_rcp(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return __builtin_amdgcn_rcpf(x);
+#else
+  return (x != 0.0f && fabsf(x) < 1.17549435e-38f) ? copysignf(__builtin_inff(), x) : 1.0f / x;
+#endif
+}
+__host__ __device__ __forceinline__ double fast_rcp(double x) { return 1.0 / x; }
+
+// q_1..q_3 of the window v1..v5 after scaling by 1/max|v| (see above);
+// returns false when the window is (numerically) flat.
+template <typename T>
+__host__ __device__ __forceinline__ bool weno5_q(T v1, T v2, T v3, T v4, T v5, T& q1, T& q2, T& q3) {
+#pragma clang fp contract(fast)
+  T m = tmax<T>(tmax<T>(tmax<T>(tmax<T>(fabs(v1), fabs(v2)), fabs(v3)), fabs(v4)), fabs(v5));
+  if (!(m > T(1e-18))) return false;
+  T im = fast_rcp(m);
+  T e1 = v1 * im, e2 = v2 * im, e3 = v3 * im, e4 = v4 * im, e5 = v5 * im;
+  T eps = T(1e-6) + T(1e-35) * im * im;
+  T t;
+  t = e1 - T(2) * e2 + e3; T S1 = (T(13) / T(12)) * t * t;
+  t = e1 - T(4) * e2 + T(3) * e3; S1 += T(0.25) * t * t;
+  t = e2 - T(2) * e3 + e4; T S2 = (T(13) / T(12)) * t * t;
+  t = e2 - e4; S2 += T(0.25) * t * t;
+  t = e3 - T(2) * e4 + e5; T S3 = (T(13) / T(12)) * t * t;
+  t = T(3) * e3 - T(4) * e4 + e5; S3 += T(0.25) * t * t;
+  q1 = (S1 + eps) * (S1 + eps);
+  q2 = (S2 + eps) * (S2 + eps);
+  q3 = (S3 + eps) * (S3 + eps);
+  return true;
+}
+
+// candidate polynomials of the window v1..v5
+template <typename T>
+__host__ __device__ __forceinline__ void weno5_cands(T v1, T v2, T v3, T v4, T v5, T& p1, T& p2, T& p3) {
+#pragma clang fp contract(fast)
+  p1 = (T(1) / T(3)) * v1 - (T(7) / T(6)) * v2 + (T(11) / T(6)) * v3;
+  p2 = -(T(1) / T(6)) * v2 + (T(5) / T(6)) * v3 + (T(1) / T(3)) * v4;
+  p3 = (T(1) / T(3)) * v3 + (T(5) / T(6)) * v4 - (T(1) / T(6)) * v5;
+}
+
+// One-sided WENO5 of the five differences (v1..v5), weights 0.1/0.6/0.3.
+template <typename T>
+__host__ __device__ __forceinline__ T weno5_fast(T v1, T v2, T v3, T v4, T v5) {
+#pragma clang fp contract(fast)
+  T p1, p2, p3, q1, q2, q3;
+  weno5_cands(v1, v2, v3, v4, v5, p1, p2, p3);
+  if (!weno5_q(v1, v2, v3, v4, v5, q1, q2, q3)) return p2;
+  T a1 = T(0.1) * q2 * q3, a2 = T(0.6) * q1 * q3, a3 = T(0.3) * q1 * q2;
+  return (a1 * p1 + a2 * p2 + a3 * p3) * fast_rcp(a1 + a2 + a3);
+}
+
+// Both one-sided values that share the face between voxels i and i+1:
+// q[0..5] = phi(i-2..i+3).  phi^-(i+1) uses the differences d0..d4 in order,
+// phi^+(i) the same differences reversed, so the scaled smoothness
+// indicators are computed once (the reversed window's S1 and S3 are the
+// window's S3 and S1).
+template <typename T>
+__host__ __device__ __forceinline__ Weno5Face<T> weno5_face(const T* q, T inv) {
+#pragma clang fp contract(fast)
+  T d0 = (q[1] - q[0]) * inv, d1 = (q[2] - q[1]) * inv, d2 = (q[3] - q[2]) * inv;
+  T d3 = (q[4] - q[3]) * inv, d4 = (q[5] - q[4]) * inv;
+  Weno5Face<T> f;
+  T pm1, pm2, pm3, pp1, pp2, pp3, q1, q2, q3;
+  weno5_cands(d0, d1, d2, d3, d4, pm1, pm2, pm3);
+  weno5_cands(d4, d3, d2, d1, d0, pp1, pp2, pp3);
+  if (!weno5_q(d0, d1, d2, d3, d4, q1, q2, q3)) {
+    f.minus_right = pm2;
+    f.plus_left = pp2;
+    return f;
+  }
+  T q12 = q1 * q2, q13 = q1 * q3, q23 = q2 * q3;
+  {  // phi^-(i+1)
+    T a1 = T(0.1) * q23, a2 = T(0.6) * q13, a3 = T(0.3) * q12;
+    f.minus_right = (a1 * pm1 + a2 * pm2 + a3 * pm3) * fast_rcp(a1 + a2 + a3);
+  }
+  {  // phi^+(i)
+    T a1 = T(0.1) * q12, a2 = T(0.6) * q13, a3 = T(0.3) * q23;
+    f.plus_left = (a1 * pp1 + a2 * pp2 + a3 * pp3) * fast_rcp(a1 + a2 + a3);
+  }
+  return f;
+}
+
+template <typename T>
+__device__ __forceinline__ void weno5_fast_pair(const T* p, T inv, T& plus, T& minus) {
+#pragma clang fp contract(fast)
+  T D1 = p[3] - p[2], D1p = p[4] - p[3], D1m = p[2] - p[1];
+  T D1pp = p[5] - p[4], D1ppp = p[6] - p[5], D1mm = p[1] - p[0];
+  plus = weno5_fast(D1ppp * inv, D1pp * inv, D1p * inv, D1 * inv, D1m * inv);
+  minus = weno5_fast(D1mm * inv, D1m * inv, D1 * inv, D1p * inv, D1pp * inv);
+}
+
+// kappa |grad phi| with FMA contraction and a reciprocal (fast T path)
+template <typename T>
+__device__ __forceinline__ T curvature_fast(T c, T xm2, T xm1, T xp1, T xp2, T ym2, T ym1, T yp1,
+                                            T yp2, T zm2, T zm1, T zp1, T zp2, const T* xy,
+                                            const T* xz, const T* yz, T fx, T fy, T fz) {
+#pragma clang fp contract(fast)
+  T px = (xp1 - xm1) * fx, py = (yp1 - ym1) * fy, pz = (zp1 - zm1) * fz;
+  T pxx = (xp2 + xm2 - T(2.0) * c) * (fx * fx);
+  T pzz = (zp2 + zm2 - T(2.0) * c) * (fz * fz);
+  T pyy = (yp2 + ym2 - T(2.0) * c) * (fy * fy);
+  T pxy = (xy[0] + xy[1] - xy[2] - xy[3]) * (fx * fy);
+  T pxz = (xz[0] + xz[1] - xz[2] - xz[3]) * (fx * fz);
+  T pyz = (yz[0] + yz[1] - yz[2] - yz[3]) * (fy * fz);
+  T g2 = px * px + py * py + pz * pz;
+  if (g2 < 1.e-5f) return T(0.0);
+  T curv = pxx * (py * py + pz * pz) + pyy * (px * px + pz * pz) + pzz * (px * px + py * py)
+             - T(2.0) * (pxy * px * py + pxz * px * pz + pyz * py * pz);
+  return curv * fast_rcp(g2);
 }
 
 // Upwind derivative pair along one axis with the scheme's scalings.

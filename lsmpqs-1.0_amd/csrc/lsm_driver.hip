@@ -189,6 +189,7 @@ static void inner_step(Session* s, const StepCfg& c) {
   sa.overlap = c.overlap; sa.conn = c.conn;
   sa.stale_idx = s->stale_idx; sa.stale_src = s->stale_src;
   sa.nstale = (c.var ? s->nstale : 0);
+  sa.fast = (sizeof(real) == 4 && !(quirks() & EXACT_FP32)) ? 1 : 0;
   BcArgs bc{};
   bc.bx = bx;
 

@@ -30,6 +30,7 @@ struct StageArgs {
   const int64_t* stale_idx;  // sorted voxels with |vel_n| < 1e-5 (var path)
   const int64_t* stale_src;  // voxel whose normal term each of them reuses, or -1
   int nstale;
+  int fast;              // single precision: fast WENO5/curvature arithmetic
 };
 
 // scheme: 2, 3 or 5; var: variable coefficients; stage: 1, 2 or 3;

@@ -16,6 +16,7 @@ enum Quirk : unsigned {
   QUIRK_ENO3_READ = 1u,    // ENO3 x line reads phi(i+3,j,j)
   QUIRK_WENO5_SIGN = 2u,   // WENO5 normal term enters with + sign
   QUIRK_STALE_NORM = 4u,   // |vel_n| < 1e-5 reuses the previous voxel's normal term
+  EXACT_FP32 = 8u,         // single precision keeps the reference's expression order
 };
 
 struct Timers {

@@ -11,6 +11,7 @@
 // stage result in registers, and the last stage also applies the mask
 // imposition, so a full step writes only the stage buffers and phi.
 #include <algorithm>
+#include <type_traits>
 #include "lsm_kernels.h"
 
 namespace lsm {
@@ -73,7 +74,10 @@ __device__ real stale_norm(const StageArgs& a, int64_t idx) {
   return real(0);
 }
 
-template <int S, bool VAR, int STAGE, bool FINAL>
+// FAST (single precision only): WENO5 with one reciprocal per reconstruction,
+// z faces shared between consecutive planes, FMA contraction in the
+// curvature.  Otherwise every expression follows the reference's order.
+template <int S, bool VAR, int STAGE, bool FINAL, bool FAST>
 __global__ __launch_bounds__(TX * TY) void stage_kernel(StageArgs a) {
   __shared__ real pl[3][PY][PX];
   const Box bx = a.bx;
@@ -108,6 +112,11 @@ __global__ __launch_bounds__(TX * TY) void stage_kernel(StageArgs a) {
 #pragma unroll
     for (int o = 0; o < 6; ++o) w[o] = in[bx.at(i, j, k0 - 3 + o)];
   }
+  constexpr bool FW = FAST && S == WENO5;
+  real zcarry = real(0);  // phi_z^-(k) from the face k-1/2 (FW only)
+  if constexpr (FW) {
+    if (active) zcarry = weno5_face(&w[0], inv_dz).minus_right;
+  }
   int sm = 0, sc = 1, sp = 2;
   for (int k = k0; k <= k1; ++k) {
     __syncthreads();
@@ -124,9 +133,18 @@ __global__ __launch_bounds__(TX * TY) void stage_kernel(StageArgs a) {
       }
       if (S == ENO3 && a.eno3_quirk) lx[6] = eno3_quirk_read(in, bx, i, j);
       real d[6];
-      axis_derivs<real, S>(lx, inv_dx, inv_dx, d[0], d[1]);
-      axis_derivs<real, S>(ly, inv_dy, inv_dx, d[2], d[3]);
-      axis_derivs<real, S>(w, inv_dz, inv_dx, d[4], d[5]);
+      if constexpr (FW) {
+        weno5_fast_pair(lx, inv_dx, d[0], d[1]);
+        weno5_fast_pair(ly, inv_dy, d[2], d[3]);
+        Weno5Face fz_ = weno5_face(&w[1], inv_dz);
+        d[4] = fz_.plus_left;
+        d[5] = zcarry;
+        zcarry = fz_.minus_right;
+      } else {
+        axis_derivs<real, S>(lx, inv_dx, inv_dx, d[0], d[1]);
+        axis_derivs<real, S>(ly, inv_dy, inv_dx, d[2], d[3]);
+        axis_derivs<real, S>(w, inv_dz, inv_dx, d[4], d[5]);
+      }
 
       const real vn = VAR ? a.vn[idx] : a.a_const;
       real temp_norm;
@@ -141,8 +159,13 @@ __global__ __launch_bounds__(TX * TY) void stage_kernel(StageArgs a) {
                     pl[sm][ty + 3][tx + 4]};
       real yz[4] = {pl[sp][ty + 4][tx + 3], pl[sm][ty + 2][tx + 3], pl[sp][ty + 2][tx + 3],
                     pl[sm][ty + 4][tx + 3]};
-      real curv = curvature(lx[3], lx[1], lx[2], lx[4], lx[5], ly[1], ly[2], ly[4], ly[5], w[1],
-                            w[2], w[4], w[5], xy, xz, yz, fx, fy, fz);
+      real curv;
+      if constexpr (FAST)
+        curv = curvature_fast(lx[3], lx[1], lx[2], lx[4], lx[5], ly[1], ly[2], ly[4], ly[5], w[1],
+                              w[2], w[4], w[5], xy, xz, yz, fx, fy, fz);
+      else
+        curv = curvature(lx[3], lx[1], lx[2], lx[4], lx[5], ly[1], ly[2], ly[4], ly[5], w[1],
+                         w[2], w[4], w[5], xy, xz, yz, fx, fy, fz);
       real rhs;
       if (VAR) {
         real vxv = a.vx[idx], vyv = a.vy[idx], vzv = a.vz[idx];
@@ -183,20 +206,43 @@ __global__ __launch_bounds__(TX * TY) void stage_kernel(StageArgs a) {
   }
 }
 
-template <int S, bool VAR, int STAGE>
+template <int S, bool VAR, int STAGE, bool FAST>
 static hipError_t launch_sf(const StageArgs& a, bool fin, dim3 g, dim3 b, hipStream_t st) {
-  if (fin) hipLaunchKernelGGL((stage_kernel<S, VAR, STAGE, true>), g, b, 0, st, a);
-  else hipLaunchKernelGGL((stage_kernel<S, VAR, STAGE, false>), g, b, 0, st, a);
+  if (fin) hipLaunchKernelGGL((stage_kernel<S, VAR, STAGE, true, FAST>), g, b, 0, st, a);
+  else hipLaunchKernelGGL((stage_kernel<S, VAR, STAGE, false, FAST>), g, b, 0, st, a);
   return hipGetLastError();
 }
 
-template <int S, bool VAR>
+template <int S, bool VAR, bool FAST>
 static hipError_t launch_s(const StageArgs& a, int stage, bool fin, dim3 g, dim3 b, hipStream_t st) {
   switch (stage) {
-  case 1: return launch_sf<S, VAR, 1>(a, fin, g, b, st);
-  case 2: return launch_sf<S, VAR, 2>(a, fin, g, b, st);
-  default: return launch_sf<S, VAR, 3>(a, fin, g, b, st);
+  case 1: return launch_sf<S, VAR, 1, FAST>(a, fin, g, b, st);
+  case 2: return launch_sf<S, VAR, 2, FAST>(a, fin, g, b, st);
+  default: return launch_sf<S, VAR, 3, FAST>(a, fin, g, b, st);
   }
+}
+
+template <bool FAST>
+static hipError_t launch_scheme(const StageArgs& a, int scheme, bool var, int stage, bool fin,
+                                dim3 grid, dim3 block, hipStream_t st) {
+  switch (scheme) {
+  case 2: return var ? launch_s<ENO2, true, FAST>(a, stage, fin, grid, block, st)
+                     : launch_s<ENO2, false, FAST>(a, stage, fin, grid, block, st);
+  case 3: return var ? launch_s<ENO3, true, FAST>(a, stage, fin, grid, block, st)
+                     : launch_s<ENO3, false, FAST>(a, stage, fin, grid, block, st);
+  case 5: return var ? launch_s<WENO5, true, FAST>(a, stage, fin, grid, block, st)
+                     : launch_s<WENO5, false, FAST>(a, stage, fin, grid, block, st);
+  default: return hipErrorInvalidValue;
+  }
+}
+
+template <typename R>
+static hipError_t pick_arith(const StageArgs& a, int scheme, bool var, int stage, bool fin,
+                             dim3 grid, dim3 block, hipStream_t st) {
+  if constexpr (std::is_same<R, float>::value) {
+    if (a.fast) return launch_scheme<true>(a, scheme, var, stage, fin, grid, block, st);
+  }
+  return launch_scheme<false>(a, scheme, var, stage, fin, grid, block, st);
 }
 
 hipError_t launch_stage(const StageArgs& a, int scheme, bool var, int stage, bool fin,
@@ -209,15 +255,7 @@ hipError_t launch_stage(const StageArgs& a, int scheme, bool var, int stage, boo
   int gz = (2048 + cols - 1) / cols;
   gz = max(1, min(gz, (nzf + 7) / 8));
   dim3 grid(gx, gy, gz), block(TX, TY, 1);
-  switch (scheme) {
-  case 2: return var ? launch_s<ENO2, true>(a, stage, fin, grid, block, st)
-                     : launch_s<ENO2, false>(a, stage, fin, grid, block, st);
-  case 3: return var ? launch_s<ENO3, true>(a, stage, fin, grid, block, st)
-                     : launch_s<ENO3, false>(a, stage, fin, grid, block, st);
-  case 5: return var ? launch_s<WENO5, true>(a, stage, fin, grid, block, st)
-                     : launch_s<WENO5, false>(a, stage, fin, grid, block, st);
-  default: return hipErrorInvalidValue;
-  }
+  return pick_arith<real>(a, scheme, var, stage, fin, grid, block, st);
 }
 
 // ---------------------------------------------------------------------------

@@ -132,27 +132,46 @@ def test_trap_components_bit_exact():
         assert np.array_equal(outs[0][f], outs[1][f]), f
 
 
-@pytest.mark.parametrize("name,opts", [
+FP32_CASES = [
     ("weno5_rk3_theta30", dict(space=5, time=3, theta_deg=30)),
     ("eno2_rk2_theta0", dict(space=2, time=2, theta_deg=0)),
-], ids=["weno5t30", "eno2t0"])
-def test_fp32_path(name, opts):
-    """The FP32 library against the FP32 oracle (same arithmetic type) and,
-    as a reported error, against the FP64 oracle."""
-    print(f"case fp32 {name}", flush=True)
+    ("eno3_rk3_theta45", dict(space=3, time=3, theta_deg=45)),
+]
+
+
+@pytest.mark.parametrize("name,opts", FP32_CASES, ids=[c[0] for c in FP32_CASES])
+def test_fp32_fast_path_against_fp64_oracle(name, opts):
+    """The FP32 library (fast WENO5 arithmetic, FP64 reductions) against the
+    FP64 oracle: saturation within 1e-4, errors reported as the north star
+    asks (saturation, phi L-infinity in the |phi| < 3 band, interface voxels)."""
+    print(f"case fp32 fast {name}", flush=True)
     A64, _, g = run_model("oracle", np.float64, 32, opts)
-    A32, _, g32 = run_model("oracle", np.float32, 32, opts)
-    Ah, _, _ = run_model("hip", np.float32, 32, opts)
+    Ah, _, g32 = run_model("hip", np.float32, 32, opts)
     s64 = saturation(A64["phi"], A64["mask"], g)
-    s32 = saturation(A32["phi"], A32["mask"], g32)
     sh = saturation(Ah["phi"], Ah["mask"], g32)
     f = fill(g)
     band = np.abs(A64["phi"][f]) < 3.0
     linf = float(np.max(np.abs(A64["phi"][f][band] - Ah["phi"][f][band].astype(np.float64))))
     mism = int(np.sum((A64["phi"][f] < 0) != (Ah["phi"][f] < 0)))
-    print(f"  saturation fp64-oracle {s64:.8f} fp32-oracle {s32:.8f} fp32-gpu {sh:.8f}; "
-          f"vs fp64: |dS|={abs(s64 - sh):.2e} band Linf {linf:.2e} interface mismatch {mism}")
-    # same arithmetic type: the GPU and the oracle follow the same trajectory
+    print(f"  saturation fp64 {s64:.8f} fp32 {sh:.8f} |dS|={abs(s64 - sh):.2e}; "
+          f"band Linf {linf:.2e}; interface mismatch {mism} voxels")
+    assert abs(s64 - sh) < 1e-4
+    assert mism <= max(10, band.sum() // 1000)
+
+
+def test_fp32_exact_order_matches_fp32_oracle():
+    """With the exact-arithmetic switch the FP32 library evaluates the
+    reference's expressions in order and follows the FP32 oracle."""
+    opts = dict(space=5, time=3, theta_deg=30)
+    print("case fp32 exact-order weno5", flush=True)
+    lib = hiplib.load(np.float32)
+    old = lib.lsmpqs_set_quirks(1 | 2 | 4 | 8)
+    try:
+        Ah, _, g32 = run_model("hip", np.float32, 32, opts)
+    finally:
+        lib.lsmpqs_set_quirks(old)
+    A32, _, _ = run_model("oracle", np.float32, 32, opts)
+    s32 = saturation(A32["phi"], A32["mask"], g32)
+    sh = saturation(Ah["phi"], Ah["mask"], g32)
+    print(f"  saturation fp32-oracle {s32:.8f} fp32-gpu-exact {sh:.8f}")
     assert abs(sh - s32) < 1e-5
-    band32 = np.abs(A32["phi"][f]) < 3.0
-    assert maxdiff(A32["phi"][f][band32], Ah["phi"][f][band32]) < 1e-3
