@@ -99,6 +99,32 @@ def stage_roofline(res, itemsize):
             "alg_bytes_per_launch": per_voxel * res["nvox"]}
 
 
+def curve_leg(dtype, g64, mask, phi0, curv_steps, tmax, a, b):
+    """Drainage-curve time: drain_model3d end to end on the GPU (set-up,
+    curv_steps curvature steps, each running outer steps until the reference's
+    stopping tests fire or pseudo-time tmax; step files off), host wall time
+    from host arrays in to host arrays out."""
+    lib = hiplib.load(dtype)
+    g = make_grid(tuple(g64.grid_dims), dtype)
+    rec, keep = lean_arrays(dtype, mask, phi0)
+    o = options(dtype, a, b)
+    o.dc = 0.25
+    o.amax = a + b * o.dc * (curv_steps - 0.5)
+    o.tmax = tmax
+    lib.lsmpqs_set_dt_slabs(16)
+    os.environ["LSMPQS_NO_STEP_FILES"] = "1"
+    w0 = time.perf_counter()
+    lib.drain_model3d(C.byref(o), C.byref(rec), C.byref(g), None)
+    wall = time.perf_counter() - w0
+    hiplib.check(lib)
+    st = hiplib.Timing()
+    lib.lsmpqs_session_stats(C.byref(rec), C.byref(g), C.byref(st))
+    lib.lsmpqs_release(C.byref(rec))
+    return {"wall_s": round(wall, 3), "outer_steps": int(st.outer_steps),
+            "inner_steps": int(st.inner_steps), "inner_loop_s": round(st.inner_loop_ms * 1e-3, 3),
+            "nw_voxels": int((keep[0] < 0).sum())}
+
+
 def cpu_leg(n, steps=8):
     """The reference build (oracle/_ref, OpenMP, FP64) on a bounded sample:
     one outer step of constCurvModel3d at n^3 whose pseudo-time window holds
@@ -172,6 +198,22 @@ def cpu_leg(n, steps=8):
                       f"the reference's own inner-loop timer ({per_step * 1e3:.1f} ms/step)"}
 
 
+def calibration_copy(mbytes):
+    """One device-to-device copy of `mbytes` MiB (hipMemcpy's blit kernel), so a
+    counter run records FETCH_SIZE / WRITE_SIZE of a plain copy of known size."""
+    hip = C.CDLL("libamdhip64.so")
+    n = int(mbytes) << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0   # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(f"# calibration copy: {mbytes} MiB device to device", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -180,11 +222,30 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--cpu-n", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--curve-n", type=int, default=256)
+    ap.add_argument("--curve-steps", type=int, default=3)
+    ap.add_argument("--curve-tmax", type=float, default=20.0)
+    ap.add_argument("--precisions", default="f32,f64")
+    ap.add_argument("--calib-copy", type=int, default=0, help="MiB copied once before the run")
+    ap.add_argument("--curve-only", action="store_true", help="only the drainage-curve leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(z-slab sharding is described in DESIGN.md, not built)", file=sys.stderr)
         sys.exit(2)
+    if args.calib_copy:
+        calibration_copy(args.calib_copy)
+    if args.curve_only:
+        gc = make_grid((args.curve_n,) * 3, np.float64)
+        mc, _, _, _ = sphere_pack_mask(gc, (6, 12), 0.36, 1002)
+        pc = initial_phi(gc, mc)
+        out = {}
+        for key in args.precisions.split(","):
+            dt_ = np.float32 if key == "f32" else np.float64
+            out[key] = curve_leg(dt_, gc, mc, pc, args.curve_steps, args.curve_tmax, 0.02, 0.04)
+        print(json.dumps({"drainage_curve": out, "grid": args.curve_n}), flush=True)
+        return
+    precs = args.precisions.split(",")
     n = args.n
     g64 = make_grid((n, n, n), np.float64)
     t0 = time.perf_counter()
@@ -195,9 +256,23 @@ def main():
     a, b = 0.02, 0.04
     r32 = gpu_leg(np.float32, g64, mask, phi0, args.steps, args.warmup, a, b)
     print(f"# fp32 done: {r32['elapsed_ms'] / args.steps:.3f} ms/step", file=sys.stderr, flush=True)
+    if "f64" not in precs:
+        print(json.dumps({"f32_ms_per_step": r32["elapsed_ms"] / args.steps,
+                          "stage_avg_ms": r32["stage_avg_ms"]}), flush=True)
+        return
     r64 = gpu_leg(np.float64, g64, mask, phi0, args.steps, args.warmup, a, b)
     print(f"# fp64 done: {r64['elapsed_ms'] / args.steps:.3f} ms/step", file=sys.stderr, flush=True)
     del mask, phi0
+    curve = None
+    if args.curve_n > 0:
+        gc = make_grid((args.curve_n,) * 3, np.float64)
+        mc, _, _, _ = sphere_pack_mask(gc, (6, 12), 0.36, 1002)
+        pc = initial_phi(gc, mc)
+        curve = {"grid": args.curve_n, "curvature_steps": args.curve_steps, "tmax": args.curve_tmax}
+        for dt_, key in ((np.float32, "f32"), (np.float64, "f64")):
+            curve[key] = curve_leg(dt_, gc, mc, pc, args.curve_steps, args.curve_tmax, a, b)
+            print(f"# curve {key}: {curve[key]}", file=sys.stderr, flush=True)
+        del mc, pc
     v32 = r32["nvox"] * args.steps / (r32["elapsed_ms"] * 1e-3)
     v64 = r64["nvox"] * args.steps / (r64["elapsed_ms"] * 1e-3)
     cpu = None if args.no_cpu else cpu_leg(args.cpu_n)
@@ -221,6 +296,7 @@ def main():
         "fp64": {"value": round(v64, 1), "ms_per_step": round(r64["elapsed_ms"] / args.steps, 4),
                  "roofline": stage_roofline(r64, 8)},
         "fp32_over_fp64": round(v32 / v64, 3),
+        "drainage_curve": curve,
         "stage_avg_ms": {"f32": [round(x, 4) for x in r32["stage_avg_ms"]],
                          "f64": [round(x, 4) for x in r64["stage_avg_ms"]]},
         "host_wall_s": {"f32": round(r32["wall_s"], 4), "f64": round(r64["wall_s"], 4)},

@@ -109,6 +109,8 @@ typedef struct {
   long long inner_steps;
   double last_dt;
   int nstale;                 /* voxels reusing a previous normal term */
+  long long outer_steps;      /* lsmpqs_session_stats: outer steps since the last model call */
+  double inner_loop_ms;       /* lsmpqs_session_stats: GPU time inside inner loops */
 } lsmpqs_timing;
 int lsmpqs_bench_inner_steps(Options *o, QSS_DataArrays *p, Grid *g, int nsteps, int var_path,
                              QSSLIB_REAL a, int upload, lsmpqs_timing *out);
@@ -117,6 +119,9 @@ int lsmpqs_bench_inner_steps(Options *o, QSS_DataArrays *p, Grid *g, int nsteps,
    curvature step (a = options->a), leaving the state on the device for
    lsmpqs_bench_inner_steps(..., upload = 0, ...). */
 int lsmpqs_model_prepare(Options *o, QSS_DataArrays *p, Grid *g, int drain);
+/* counters of the last drain/imbibe/constCurvModel3d call on record p
+   (inner_steps, outer_steps, inner_loop_ms) */
+int lsmpqs_session_stats(QSS_DataArrays *p, Grid *g, lsmpqs_timing *out);
 /* copy one device field ("phi", "mask_w", "mask_nw") back to p */
 int lsmpqs_download(QSS_DataArrays *p, Grid *g, const char *field);
 

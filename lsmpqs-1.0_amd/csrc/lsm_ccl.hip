@@ -50,7 +50,8 @@ __device__ __forceinline__ int lds_find(volatile int* p, int x) {
   return x;
 }
 
-__global__ __launch_bounds__(CN) void ccl_local_kernel(Box bx, const uint8_t* bin, int* L, int conn) {
+__global__ __launch_bounds__(CN) void ccl_local_kernel(Box bx, const uint8_t* bin, int* L, int* T,
+                                                     int conn) {
   __shared__ int p[CN];
   int tx = threadIdx.x, ty = threadIdx.y, tz = threadIdx.z;
   int l = tx + CX * (ty + CY * tz);
@@ -86,16 +87,29 @@ __global__ __launch_bounds__(CN) void ccl_local_kernel(Box bx, const uint8_t* bi
     if (fg) {
       int r = lds_find(p, l);
       int rx = r % CX, ry = (r / CX) % CY, rz = r / (CX * CY);
-      L[g] = (int)bx.at(blockIdx.x * CX + rx, blockIdx.y * CY + ry, blockIdx.z * CZ + rz);
+      int root = (int)bx.at(blockIdx.x * CX + rx, blockIdx.y * CY + ry, blockIdx.z * CZ + rz);
+      L[g] = root;
+      T[g] = root;
     } else {
       L[g] = -1;
+      T[g] = -1;
     }
   }
 }
 
 // Pass 2: unite across tile boundaries only (the forward half of the
-// neighbourhood, as in pass 1).
-__global__ void ccl_border_kernel(Box bx, const uint8_t* bin, int* L, int conn) {
+// neighbourhood, as in pass 1).  T holds every voxel's tile root and does not
+// change; unions start from tile roots, so only root entries of L move.  A
+// union (l, m) is skipped when the voxel before l along x (or, failing that,
+// along y) lies in the same tiles on both sides and joins the same pair of
+// tile roots: that predecessor's union, or its own predecessor's, performs
+// it.  This removes the repeated unions of one root pair along a tile face.
+__device__ __forceinline__ bool same_pair(const uint8_t* bin, const int* T, int64_t l, int64_t m,
+                                          int64_t e, int a, int b) {
+  return bin[l - e] && bin[m - e] && T[l - e] == a && T[m - e] == b;
+}
+
+__global__ void ccl_border_kernel(Box bx, const uint8_t* bin, int* L, const int* T, int conn) {
   int64_t n = bx.size();
   for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n;
        l += (int64_t)gridDim.x * blockDim.x) {
@@ -106,6 +120,7 @@ __global__ void ccl_border_kernel(Box bx, const uint8_t* bin, int* L, int conn) 
     bool edge = (i % CX == 0) || (i % CX == CX - 1) || (j % CY == 0) || (j % CY == CY - 1) ||
                 (k % CZ == CZ - 1);
     if (!edge) continue;
+    int a0 = T[l];
     for (int dk = 0; dk <= 1; ++dk)
       for (int dj = (dk ? -1 : 0); dj <= 1; ++dj)
         for (int di = ((dk || dj) ? -1 : 1); di <= 1; ++di) {
@@ -114,7 +129,12 @@ __global__ void ccl_border_kernel(Box bx, const uint8_t* bin, int* L, int conn) 
           if (i2 < 0 || i2 >= bx.nx || j2 < 0 || j2 >= bx.ny || k2 >= bx.nz) continue;
           if (i2 / CX == ti && j2 / CY == tj && k2 / CZ == tk) continue;
           int64_t m = bx.at(i2, j2, k2);
-          if (bin[m]) uf_unite(L, (int)l, (int)m);
+          if (!bin[m]) continue;
+          int b0 = T[m];
+          if (a0 == b0) continue;
+          if (i % CX != 0 && i2 % CX != 0 && same_pair(bin, T, l, m, 1, a0, b0)) continue;
+          if (j % CY != 0 && j2 % CY != 0 && same_pair(bin, T, l, m, bx.nx, a0, b0)) continue;
+          uf_unite(L, a0, b0);
         }
   }
 }
@@ -132,12 +152,13 @@ static int grid_for(int64_t n) {
 }
 
 // labels: root (largest linear index of the component) on foreground, -1 on background
-hipError_t launch_label_components(const Box& bx, const uint8_t* bin, int* L, int conn,
+hipError_t launch_label_components(const Box& bx, const uint8_t* bin, int* L, int* T, int conn,
                                    hipStream_t st) {
   int64_t n = bx.size();
   dim3 tiles((bx.nx + CX - 1) / CX, (bx.ny + CY - 1) / CY, (bx.nz + CZ - 1) / CZ);
-  hipLaunchKernelGGL(ccl_local_kernel, tiles, dim3(CX, CY, CZ), 0, st, bx, bin, L, conn);
-  hipLaunchKernelGGL(ccl_border_kernel, dim3(grid_for(n)), dim3(256), 0, st, bx, bin, L, conn);
+  hipLaunchKernelGGL(ccl_local_kernel, tiles, dim3(CX, CY, CZ), 0, st, bx, bin, L, T, conn);
+  hipLaunchKernelGGL(ccl_border_kernel, dim3(grid_for(n)), dim3(256), 0, st, bx, bin, L,
+                     (const int*)T, conn);
   hipLaunchKernelGGL(ccl_compress_kernel, dim3(grid_for(n)), dim3(256), 0, st, L, n);
   return hipGetLastError();
 }
@@ -220,10 +241,10 @@ __global__ void trap_w_mask_kernel(TrapArgs a) {
 hipError_t launch_trap_components(const TrapArgs& a, int conn, hipStream_t st) {
   int64_t n = a.bx.size();
   hipLaunchKernelGGL(trap_nw_bin_kernel, dim3(grid_for(n)), dim3(256), 0, st, a);
-  hipError_t e = launch_label_components(a.bx, a.bin, a.labels, conn, st);
+  hipError_t e = launch_label_components(a.bx, a.bin, a.labels, a.troot, conn, st);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(trap_w_bin_kernel, dim3(grid_for(n)), dim3(256), 0, st, a);
-  e = launch_label_components(a.bx, a.bin, a.labels, conn, st);
+  e = launch_label_components(a.bx, a.bin, a.labels, a.troot, conn, st);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(trap_w_mask_kernel, dim3(grid_for(n)), dim3(256), 0, st, a);
   return hipGetLastError();

@@ -76,7 +76,7 @@ static void trap_dev(Session* s, const Grid* g, const Options* o) {
   TrapArgs ta{};
   ta.bx = s->bx;
   ta.phi = s->phi; ta.mask = s->mask; ta.mask_nw = s->mask_nw; ta.mask_w = s->mask_w;
-  ta.s = s->S; ta.bin = s->bin; ta.labels = s->labels;
+  ta.s = s->S; ta.bin = s->bin; ta.labels = s->labels; ta.troot = s->troot;
   ta.nw_ind = o->phi_nw_ind; ta.w_ind = o->phi_w_ind;
   ta.center_inlet = o->center_inlet;
   ta.fill = -20 * g->dx[0];
@@ -630,7 +630,7 @@ void findConnectivity3d(QSS_DataArrays* p, Grid* g) {
   ensure_labels(s);
   DCHECK(hipMemcpyAsync(s->bin, p->phi_bin, (size_t)s->n, hipMemcpyHostToDevice, s->st));
   int conn = getenv("LSMPQS_CONNECTIVITY") ? atoi(getenv("LSMPQS_CONNECTIVITY")) : 26;
-  DCHECK(launch_label_components(s->bx, s->bin, s->labels, conn == 6 ? 6 : 26, s->st));
+  DCHECK(launch_label_components(s->bx, s->bin, s->labels, s->troot, conn == 6 ? 6 : 26, s->st));
   std::vector<int> lab((size_t)s->n);
   DCHECK(hipMemcpyAsync(lab.data(), s->labels, (size_t)s->n * sizeof(int), hipMemcpyDeviceToHost,
                         s->st));
@@ -763,6 +763,16 @@ int lsmpqs_model_prepare(Options* o, QSS_DataArrays* p, Grid* g, int drain) {
   model_coeffs(s, o, g, var, drain != 0, o->a);
   if (o->check_connectivity) trap_dev(s, g, o);
   sync(s);
+  return 0;
+  API_CATCH(-1)
+}
+
+int lsmpqs_session_stats(QSS_DataArrays* p, Grid* g, lsmpqs_timing* out) {
+  API_TRY
+  Session* s = session_for(p, g);
+  out->inner_steps = s->tm.inner_steps;
+  out->outer_steps = s->tm.outer_steps;
+  out->inner_loop_ms = s->tm.inner_ms;
   return 0;
   API_CATCH(-1)
 }

@@ -122,12 +122,15 @@ struct TrapArgs {
   real* s;               // wetting-phase field max(-phi, mask, mask_nw)
   uint8_t* bin;
   int* labels;
+  int* troot;            // tile roots (labelling scratch)
   int nw_ind, w_ind;
   int center_inlet;
   real fill;             // -20 dx
 };
 hipError_t launch_trap_components(const TrapArgs& a, int connectivity, hipStream_t st);
-hipError_t launch_label_components(const Box& bx, const uint8_t* bin, int* labels,
+// labels: largest linear index of the component on foreground, -1 on
+// background; troot: scratch of the same size
+hipError_t launch_label_components(const Box& bx, const uint8_t* bin, int* labels, int* troot,
                                    int connectivity, hipStream_t st);
 
 }  // namespace lsm

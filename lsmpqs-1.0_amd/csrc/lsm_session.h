@@ -38,6 +38,7 @@ struct Session {
   real *vn = nullptr, *b = nullptr, *vx = nullptr, *vy = nullptr, *vz = nullptr;
   real *gx = nullptr, *gy = nullptr, *gz = nullptr;
   int* labels = nullptr;
+  int* troot = nullptr;
   uint8_t* bin = nullptr;
   double *part = nullptr, *red = nullptr;
   real* dt = nullptr;

@@ -68,7 +68,7 @@ static void dalloc(P*& p, int64_t count) {
 static void free_all(Session* s) {
   void* bufs[] = {s->phi, s->A, s->B, s->S, s->D, s->phi_prev, s->mask, s->mask_w, s->mask_nw,
                   s->phi_extra, s->vn, s->b, s->vx, s->vy, s->vz, s->gx, s->gy, s->gz,
-                  s->labels, s->bin, s->part, s->red, s->dt, s->count, s->stale_idx,
+                  s->labels, s->troot, s->bin, s->part, s->red, s->dt, s->count, s->stale_idx,
                   s->stale_src, s->stale_list};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
@@ -151,6 +151,7 @@ void ensure_grad(Session* s) {
 }
 void ensure_labels(Session* s) {
   dalloc(s->labels, s->n);
+  dalloc(s->troot, s->n);
   dalloc(s->bin, s->n);
 }
 

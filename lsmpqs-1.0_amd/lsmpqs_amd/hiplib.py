@@ -21,7 +21,8 @@ class LsmpqsError(RuntimeError):
 class Timing(C.Structure):
     _fields_ = [("profile", C.c_int), ("elapsed_ms", C.c_double),
                 ("stage_ms", C.c_double * 3), ("stage_launches", C.c_longlong * 3),
-                ("inner_steps", C.c_longlong), ("last_dt", C.c_double), ("nstale", C.c_int)]
+                ("inner_steps", C.c_longlong), ("last_dt", C.c_double), ("nstale", C.c_int),
+                ("outer_steps", C.c_longlong), ("inner_loop_ms", C.c_double)]
 
 
 # every exported entry point declared in include/lsmpqs_hip.h and qss_data_arrays.h
@@ -33,7 +34,7 @@ EXPORTS = (
     "writeDataArrayQSS", "readDataArrayQSS", "allocateQSSDataArrays",
     "allocateMemoryForQSSDataArrays", "destroyQSSDataArrays", "lsmpqs_hip_last_error",
     "lsmpqs_set_dt_slabs", "lsmpqs_set_quirks", "lsmpqs_release", "lsmpqs_bench_inner_steps",
-    "lsmpqs_download", "lsmpqs_model_prepare",
+    "lsmpqs_download", "lsmpqs_model_prepare", "lsmpqs_session_stats",
 )
 
 
