@@ -114,18 +114,36 @@ __device__ __forceinline__ void weno5(const T* p, T inv_d, T& plus, T& minus) {
 // throughput: with q_k = (S_k + eps)^2 the normalised weights are
 // w_k = c_k prod_{m != k} q_m / sum, so one hardware reciprocal replaces the
 // four IEEE divisions, and FMA contraction is allowed.  The weights depend
-// only on ratios, so the smoothness indicators are formed from the
-// differences divided by their largest magnitude m, with eps divided by m^2
-// (1e-6 + 1e-35/m^2): every q_k then lies in [1e-12, 2e3] and no product
-// underflows into the denormal range, where the hardware reciprocal returns
-// infinity.  Rounding differs from the reference's expression order by a few
-// ulp.  Below m = 1e-18 the reconstruction is the middle candidate (the
-// reference's float build forms 0/0 when all differences vanish).
+// only on ratios, so the smoothness indicators are formed from the raw
+// differences divided by their largest magnitude m, with eps divided by
+// (m/dx)^2 (1e-6 + 1e-35 dx^2/m^2): every q_k then lies in [1e-12, 2e3] and
+// no product underflows into the denormal range, where the hardware
+// reciprocal returns infinity.  The candidates are formed from the raw
+// differences and scaled by 1/dx once.  Rounding differs from the
+// reference's expression order by a few ulp.  Below m = 1e-18 dx the
+// reconstruction is the middle candidate (the reference's float build forms
+// 0/0 when all differences vanish).
 template <typename T>
 struct Weno5Face {
   T minus_right;  // phi^-(i+1)
   T plus_left;    // phi^+(i)
 };
+
+// Per-axis constants of the fast reconstruction.
+template <typename T>
+struct FaceScale {
+  T inv;  // 1/dx
+  T thr;  // 1e-18 dx: flat-window bound on the largest raw difference
+  T c1;   // sqrt(1e-35) dx: eps / (m/dx)^2 = 1e-6 + (c1/m)^2
+};
+template <typename T>
+__host__ __device__ inline FaceScale<T> face_scale(T dx) {
+  FaceScale<T> s;
+  s.inv = T(1) / dx;
+  s.thr = T(1e-18) * dx;
+  s.c1 = T(3.16227766016837933e-18) * dx;
+  return s;
+}
 
 // v_rcp_f32 (1 ulp, denormal inputs read as zero); the host version, used
 // only by host-side tests of these formulas, models the same flush
@@ -138,16 +156,33 @@ __host__ __device__ __forceinline__ float fast_rcp(float x) {
 }
 __host__ __device__ __forceinline__ double fast_rcp(double x) { return 1.0 / x; }
 
-// q_1..q_3 of the window v1..v5 after scaling by 1/max|v| (see above);
-// returns false when the window is (numerically) flat.
+// Both one-sided values that share the face between voxels i and i+1:
+// q[0..5] = phi(i-2..i+3).  phi^-(i+1) uses the differences r0..r4 in order,
+// phi^+(i) the same differences reversed, so the scaled smoothness
+// indicators are computed once (the reversed window's S1 and S3 are the
+// window's S3 and S1).  The combination is written as
+// p2 + w1 (p1 - p2) + w3 (p3 - p2), where p1 - p2 = g1/3 and p3 - p2 = g2/6
+// with the third differences g1 = r0 - 3r1 + 3r2 - r3, g2 = r1 - 3r2 + 3r3 - r4
+// (Jiang and Peng, SIAM J. Sci. Comput. 21 (2000), eq. 2.10); the reversed
+// window has g1' = -g2 and g2' = -g1.
 template <typename T>
-__host__ __device__ __forceinline__ bool weno5_q(T v1, T v2, T v3, T v4, T v5, T& q1, T& q2, T& q3) {
+__host__ __device__ __forceinline__ Weno5Face<T> weno5_face(const T* q, const FaceScale<T>& sc) {
 #pragma clang fp contract(fast)
-  T m = tmax<T>(tmax<T>(tmax<T>(tmax<T>(fabs(v1), fabs(v2)), fabs(v3)), fabs(v4)), fabs(v5));
-  if (!(m > T(1e-18))) return false;
+  T r0 = q[1] - q[0], r1 = q[2] - q[1], r2 = q[3] - q[2], r3 = q[4] - q[3], r4 = q[5] - q[4];
+  Weno5Face<T> f;
+  // middle candidates of the two windows
+  T pm2 = -(T(1) / T(6)) * r1 + (T(5) / T(6)) * r2 + (T(1) / T(3)) * r3;
+  T pp2 = -(T(1) / T(6)) * r3 + (T(5) / T(6)) * r2 + (T(1) / T(3)) * r1;
+  T m = fmax(fmax(fmax(fabs(r0), fabs(r1)), fmax(fabs(r2), fabs(r3))), fabs(r4));
+  if (!(m > sc.thr)) {
+    f.minus_right = pm2 * sc.inv;
+    f.plus_left = pp2 * sc.inv;
+    return f;
+  }
   T im = fast_rcp(m);
-  T e1 = v1 * im, e2 = v2 * im, e3 = v3 * im, e4 = v4 * im, e5 = v5 * im;
-  T eps = T(1e-6) + T(1e-35) * im * im;
+  T e1 = r0 * im, e2 = r1 * im, e3 = r2 * im, e4 = r3 * im, e5 = r4 * im;
+  T ce = sc.c1 * im;
+  T eps = T(1e-6) + ce * ce;
   T t;
   t = e1 - T(2) * e2 + e3; T S1 = (T(13) / T(12)) * t * t;
   t = e1 - T(4) * e2 + T(3) * e3; S1 += T(0.25) * t * t;
@@ -155,70 +190,22 @@ __host__ __device__ __forceinline__ bool weno5_q(T v1, T v2, T v3, T v4, T v5, T
   t = e2 - e4; S2 += T(0.25) * t * t;
   t = e3 - T(2) * e4 + e5; T S3 = (T(13) / T(12)) * t * t;
   t = T(3) * e3 - T(4) * e4 + e5; S3 += T(0.25) * t * t;
-  q1 = (S1 + eps) * (S1 + eps);
-  q2 = (S2 + eps) * (S2 + eps);
-  q3 = (S3 + eps) * (S3 + eps);
-  return true;
-}
-
-// candidate polynomials of the window v1..v5
-template <typename T>
-__host__ __device__ __forceinline__ void weno5_cands(T v1, T v2, T v3, T v4, T v5, T& p1, T& p2, T& p3) {
-#pragma clang fp contract(fast)
-  p1 = (T(1) / T(3)) * v1 - (T(7) / T(6)) * v2 + (T(11) / T(6)) * v3;
-  p2 = -(T(1) / T(6)) * v2 + (T(5) / T(6)) * v3 + (T(1) / T(3)) * v4;
-  p3 = (T(1) / T(3)) * v3 + (T(5) / T(6)) * v4 - (T(1) / T(6)) * v5;
-}
-
-// One-sided WENO5 of the five differences (v1..v5), weights 0.1/0.6/0.3.
-template <typename T>
-__host__ __device__ __forceinline__ T weno5_fast(T v1, T v2, T v3, T v4, T v5) {
-#pragma clang fp contract(fast)
-  T p1, p2, p3, q1, q2, q3;
-  weno5_cands(v1, v2, v3, v4, v5, p1, p2, p3);
-  if (!weno5_q(v1, v2, v3, v4, v5, q1, q2, q3)) return p2;
-  T a1 = T(0.1) * q2 * q3, a2 = T(0.6) * q1 * q3, a3 = T(0.3) * q1 * q2;
-  return (a1 * p1 + a2 * p2 + a3 * p3) * fast_rcp(a1 + a2 + a3);
-}
-
-// Both one-sided values that share the face between voxels i and i+1:
-// q[0..5] = phi(i-2..i+3).  phi^-(i+1) uses the differences d0..d4 in order,
-// phi^+(i) the same differences reversed, so the scaled smoothness
-// indicators are computed once (the reversed window's S1 and S3 are the
-// window's S3 and S1).
-template <typename T>
-__host__ __device__ __forceinline__ Weno5Face<T> weno5_face(const T* q, T inv) {
-#pragma clang fp contract(fast)
-  T d0 = (q[1] - q[0]) * inv, d1 = (q[2] - q[1]) * inv, d2 = (q[3] - q[2]) * inv;
-  T d3 = (q[4] - q[3]) * inv, d4 = (q[5] - q[4]) * inv;
-  Weno5Face<T> f;
-  T pm1, pm2, pm3, pp1, pp2, pp3, q1, q2, q3;
-  weno5_cands(d0, d1, d2, d3, d4, pm1, pm2, pm3);
-  weno5_cands(d4, d3, d2, d1, d0, pp1, pp2, pp3);
-  if (!weno5_q(d0, d1, d2, d3, d4, q1, q2, q3)) {
-    f.minus_right = pm2;
-    f.plus_left = pp2;
-    return f;
-  }
+  T q1 = (S1 + eps) * (S1 + eps);
+  T q2 = (S2 + eps) * (S2 + eps);
+  T q3 = (S3 + eps) * (S3 + eps);
   T q12 = q1 * q2, q13 = q1 * q3, q23 = q2 * q3;
-  {  // phi^-(i+1)
-    T a1 = T(0.1) * q23, a2 = T(0.6) * q13, a3 = T(0.3) * q12;
-    f.minus_right = (a1 * pm1 + a2 * pm2 + a3 * pm3) * fast_rcp(a1 + a2 + a3);
-  }
-  {  // phi^+(i)
-    T a1 = T(0.1) * q12, a2 = T(0.6) * q13, a3 = T(0.3) * q23;
-    f.plus_left = (a1 * pp1 + a2 * pp2 + a3 * pp3) * fast_rcp(a1 + a2 + a3);
-  }
+  T g1 = (r0 - r3) + T(3) * (r2 - r1);
+  T g2 = (r1 - r4) + T(3) * (r3 - r2);
+  T u = q23 * g1, v = q12 * g2;
+  T mid = T(0.6) * q13;
+  // phi^-: weights 0.1 q23, 0.6 q13, 0.3 q12; phi^+: 0.1 q12, 0.6 q13, 0.3 q23
+  T sum_m = T(0.1) * q23 + T(0.3) * q12 + mid;
+  T sum_p = T(0.1) * q12 + T(0.3) * q23 + mid;
+  T num_m = (T(1) / T(30)) * u + (T(1) / T(20)) * v;
+  T num_p = -((T(1) / T(20)) * u + (T(1) / T(30)) * v);
+  f.minus_right = (pm2 + num_m * fast_rcp(sum_m)) * sc.inv;
+  f.plus_left = (pp2 + num_p * fast_rcp(sum_p)) * sc.inv;
   return f;
-}
-
-template <typename T>
-__device__ __forceinline__ void weno5_fast_pair(const T* p, T inv, T& plus, T& minus) {
-#pragma clang fp contract(fast)
-  T D1 = p[3] - p[2], D1p = p[4] - p[3], D1m = p[2] - p[1];
-  T D1pp = p[5] - p[4], D1ppp = p[6] - p[5], D1mm = p[1] - p[0];
-  plus = weno5_fast(D1ppp * inv, D1pp * inv, D1p * inv, D1 * inv, D1m * inv);
-  minus = weno5_fast(D1mm * inv, D1m * inv, D1 * inv, D1p * inv, D1pp * inv);
 }
 
 // kappa |grad phi| with FMA contraction and a reciprocal (fast T path)
@@ -249,20 +236,18 @@ __device__ __forceinline__ void axis_derivs(const T* p, T inv_axis, T inv_x, T& 
   else weno5(p, inv_axis, plus, minus);
 }
 
-// Godunov |grad phi|^2 selected by the sign of vel_n (:460-480)
+// Godunov |grad phi|^2 selected by the sign of vel_n (:460-480).  Per axis
+// the reference takes max(max(d-,0)^2, min(d+,0)^2) for vel_n > 0 and
+// max(min(d-,0)^2, max(d+,0)^2) otherwise; since rounding of a square is
+// monotone this equals max(s d-, -s d+, 0)^2 with s = sign(vel_n), bit for
+// bit, without a branch on the sign.
 template <typename T>
 __device__ __forceinline__ T godunov_sq(T vn, const T* d) {
-  T a, b, s;
-  if (vn > T(0)) {
-    a = tmax(d[1], T(0)); a = a * a; b = tmin(d[0], T(0)); b = b * b; s = tmax(a, b);
-    a = tmax(d[3], T(0)); a = a * a; b = tmin(d[2], T(0)); b = b * b; s = s + tmax(a, b);
-    a = tmax(d[5], T(0)); a = a * a; b = tmin(d[4], T(0)); b = b * b; s = s + tmax(a, b);
-  } else {
-    a = tmin(d[1], T(0)); a = a * a; b = tmax(d[0], T(0)); b = b * b; s = tmax(a, b);
-    a = tmin(d[3], T(0)); a = a * a; b = tmax(d[2], T(0)); b = b * b; s = s + tmax(a, b);
-    a = tmin(d[5], T(0)); a = a * a; b = tmax(d[4], T(0)); b = b * b; s = s + tmax(a, b);
-  }
-  return s;
+  const T s = vn > T(0) ? T(1) : T(-1);
+  T x = fmax(fmax(s * d[1], -(s * d[0])), T(0));
+  T y = fmax(fmax(s * d[3], -(s * d[2])), T(0));
+  T z = fmax(fmax(s * d[5], -(s * d[4])), T(0));
+  return x * x + y * y + z * z;
 }
 
 // H/dx of the CFL bound (:488-513); d = {x+, x-, y+, y-, z+, z-}

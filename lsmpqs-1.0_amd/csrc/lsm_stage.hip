@@ -3,13 +3,24 @@
 // the composed signed-linear-extrapolation boundary condition.
 //
 // Stage kernel layout: a 64 x 8 thread block owns a 64 x 8 column of fill
-// voxels and marches it along z.  Three z-planes (k-1, k, k+1) of the input
-// field, each with a 3-voxel halo in x and y, live in LDS; the z line of
-// every voxel (k-3 .. k+3) lives in registers.  Each input plane is read from
-// HBM once per block column with x-fastest, 64-lane coalesced loads.  The
-// right-hand side never touches HBM: it is combined with u^n and dt into the
-// stage result in registers, and the last stage also applies the mask
-// imposition, so a full step writes only the stage buffers and phi.
+// voxels and marches it along z.  The z line of every voxel (k-3 .. k+3)
+// lives in registers, one plane ahead prefetched.  Four z-planes of the
+// input field, each with a 3-voxel halo in x and y, rotate through LDS: the
+// tile interior of plane k+1 is written from the register line, only the
+// 468-voxel halo ring is read from memory (each thread owns one fixed halo
+// slot), and with four slots one barrier per plane separates writing plane
+// k+1 from reading planes k-1..k+1.  The right-hand side never touches HBM:
+// it is combined with u^n and dt into the stage result in registers, and the
+// last stage also applies the mask imposition, so a full step writes only
+// the stage buffers and phi.
+//
+// Single-precision WENO5 fast path: every x, y and z face of the tile is
+// reconstructed once (both one-sided values of a face share the smoothness
+// indicators).  Each thread takes the x and y faces on its low side and
+// keeps their phi^- values; the phi^+ values go through LDS to the voxel on
+// the other side.  Threads 0..71 also take the tile's far y row (64 faces)
+// and far x column (8 faces).  z faces are carried in registers from one
+// plane to the next.
 #include <algorithm>
 #include <type_traits>
 #include "lsm_kernels.h"
@@ -20,6 +31,9 @@ constexpr int TX = 64;
 constexpr int TY = 8;
 constexpr int PX = TX + 6;
 constexpr int PY = TY + 6;
+constexpr int NSLOT = 4;
+constexpr int NHALO = PX * PY - TX * TY;  // 468
+static_assert(NHALO <= TX * TY, "one halo voxel per thread");
 
 // phi(i+3,j,j): the reference's ENO3 x line uses the k index j
 // (qss_spatial_derivatives3d.f:677,1454); outside the array it reads 0.
@@ -74,12 +88,45 @@ __device__ real stale_norm(const StageArgs& a, int64_t idx) {
   return real(0);
 }
 
-// FAST (single precision only): WENO5 with one reciprocal per reconstruction,
-// z faces shared between consecutive planes, FMA contraction in the
-// curvature.  Otherwise every expression follows the reference's order.
+// LDS position (row * PX + column) of the halo voxel thread `tid` stages,
+// and that voxel's offset within a z-plane; lds = -1 when the thread stages
+// none, off = -1 when the voxel lies outside the ghost box (staged as 0).
+struct HaloSlot {
+  int lds, off;
+};
+__device__ __forceinline__ HaloSlot halo_slot(const Box& bx, int i0, int j0, int tid) {
+  HaloSlot h{-1, -1};
+  if (tid >= NHALO) return h;
+  int r, c;
+  if (tid < 3 * PX) {  // rows 0..2
+    r = tid / PX;
+    c = tid - r * PX;
+  } else if (tid < 6 * PX) {  // rows TY+3 .. TY+5
+    int e = tid - 3 * PX;
+    r = TY + 3 + e / PX;
+    c = e - (e / PX) * PX;
+  } else {  // rows 3 .. TY+2, columns 0..2 and TX+3..TX+5
+    int e = tid - 6 * PX;
+    r = 3 + e / 6;
+    int q = e - (e / 6) * 6;
+    c = q < 3 ? q : TX + q;
+  }
+  h.lds = r * PX + c;
+  int gi = i0 - 3 + c, gj = j0 - 3 + r;
+  if (gi >= 0 && gi < bx.nx && gj >= 0 && gj < bx.ny) h.off = gi + bx.nx * gj;
+  return h;
+}
+
+// FAST (single precision only): WENO5 faces with one reciprocal per
+// reconstruction shared between neighbours, FMA contraction in the
+// curvature, hardware square root.  Otherwise every expression follows the
+// reference's order.
 template <int S, bool VAR, int STAGE, bool FINAL, bool FAST>
 __global__ __launch_bounds__(TX * TY) void stage_kernel(StageArgs a) {
-  __shared__ real pl[3][PY][PX];
+  constexpr bool FW = FAST && S == WENO5;
+  __shared__ real pl[NSLOT][PY * PX];
+  __shared__ real xp[FW ? 2 : 1][FW ? TY : 1][TX + 1];  // phi_x^+ by face, double-buffered
+  __shared__ real yp[FW ? 2 : 1][FW ? TY + 1 : 1][TX];  // phi_y^+ by face
   const Box bx = a.bx;
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
   const int i0 = bx.ilo + blockIdx.x * TX, j0 = bx.jlo + blockIdx.y * TY;
@@ -89,76 +136,134 @@ __global__ __launch_bounds__(TX * TY) void stage_kernel(StageArgs a) {
   const int k1 = min(k0 + zc - 1, bx.khi);
   if (k0 > k1) return;
   const bool active = (i <= bx.ihi) && (j <= bx.jhi);
+  const bool inbox = (i < bx.nx) && (j < bx.ny);
+  const int64_t plane = (int64_t)bx.nx * bx.ny;
   const real* __restrict__ in = a.in;
-
-  auto load_plane = [&](int slot, int k) {
-    for (int e = tid; e < PX * PY; e += TX * TY) {
-      int r = e / PX, c = e - r * PX;
-      int gi = i0 - 3 + c, gj = j0 - 3 + r;
-      real v = real(0);
-      if (gi >= 0 && gi < bx.nx && gj >= 0 && gj < bx.ny) v = in[bx.at(gi, gj, k)];
-      pl[slot][r][c] = v;
-    }
-  };
+  const HaloSlot hs = halo_slot(bx, i0, j0, tid);
+  const int me = (ty + 3) * PX + tx + 3;  // this thread's voxel in an LDS plane
 
   const real inv_dx = real(1.0) / a.dx, inv_dy = real(1.0) / a.dy, inv_dz = real(1.0) / a.dz;
   const real fx = real(0.5) / a.dx, fy = real(0.5) / a.dy, fz = real(0.5) / a.dz;
+  const FaceScale<real> sx = face_scale(a.dx), sy = face_scale(a.dy), sz = face_scale(a.dz);
   const real dt = *a.dt;
 
+  // z line k-3..k+3 of this thread's column; wnext = plane k+4 in flight
+  const real* col = in + (inbox ? (int64_t)i + (int64_t)bx.nx * j : 0);
   real w[7];
-  load_plane(0, k0 - 1);
-  load_plane(1, k0);
-  if (active) {
 #pragma unroll
-    for (int o = 0; o < 6; ++o) w[o] = in[bx.at(i, j, k0 - 3 + o)];
+  for (int o = 0; o < 7; ++o) w[o] = real(0);
+  real wnext = real(0);
+  if (inbox) {
+#pragma unroll
+    for (int o = 0; o < 6; ++o) w[o] = col[(int64_t)(k0 - 3 + o) * plane];
+    wnext = col[(int64_t)(k0 + 3) * plane];
   }
-  constexpr bool FW = FAST && S == WENO5;
+  // halo voxel of plane k+1 in flight
+  const real* hp = hs.off >= 0 ? in + hs.off : nullptr;
+  auto halo = [&](int k) { return hp ? hp[(int64_t)k * plane] : real(0); };
+  if (hs.lds >= 0) {
+    pl[0][hs.lds] = halo(k0 - 1);
+    pl[1][hs.lds] = halo(k0);
+  }
+  pl[0][me] = w[2];
+  pl[1][me] = w[3];
+  real hnext = hs.lds >= 0 ? halo(k0 + 1) : real(0);
+
   real zcarry = real(0);  // phi_z^-(k) from the face k-1/2 (FW only)
-  if constexpr (FW) {
-    if (active) zcarry = weno5_face(&w[0], inv_dz).minus_right;
-  }
-  int sm = 0, sc = 1, sp = 2;
-  for (int k = k0; k <= k1; ++k) {
-    __syncthreads();
-    load_plane(sp, k + 1);
-    if (active) w[6] = in[bx.at(i, j, k + 3)];
-    __syncthreads();
+  if constexpr (FW) zcarry = weno5_face(&w[0], sz).minus_right;
+  __syncthreads();
+
+  int64_t idx = bx.at(i, j, k0);
+  for (int k = k0; k <= k1; ++k, idx += plane) {
+    const int sm = (k - k0) & 3, sc = (k - k0 + 1) & 3, sp = (k - k0 + 2) & 3;
+    const int buf = (k - k0) & 1;
+    // ---- phase A: stage plane k+1, start this voxel's loads, x/y/z faces of plane k
+    w[6] = wnext;
+    if (inbox && k + 4 < bx.nz) wnext = col[(int64_t)(k + 4) * plane];
+    pl[sp][me] = w[4];
+    if (hs.lds >= 0) {
+      pl[sp][hs.lds] = hnext;
+      hnext = halo(k + 2);
+    }
+    real vn = VAR ? real(0) : a.a_const, bv = a.b_const, vxv = 0, vyv = 0, vzv = 0;
+    real unv = 0, mk = 0, mw = 0, mnw = 0;
     if (active) {
-      const int64_t idx = bx.at(i, j, k);
-      real lx[7], ly[7];
-#pragma unroll
-      for (int o = 0; o < 7; ++o) {
-        lx[o] = pl[sc][ty + 3][tx + o];
-        ly[o] = pl[sc][ty + o][tx + 3];
+      if (VAR) { vn = a.vn[idx]; bv = a.b[idx]; vxv = a.vx[idx]; vyv = a.vy[idx]; vzv = a.vz[idx]; }
+      if (STAGE > 1) unv = a.un[idx];
+      if (FINAL) {
+        mk = a.mask[idx];
+        if (a.conn) { mw = a.mask_w[idx]; mnw = a.mask_nw[idx]; }
       }
-      if (S == ENO3 && a.eno3_quirk) lx[6] = eno3_quirk_read(in, bx, i, j);
-      real d[6];
+    }
+    real d[6];
+    real qx[6], qy[6];
+    if constexpr (FW) {
+      const real* P = pl[sc];
+#pragma unroll
+      for (int o = 0; o < 6; ++o) {
+        qx[o] = P[(ty + 3) * PX + tx + o];   // phi(i-3+o, j)
+        qy[o] = P[(ty + o) * PX + tx + 3];   // phi(i, j-3+o)
+      }
+      Weno5Face<real> f = weno5_face(qx, sx);
+      d[1] = f.minus_right;
+      xp[buf][ty][tx] = f.plus_left;
+      f = weno5_face(qy, sy);
+      d[3] = f.minus_right;
+      yp[buf][ty][tx] = f.plus_left;
+      if (tid < TX + TY) {  // far y row and far x column of the tile
+        const bool yrow = tid < TX;
+        const real* q0 = yrow ? P + TY * PX + tid + 3 : P + (tid - TX + 3) * PX + TX;
+        const int st = yrow ? PX : 1;
+        real q[6];
+#pragma unroll
+        for (int o = 0; o < 6; ++o) q[o] = q0[o * st];
+        f = weno5_face(q, yrow ? sy : sx);
+        real* dst = yrow ? &yp[buf][TY][tid] : &xp[buf][tid - TX][TX];
+        *dst = f.plus_left;
+      }
+      f = weno5_face(&w[1], sz);
+      d[4] = f.plus_left;
+      d[5] = zcarry;
+      zcarry = f.minus_right;
+    }
+    __syncthreads();
+    // ---- phase B: right-hand side and stage update of voxel (i, j, k)
+    if (active) {
+      const real* P = pl[sc];
+      real lx[7], ly[7];
       if constexpr (FW) {
-        weno5_fast_pair(lx, inv_dx, d[0], d[1]);
-        weno5_fast_pair(ly, inv_dy, d[2], d[3]);
-        Weno5Face fz_ = weno5_face(&w[1], inv_dz);
-        d[4] = fz_.plus_left;
-        d[5] = zcarry;
-        zcarry = fz_.minus_right;
+        d[0] = xp[buf][ty][tx + 1];
+        d[2] = yp[buf][ty + 1][tx];
+#pragma unroll
+        for (int o = 0; o < 6; ++o) { lx[o] = qx[o]; ly[o] = qy[o]; }
+        lx[6] = ly[6] = real(0);  // not used by the curvature
       } else {
+#pragma unroll
+        for (int o = 0; o < 7; ++o) {
+          lx[o] = P[me - 3 + o];
+          ly[o] = P[me + (o - 3) * PX];
+        }
+        if (S == ENO3 && a.eno3_quirk) lx[6] = eno3_quirk_read(in, bx, i, j);
         axis_derivs<real, S>(lx, inv_dx, inv_dx, d[0], d[1]);
         axis_derivs<real, S>(ly, inv_dy, inv_dx, d[2], d[3]);
         axis_derivs<real, S>(w, inv_dz, inv_dx, d[4], d[5]);
       }
 
-      const real vn = VAR ? a.vn[idx] : a.a_const;
       real temp_norm;
       if (fabs(vn) >= real(1.e-5)) {
-        temp_norm = a.norm_sign * vn * sqrt(godunov_sq(vn, d));
+        real g2 = godunov_sq(vn, d);
+        real gn;
+        if constexpr (FAST) gn = __builtin_amdgcn_sqrtf(g2);
+        else gn = sqrt(g2);
+        temp_norm = a.norm_sign * vn * gn;
       } else {
         temp_norm = (VAR && a.nstale > 0) ? stale_norm<S>(a, idx) : real(0);
       }
-      real xy[4] = {pl[sc][ty + 4][tx + 4], pl[sc][ty + 2][tx + 2], pl[sc][ty + 4][tx + 2],
-                    pl[sc][ty + 2][tx + 4]};
-      real xz[4] = {pl[sp][ty + 3][tx + 4], pl[sm][ty + 3][tx + 2], pl[sp][ty + 3][tx + 2],
-                    pl[sm][ty + 3][tx + 4]};
-      real yz[4] = {pl[sp][ty + 4][tx + 3], pl[sm][ty + 2][tx + 3], pl[sp][ty + 2][tx + 3],
-                    pl[sm][ty + 4][tx + 3]};
+      const real* Pp = pl[sp];
+      const real* Pm = pl[sm];
+      real xy[4] = {P[me + PX + 1], P[me - PX - 1], P[me + PX - 1], P[me - PX + 1]};
+      real xz[4] = {Pp[me + 1], Pm[me - 1], Pp[me - 1], Pm[me + 1]};
+      real yz[4] = {Pp[me + PX], Pm[me - PX], Pp[me - PX], Pm[me + PX]};
       real curv;
       if constexpr (FAST)
         curv = curvature_fast(lx[3], lx[1], lx[2], lx[4], lx[5], ly[1], ly[2], ly[4], ly[5], w[1],
@@ -168,41 +273,37 @@ __global__ __launch_bounds__(TX * TY) void stage_kernel(StageArgs a) {
                          w[2], w[4], w[5], xy, xz, yz, fx, fy, fz);
       real rhs;
       if (VAR) {
-        real vxv = a.vx[idx], vyv = a.vy[idx], vzv = a.vz[idx];
         real ax = vxv >= real(0) ? d[1] : d[0];
         real ay = vyv >= real(0) ? d[3] : d[2];
         real az = vzv >= real(0) ? d[5] : d[4];
         real temp_vel = -(vxv * ax + vyv * ay + vzv * az);
-        rhs = temp_norm + a.b[idx] * curv + temp_vel;
+        rhs = temp_norm + bv * curv + temp_vel;
       } else {
-        rhs = temp_norm + a.b_const * curv;
+        rhs = temp_norm + bv * curv;
       }
       real val;
       if (STAGE == 1) {
         val = lx[3] + dt * rhs;
       } else if (STAGE == 2) {
-        val = real(0.75) * a.un[idx] + real(0.25) * (lx[3] + dt * rhs);
+        val = real(0.75) * unv + real(0.25) * (lx[3] + dt * rhs);
       } else {
-        val = real(1.0 / 3.0) * a.un[idx] + real(2.0 / 3.0) * (lx[3] + dt * rhs);
+        val = real(1.0 / 3.0) * unv + real(2.0 / 3.0) * (lx[3] + dt * rhs);
       }
       if (FINAL) {
         bool face = (i <= bx.ilo + 1) || (i >= bx.ihi - 1) || (j <= bx.jlo + 1) ||
                     (j >= bx.jhi - 1) || (k <= bx.klo + 1) || (k >= bx.khi - 1);
         if (face) a.pre[idx] = val;
-        real m = a.mask[idx] - a.overlap;
+        real m = mk - a.overlap;
         val = (m > val) ? m : val;
         if (a.conn) {
-          real mw = a.mask_w[idx];
           val = (mw > val) ? mw : val;
-          real mn = a.mask_nw[idx];
-          val = (mn > val) ? mn : val;
+          val = (mnw > val) ? mnw : val;
         }
       }
       a.out[idx] = val;
     }
 #pragma unroll
     for (int o = 0; o < 6; ++o) w[o] = w[o + 1];
-    int t = sm; sm = sc; sc = sp; sp = t;
   }
 }
 

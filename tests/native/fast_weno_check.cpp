@@ -1,7 +1,8 @@
 // Host-side check of the single-precision fast HJ-WENO5 face reconstruction
 // of lsm_device.h: over windows whose differences span 1e-40 .. 1e3 it must
 // return finite values and agree with the textbook double-precision WENO5
-// (weights 0.1/0.6/0.3, eps = 1e-6 max v^2 + 1e-35) to float accuracy.
+// (weights 0.1/0.6/0.3, eps = 1e-6 max v^2 + 1e-35, v the differences
+// divided by dx) to float accuracy, for dx = 1 and dx = 0.01.
 // Prints "<non-finite count> <worst error / max|difference|>".
 #include <algorithm>
 #include <cmath>
@@ -33,11 +34,15 @@ int main() {
     float q[6];
     q[0] = u(g);
     for (int m = 1; m < 6; ++m) q[m] = q[m - 1] + sc * u(g);
-    lsm::Weno5Face<float> f = lsm::weno5_face(q, 1.0f);
+    const float dx = (it & 1) ? 1.0f : 0.01f;
+    lsm::Weno5Face<float> f = lsm::weno5_face(q, lsm::face_scale(dx));
     if (!std::isfinite(f.minus_right) || !std::isfinite(f.plus_left)) { ++bad; continue; }
     double d[5];
     double scale = 0;
-    for (int m = 0; m < 5; ++m) { d[m] = (double)q[m + 1] - (double)q[m]; scale = std::max(scale, std::fabs(d[m])); }
+    for (int m = 0; m < 5; ++m) {
+      d[m] = ((double)q[m + 1] - (double)q[m]) / dx;
+      scale = std::max(scale, std::fabs(d[m]));
+    }
     if (scale < 1e-17) continue;
     double mr = weno_ref(d[0], d[1], d[2], d[3], d[4]), pl = weno_ref(d[4], d[3], d[2], d[1], d[0]);
     worst = std::max({worst, std::fabs(f.minus_right - mr) / scale, std::fabs(f.plus_left - pl) / scale});
