@@ -158,9 +158,8 @@ __host__ __device__ __forceinline__ double fast_rcp(double x) { return 1.0 / x; 
 
 // Both one-sided values that share the face between voxels i and i+1:
 // q[0..5] = phi(i-2..i+3).  phi^-(i+1) uses the differences r0..r4 in order,
-// phi^+(i) the same differences reversed, so the scaled smoothness
-// indicators are computed once (the reversed window's S1 and S3 are the
-// window's S3 and S1).  The combination is written as
+// phi^+(i) the same differences reversed, so the smoothness indicators are
+// computed once (the reversed window's S1 and S3 are the window's S3 and S1).  The combination is written as
 // p2 + w1 (p1 - p2) + w3 (p3 - p2), where p1 - p2 = g1/3 and p3 - p2 = g2/6
 // with the third differences g1 = r0 - 3r1 + 3r2 - r3, g2 = r1 - 3r2 + 3r3 - r4
 // (Jiang and Peng, SIAM J. Sci. Comput. 21 (2000), eq. 2.10); the reversed
@@ -179,23 +178,25 @@ __host__ __device__ __forceinline__ Weno5Face<T> weno5_face(const T* q, const Fa
     f.plus_left = pp2 * sc.inv;
     return f;
   }
+  // smoothness indicators of the raw differences (S_k = 13/12 t_k^2 + 1/4 u_k^2
+  // with t, u the reference's combinations, here from second differences);
+  // scaled by 1/m^2 inside q_k = (S_k/m^2 + eps')^2
+  T D1 = r1 - r0, D2 = r2 - r1, D3 = r3 - r2, D4 = r4 - r3;
+  T t1 = D2 - D1, t2 = D3 - D2, t3 = D4 - D3;       // e1-2e2+e3, e2-2e3+e4, e3-2e4+e5
+  T u1 = T(3) * D2 - D1, u2 = D2 + D3, u3 = T(3) * D3 - D4;  // |e1-4e2+3e3|, |e2-e4|, |3e3-4e4+e5|
+  T S1 = (T(13) / T(12)) * (t1 * t1) + T(0.25) * (u1 * u1);
+  T S2 = (T(13) / T(12)) * (t2 * t2) + T(0.25) * (u2 * u2);
+  T S3 = (T(13) / T(12)) * (t3 * t3) + T(0.25) * (u3 * u3);
   T im = fast_rcp(m);
-  T e1 = r0 * im, e2 = r1 * im, e3 = r2 * im, e4 = r3 * im, e5 = r4 * im;
+  T im2 = im * im;
   T ce = sc.c1 * im;
   T eps = T(1e-6) + ce * ce;
-  T t;
-  t = e1 - T(2) * e2 + e3; T S1 = (T(13) / T(12)) * t * t;
-  t = e1 - T(4) * e2 + T(3) * e3; S1 += T(0.25) * t * t;
-  t = e2 - T(2) * e3 + e4; T S2 = (T(13) / T(12)) * t * t;
-  t = e2 - e4; S2 += T(0.25) * t * t;
-  t = e3 - T(2) * e4 + e5; T S3 = (T(13) / T(12)) * t * t;
-  t = T(3) * e3 - T(4) * e4 + e5; S3 += T(0.25) * t * t;
-  T q1 = (S1 + eps) * (S1 + eps);
-  T q2 = (S2 + eps) * (S2 + eps);
-  T q3 = (S3 + eps) * (S3 + eps);
+  T q1 = S1 * im2 + eps; q1 = q1 * q1;
+  T q2 = S2 * im2 + eps; q2 = q2 * q2;
+  T q3 = S3 * im2 + eps; q3 = q3 * q3;
   T q12 = q1 * q2, q13 = q1 * q3, q23 = q2 * q3;
-  T g1 = (r0 - r3) + T(3) * (r2 - r1);
-  T g2 = (r1 - r4) + T(3) * (r3 - r2);
+  T g1 = t1 - t2;  // r0 - 3r1 + 3r2 - r3
+  T g2 = t2 - t3;  // r1 - 3r2 + 3r3 - r4
   T u = q23 * g1, v = q12 * g2;
   T mid = T(0.6) * q13;
   // phi^-: weights 0.1 q23, 0.6 q13, 0.3 q12; phi^+: 0.1 q12, 0.6 q13, 0.3 q23

@@ -185,8 +185,7 @@ static void inner_step(Session* s, const StepCfg& c) {
   sa.a_const = c.a_const; sa.b_const = c.b_const;
   sa.dx = c.dx; sa.dy = c.dy; sa.dz = c.dz;
   sa.norm_sign = c.norm_sign; sa.eno3_quirk = c.quirk_eno3;
-  sa.mask = s->mask; sa.mask_w = s->mask_w; sa.mask_nw = s->mask_nw;
-  sa.overlap = c.overlap; sa.conn = c.conn;
+  sa.floor = s->floor;
   sa.stale_idx = s->stale_idx; sa.stale_src = s->stale_src;
   sa.nstale = (c.var ? s->nstale : 0);
   sa.fast = (sizeof(real) == 4 && !(quirks() & EXACT_FP32)) ? 1 : 0;
@@ -197,8 +196,7 @@ static void inner_step(Session* s, const StepCfg& c) {
     // phi_new = max(phi + dt L(phi), masks) into B; face layers into A
     sa.in = s->phi; sa.un = s->phi; sa.out = s->B; sa.pre = s->A;
     stage_timed(s, sa, c, 1, true);
-    bc.src = s->A; bc.dst = s->B; bc.mask = s->mask; bc.mask_w = s->mask_w; bc.mask_nw = s->mask_nw;
-    bc.overlap = c.overlap; bc.conn = c.conn;
+    bc.src = s->A; bc.dst = s->B; bc.floor = s->floor;
     DCHECK(launch_bc(bc, true, s->st));
     std::swap(s->phi, s->B);
     return;
@@ -220,9 +218,14 @@ static void inner_step(Session* s, const StepCfg& c) {
     stage_timed(s, sa, c, 3, true);
     bc.src = s->A;
   }
-  bc.dst = s->phi; bc.mask = s->mask; bc.mask_w = s->mask_w; bc.mask_nw = s->mask_nw;
-  bc.overlap = c.overlap; bc.conn = c.conn;
+  bc.dst = s->phi; bc.floor = s->floor;
   DCHECK(launch_bc(bc, true, s->st));
+}
+
+// Imposition floor from the current masks; called whenever mask, mask_w,
+// mask_nw or the overlap may have changed before inner steps run.
+static void update_floor(Session* s, const StepCfg& c) {
+  DCHECK(launch_floor(s->floor, s->mask, s->mask_w, s->mask_nw, c.overlap, c.conn, s->n, s->st));
 }
 
 static StepCfg make_cfg(Session* s, const Options* o, const Grid* g, bool var, real a_const) {
@@ -294,6 +297,7 @@ static real ccm_dev(Session* s, Options* o, const Grid* g, FILE* fp, bool var, r
     s->stage_ev.resize(8);
     for (auto& e : s->stage_ev) DCHECK(hipEventCreate(&e));
   }
+  update_floor(s, c);
   outer_reduce(s, s->mask, nullptr, 0, eps, dV);
   vol_pore = s->h_red[0];
   outer_reduce(s, s->phi, nullptr, 0, eps, dV);
@@ -324,6 +328,7 @@ static real ccm_dev(Session* s, Options* o, const Grid* g, FILE* fp, bool var, r
         build_stale(s, g);
       }
       DCHECK(launch_merge(s->phi, s->mask_nw, s->n, s->st));
+      update_floor(s, c);
     }
     if (fp) fprintf(fp, "Reinitializing....");
     reinit_dev(s, s->phi, g, o);
@@ -730,6 +735,7 @@ int lsmpqs_bench_inner_steps(Options* o, QSS_DataArrays* p, Grid* g, int nsteps,
   }
   s->tm = Timers();
   StepCfg c = make_cfg(s, o, g, var, a);
+  update_floor(s, c);
   DCHECK(hipEventRecord(s->ev_a, s->st));
   for (int n = 0; n < nsteps; ++n) {
     inner_step(s, c);

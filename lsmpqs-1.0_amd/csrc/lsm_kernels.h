@@ -24,13 +24,12 @@ struct StageArgs {
   real dx, dy, dz;
   real norm_sign;        // -1, or +1 for the reference's WENO5 normal term
   int eno3_quirk;        // read phi(i+3,j,j) in the ENO3 x line, as the reference does
-  const real *mask, *mask_w, *mask_nw; // final stage: phi = max(phi, mask - overlap, mask_w, mask_nw)
-  real overlap;
-  int conn;              // trapped-phase masks active
+  const real* floor;     // final stage: phi = max(phi, floor), see launch_floor
   const int64_t* stale_idx;  // sorted voxels with |vel_n| < 1e-5 (var path)
   const int64_t* stale_src;  // voxel whose normal term each of them reuses, or -1
   int nstale;
   int fast;              // single precision: fast WENO5/curvature arithmetic
+  int tiles_x, tiles_y, tiles_z;  // launch decomposition, set by launch_stage
 };
 
 // scheme: 2, 3 or 5; var: variable coefficients; stage: 1, 2 or 3;
@@ -55,14 +54,12 @@ hipError_t launch_probe(const ProbeArgs& a, int scheme, hipStream_t st);
 
 // Signed linear extrapolation of every ghost voxel from the fill box, composed
 // in the reference's face order x-, x+, y-, y+, z-, z+.  With final_mask set,
-// reads `src` fill values, writes dst ghosts = max(BC, mask - overlap [, mask_w, mask_nw]).
+// reads `src` fill values, writes dst ghosts = max(BC, floor).
 struct BcArgs {
   Box bx;
   const real* src;
   real* dst;
-  const real *mask, *mask_w, *mask_nw;
-  real overlap;
-  int conn;
+  const real* floor;
 };
 hipError_t launch_bc(const BcArgs& a, bool final_mask, hipStream_t st);
 
@@ -72,6 +69,12 @@ hipError_t launch_impose(real* dst, const real* m, const real* src, real overlap
 hipError_t launch_merge(real* phi, const real* m, int64_t n, hipStream_t st);  // phi = m>0 ? -m : phi
 hipError_t launch_min(real* phi, const real* e, int64_t n, hipStream_t st);    // phi = min(phi, e)
 hipError_t launch_fill(real* dst, real v, int64_t n, hipStream_t st);
+// Imposition floor of the last stage: floor = mask - overlap, and with the
+// trapped-phase masks max(mask - overlap, mask_w, mask_nw).  max(max(max(v,
+// m - ov), mw), mnw) == max(v, floor) exactly, signed zeros included (both
+// keep the first maximal operand in the order v, m - ov, mw, mnw).
+hipError_t launch_floor(real* floor, const real* m, const real* mw, const real* mnw, real overlap,
+                        int conn, int64_t n, hipStream_t st);
 
 // FP64 reductions over the fill box.  out[0] = smoothed volume of {f1 < 0}
 // (width eps), out[1] = max |f1 - f2| over |f1| < zone (-1 if none; skipped

@@ -35,6 +35,7 @@ struct Session {
   // device fields
   real *phi = nullptr, *A = nullptr, *B = nullptr, *S = nullptr, *D = nullptr, *phi_prev = nullptr;
   real *mask = nullptr, *mask_w = nullptr, *mask_nw = nullptr, *phi_extra = nullptr;
+  real* floor = nullptr;  // imposition floor of the last RK stage (launch_floor)
   real *vn = nullptr, *b = nullptr, *vx = nullptr, *vy = nullptr, *vz = nullptr;
   real *gx = nullptr, *gy = nullptr, *gz = nullptr;
   int* labels = nullptr;

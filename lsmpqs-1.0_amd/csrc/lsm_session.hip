@@ -67,7 +67,7 @@ static void dalloc(P*& p, int64_t count) {
 
 static void free_all(Session* s) {
   void* bufs[] = {s->phi, s->A, s->B, s->S, s->D, s->phi_prev, s->mask, s->mask_w, s->mask_nw,
-                  s->phi_extra, s->vn, s->b, s->vx, s->vy, s->vz, s->gx, s->gy, s->gz,
+                  s->floor, s->phi_extra, s->vn, s->b, s->vx, s->vy, s->vz, s->gx, s->gy, s->gz,
                   s->labels, s->troot, s->bin, s->part, s->red, s->dt, s->count, s->stale_idx,
                   s->stale_src, s->stale_list};
   for (void* p : bufs)
@@ -110,6 +110,7 @@ Session* session_for(const QSS_DataArrays* p, const Grid* g) {
     int64_t n = s->n;
     dalloc(s->phi, n); dalloc(s->A, n); dalloc(s->B, n); dalloc(s->S, n); dalloc(s->D, n);
     dalloc(s->phi_prev, n); dalloc(s->mask, n); dalloc(s->mask_w, n); dalloc(s->mask_nw, n);
+    dalloc(s->floor, n);
     dalloc(s->part, reduce_partials()); dalloc(s->red, 4); dalloc(s->dt, 4); dalloc(s->count, 4);
     SCHECK(hipHostMalloc((void**)&s->h_dt, 4 * sizeof(real), hipHostMallocDefault));
     SCHECK(hipHostMalloc((void**)&s->h_red, 4 * sizeof(double), hipHostMallocDefault));
@@ -118,7 +119,8 @@ Session* session_for(const QSS_DataArrays* p, const Grid* g) {
     SCHECK(hipEventCreate(&s->ev_a));
     SCHECK(hipEventCreate(&s->ev_b));
     // zero every field so ghost/unused entries are defined
-    real* f[] = {s->phi, s->A, s->B, s->S, s->D, s->phi_prev, s->mask, s->mask_w, s->mask_nw};
+    real* f[] = {s->phi, s->A, s->B, s->S, s->D, s->phi_prev, s->mask, s->mask_w, s->mask_nw,
+                 s->floor};
     for (real* q : f) SCHECK(launch_fill(q, real(0), n, s->st));
     SCHECK(hipStreamSynchronize(s->st));
   } catch (...) {

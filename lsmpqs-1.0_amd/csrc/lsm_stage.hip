@@ -117,6 +117,30 @@ __device__ __forceinline__ HaloSlot halo_slot(const Box& bx, int i0, int j0, int
   return h;
 }
 
+// Workgroups are handed to the eight XCDs round-robin by linear id, and each
+// XCD has its own L2.  Block b is given tile number xcd * share + b / 8 (the
+// shares differ by at most one when the count is not a multiple of 8), so an
+// XCD works through a contiguous run of tiles, x fastest, and the halo rows
+// and columns neighbouring tiles share are fetched into one L2 instead of
+// several.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  const int q = nb >> 3, r = nb & 7, xcd = b & 7, slot = b >> 3;
+  return xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
+}
+
+// Pointwise inputs of one voxel: coefficients (variable path), u^n (stages
+// 2 and 3), imposition floor (last stage).
+template <typename T>
+struct PointIn {
+  T vn = 0, b = 0, vx = 0, vy = 0, vz = 0, un = 0, floor = 0;
+  template <bool VAR, int STAGE, bool FINAL>
+  __device__ __forceinline__ void load(const StageArgs& a, int64_t idx) {
+    if (VAR) { vn = a.vn[idx]; b = a.b[idx]; vx = a.vx[idx]; vy = a.vy[idx]; vz = a.vz[idx]; }
+    if (STAGE > 1) un = a.un[idx];
+    if (FINAL) floor = a.floor[idx];
+  }
+};
+
 // FAST (single precision only): WENO5 faces with one reciprocal per
 // reconstruction shared between neighbours, FMA contraction in the
 // curvature, hardware square root.  Otherwise every expression follows the
@@ -129,10 +153,13 @@ __global__ __launch_bounds__(TX * TY) void stage_kernel(StageArgs a) {
   __shared__ real yp[FW ? 2 : 1][FW ? TY + 1 : 1][TX];  // phi_y^+ by face
   const Box bx = a.bx;
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
-  const int i0 = bx.ilo + blockIdx.x * TX, j0 = bx.jlo + blockIdx.y * TY;
+  const int tile = xcd_tile(blockIdx.x, a.tiles_x * a.tiles_y * a.tiles_z);
+  const int tcol = tile % a.tiles_x, trow = (tile / a.tiles_x) % a.tiles_y;
+  const int tzc = tile / (a.tiles_x * a.tiles_y);
+  const int i0 = bx.ilo + tcol * TX, j0 = bx.jlo + trow * TY;
   const int i = i0 + tx, j = j0 + ty;
-  const int zc = (bx.khi - bx.klo + gridDim.z) / gridDim.z;
-  const int k0 = bx.klo + blockIdx.z * zc;
+  const int zc = (bx.khi - bx.klo + a.tiles_z) / a.tiles_z;
+  const int k0 = bx.klo + tzc * zc;
   const int k1 = min(k0 + zc - 1, bx.khi);
   if (k0 > k1) return;
   const bool active = (i <= bx.ihi) && (j <= bx.jhi);
@@ -185,16 +212,8 @@ __global__ __launch_bounds__(TX * TY) void stage_kernel(StageArgs a) {
       pl[sp][hs.lds] = hnext;
       hnext = halo(k + 2);
     }
-    real vn = VAR ? real(0) : a.a_const, bv = a.b_const, vxv = 0, vyv = 0, vzv = 0;
-    real unv = 0, mk = 0, mw = 0, mnw = 0;
-    if (active) {
-      if (VAR) { vn = a.vn[idx]; bv = a.b[idx]; vxv = a.vx[idx]; vyv = a.vy[idx]; vzv = a.vz[idx]; }
-      if (STAGE > 1) unv = a.un[idx];
-      if (FINAL) {
-        mk = a.mask[idx];
-        if (a.conn) { mw = a.mask_w[idx]; mnw = a.mask_nw[idx]; }
-      }
-    }
+    PointIn<real> cur;  // in flight during the face computations
+    if (active) cur.template load<VAR, STAGE, FINAL>(a, idx);
     real d[6];
     real qx[6], qy[6];
     if constexpr (FW) {
@@ -229,6 +248,8 @@ __global__ __launch_bounds__(TX * TY) void stage_kernel(StageArgs a) {
     __syncthreads();
     // ---- phase B: right-hand side and stage update of voxel (i, j, k)
     if (active) {
+      const real vn = VAR ? cur.vn : a.a_const, bv = VAR ? cur.b : a.b_const;
+      const real vxv = cur.vx, vyv = cur.vy, vzv = cur.vz, unv = cur.un, flo = cur.floor;
       const real* P = pl[sc];
       real lx[7], ly[7];
       if constexpr (FW) {
@@ -293,12 +314,7 @@ __global__ __launch_bounds__(TX * TY) void stage_kernel(StageArgs a) {
         bool face = (i <= bx.ilo + 1) || (i >= bx.ihi - 1) || (j <= bx.jlo + 1) ||
                     (j >= bx.jhi - 1) || (k <= bx.klo + 1) || (k >= bx.khi - 1);
         if (face) a.pre[idx] = val;
-        real m = mk - a.overlap;
-        val = (m > val) ? m : val;
-        if (a.conn) {
-          val = (mw > val) ? mw : val;
-          val = (mnw > val) ? mnw : val;
-        }
+        val = (flo > val) ? flo : val;
       }
       a.out[idx] = val;
     }
@@ -355,8 +371,10 @@ hipError_t launch_stage(const StageArgs& a, int scheme, bool var, int stage, boo
   int cols = gx * gy;
   int gz = (2048 + cols - 1) / cols;
   gz = max(1, min(gz, (nzf + 7) / 8));
-  dim3 grid(gx, gy, gz), block(TX, TY, 1);
-  return pick_arith<real>(a, scheme, var, stage, fin, grid, block, st);
+  StageArgs t = a;
+  t.tiles_x = gx; t.tiles_y = gy; t.tiles_z = gz;
+  dim3 grid(gx * gy * gz), block(TX, TY, 1);
+  return pick_arith<real>(t, scheme, var, stage, fin, grid, block, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -468,14 +486,8 @@ __global__ void bc_kernel(BcArgs a, int64_t nA, int64_t nB, int64_t nC) {
     real v = bc_z(a.src, bx, i, j, k);
     int64_t idx = bx.at(i, j, k);
     if (FINAL) {
-      real m = a.mask[idx] - a.overlap;
-      v = (m > v) ? m : v;
-      if (a.conn) {
-        real mw = a.mask_w[idx];
-        v = (mw > v) ? mw : v;
-        real mn = a.mask_nw[idx];
-        v = (mn > v) ? mn : v;
-      }
+      real f = a.floor[idx];
+      v = (f > v) ? f : v;
     }
     a.dst[idx] = v;
   }

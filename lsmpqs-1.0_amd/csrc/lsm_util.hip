@@ -34,6 +34,27 @@ hipError_t launch_impose(real* dst, const real* m, const real* src, real ov, int
   return hipGetLastError();
 }
 
+__global__ void floor_kernel(real* f, const real* m, const real* mw, const real* mnw, real ov,
+                             int conn, int64_t n) {
+  for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n;
+       l += (int64_t)gridDim.x * blockDim.x) {
+    real v = m[l] - ov;
+    if (conn) {
+      real a = mw[l];
+      v = (a > v) ? a : v;
+      a = mnw[l];
+      v = (a > v) ? a : v;
+    }
+    f[l] = v;
+  }
+}
+hipError_t launch_floor(real* floor, const real* m, const real* mw, const real* mnw, real overlap,
+                        int conn, int64_t n, hipStream_t st) {
+  hipLaunchKernelGGL(floor_kernel, dim3(stream_blocks(n)), dim3(256), 0, st, floor, m, mw, mnw,
+                     overlap, conn, n);
+  return hipGetLastError();
+}
+
 __global__ void merge_kernel(real* phi, const real* m, int64_t n) {
   for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n;
        l += (int64_t)gridDim.x * blockDim.x) {
