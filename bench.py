@@ -29,6 +29,7 @@ from lsmpqs_amd.abi import types_for, make_grid, default_options  # noqa: E402
 from lsmpqs_amd.inputs import sphere_pack_mask, initial_phi  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12   # unpacked vector lane-ops, T/s
 THETA_DEG = 30.0
 
 
@@ -86,17 +87,48 @@ def gpu_leg(dtype, g64, mask, phi0, steps, warmup, a, b):
                 nstale=t.nstale)
 
 
-def stage_roofline(res, itemsize):
-    """Roofline of the final-stage kernel (stage 3 with imposition): its
+def counter_record(n, key):
+    """Per-launch counters of the last-stage kernel at n^3 from the committed
+    rocprofv3 summary (profiles/weno5_rk3_512_counters.md, its json block):
+    HBM bytes = FETCH_SIZE x fetch_scale + WRITE_SIZE x write_scale (KiB,
+    scales from a plain copy kernel), VALU wave instructions."""
+    path = os.path.join(ROOT, "profiles", "weno5_rk3_512_counters.md")
+    try:
+        text = open(path).read()
+        blob = json.loads(text.rsplit("```json", 1)[1].split("```", 1)[0])
+    except (OSError, IndexError, ValueError):
+        return None
+    rec = blob.get(key)
+    if not rec or rec.get("grid") != n:
+        return None
+    return rec
+
+
+def stage_roofline(res, itemsize, key, n):
+    """Roofline of the last-stage kernel (stage 3 with imposition): its
     algorithmic traffic per fill voxel is 6 fields (phi^(2), phi^n, mask,
-    mask_w, mask_nw read; phi^(n+1) written)."""
+    mask_w, mask_nw read; phi^(n+1) written), the compulsory traffic with the
+    contact-angle coefficients derived from the mask.  `traffic` is the HBM
+    bytes per launch the counters measured (the library also streams the five
+    stored coefficient fields and the imposition floor); `valu` is the vector
+    ALU issue rate against 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz."""
     per_voxel = 6 * itemsize
     t = res["stage_avg_ms"][2] * 1e-3
     ach = per_voxel * res["nvox"] / t / 1e9
-    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "stage_kernel<WENO5,var,stage3,final>",
-            "alg_bytes_per_launch": per_voxel * res["nvox"]}
+    out = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+           "kernel": "stage_kernel<WENO5,var,stage3,final>",
+           "alg_bytes_per_launch": per_voxel * res["nvox"]}
+    rec = counter_record(n, key)
+    if rec:
+        traffic = (rec["fetch_kib"] * rec["fetch_scale"] + rec["write_kib"] * rec["write_scale"]) * 1024
+        out["traffic"] = int(traffic)
+        out["traffic_rate_gbs"] = round(traffic / t / 1e9, 1)
+        lane_ops = rec["valu_insts"] * 64
+        out["valu"] = {"achieved_tops": round(lane_ops / t / 1e12, 2), "peak_tops": VALU_PEAK_TOPS,
+                       "frac": round(lane_ops / t / 1e12 / VALU_PEAK_TOPS, 4),
+                       "unit": "T lane-ops/s", "insts_per_voxel": round(lane_ops / res["nvox"], 1)}
+    return out
 
 
 def curve_leg(dtype, g64, mask, phi0, curv_steps, tmax, a, b):
@@ -291,10 +323,10 @@ def main():
         "data": "synthetic (seeded Boolean sphere pack, radius U[6,12], porosity 0.36, seed 1002)",
         "config": {"workload": f"{n}^3 drainage step, WENO5 + TVD-RK3, theta 30 deg, dx 1, "
                                f"16 dt slabs, trapping masks on", "grid": n, "scheme": "WENO5+TVD-RK3"},
-        "roofline": stage_roofline(r32, 4),
+        "roofline": stage_roofline(r32, 4, "f32", n),
         "cpu_baseline": cpu,
         "fp64": {"value": round(v64, 1), "ms_per_step": round(r64["elapsed_ms"] / args.steps, 4),
-                 "roofline": stage_roofline(r64, 8)},
+                 "roofline": stage_roofline(r64, 8, "f64", n)},
         "fp32_over_fp64": round(v32 / v64, 3),
         "drainage_curve": curve,
         "stage_avg_ms": {"f32": [round(x, 4) for x in r32["stage_avg_ms"]],
