@@ -1,7 +1,7 @@
 // lsm_ccl.hip -- connected-component labelling on the GPU and the
 // trapped-phase mask rebuild.
 //
-// Labelling is a concurrent union-find in two levels: inside 16x8x4 tiles in
+// Labelling is a concurrent union-find in two levels: inside 32x16x8 tiles in
 // LDS, then across tile boundaries in global memory.  Each voxel hooks itself
 // to its foreground neighbours (13 of the 26, the other 13 hook to it) by
 // linking the smaller root under the larger one with atomicCAS; a final pass
@@ -38,111 +38,162 @@ __device__ __forceinline__ void uf_unite(int* L, int a, int b) {
   }
 }
 
-// Pass 1: union-find inside one 16 x 8 x 4 tile in LDS.  Local indices are
-// ordered like global linear indices (x fastest, then y, then z), so the
-// local root -- the largest local index of the tile component -- is also its
-// largest global index.  Every voxel leaves pointing straight at that root.
-constexpr int CX = 16, CY = 8, CZ = 4, CN = CX * CY * CZ;
+// Pass 1: union-find inside one 32 x 16 x 8 tile in LDS (16 KB).  A block
+// of 32 x 16 threads owns the tile; each thread takes the 8 voxels of its z
+// column.  Local indices are ordered like global linear indices (x fastest,
+// then y, then z), so the local root -- the largest local index of the tile
+// component -- is also its largest global index.  Every voxel leaves
+// pointing straight at that root, in L (which pass 2 links further) and in T
+// (which stays: the tile root of every voxel).
+constexpr int CX = 32, CY = 16, CZ = 8, CN = CX * CY * CZ;
 
+// root of x with path halving (each visited entry is re-pointed at its
+// grandparent, an ancestor, so concurrent finds and links stay consistent)
 __device__ __forceinline__ int lds_find(volatile int* p, int x) {
   int q = p[x];
-  while (q != x) { x = q; q = p[x]; }
+  while (q != x) {
+    int g = p[q];
+    if (g != q) p[x] = g;
+    x = q;
+    q = g;
+  }
   return x;
 }
 
-__global__ __launch_bounds__(CN) void ccl_local_kernel(Box bx, const uint8_t* bin, int* L, int* T,
-                                                     int conn) {
+__device__ __forceinline__ void lds_unite(int* p, int a, int b) {
+  while (true) {
+    a = lds_find(p, a);
+    b = lds_find(p, b);
+    if (a == b) return;
+    if (a > b) { int t = a; a = b; b = t; }
+    int old = atomicCAS(&p[a], a, b);
+    if (old == a) return;
+    a = old;
+  }
+}
+
+__global__ __launch_bounds__(CX * CY) void ccl_local_kernel(Box bx, const uint8_t* bin, int* L,
+                                                          int* T, int conn) {
   __shared__ int p[CN];
-  int tx = threadIdx.x, ty = threadIdx.y, tz = threadIdx.z;
-  int l = tx + CX * (ty + CY * tz);
-  int i = blockIdx.x * CX + tx, j = blockIdx.y * CY + ty, k = blockIdx.z * CZ + tz;
-  bool inside = i < bx.nx && j < bx.ny && k < bx.nz;
-  int64_t g = inside ? bx.at(i, j, k) : 0;
-  bool fg = inside && bin[g];
-  p[l] = fg ? l : -1;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int i = blockIdx.x * CX + tx, j = blockIdx.y * CY + ty, kb = blockIdx.z * CZ;
+  const bool col = i < bx.nx && j < bx.ny;
+  const int plane = bx.nx * bx.ny;
+  const int g0 = i + bx.nx * j;
+  unsigned fgm = 0;  // bit z: voxel (i, j, kb + z) is foreground
+#pragma unroll
+  for (int z = 0; z < CZ; ++z) {
+    const int l = tx + CX * (ty + CY * z);
+    bool fg = col && kb + z < bx.nz && bin[g0 + (kb + z) * plane];
+    p[l] = fg ? l : -1;
+    fgm |= fg ? (1u << z) : 0u;
+  }
   __syncthreads();
-  if (fg) {
+  for (int z = 0; z < CZ; ++z) {
+    if (!(fgm >> z & 1u)) continue;
+    const int l = tx + CX * (ty + CY * z);
     for (int dk = 0; dk <= 1; ++dk)
       for (int dj = (dk ? -1 : 0); dj <= 1; ++dj)
         for (int di = ((dk || dj) ? -1 : 1); di <= 1; ++di) {
           if (conn == 6 && (abs(di) + abs(dj) + abs(dk)) != 1) continue;
-          int x2 = tx + di, y2 = ty + dj, z2 = tz + dk;
+          int x2 = tx + di, y2 = ty + dj, z2 = z + dk;
           if (x2 < 0 || x2 >= CX || y2 < 0 || y2 >= CY || z2 >= CZ) continue;
           int m = x2 + CX * (y2 + CY * z2);
           if (p[m] < 0) continue;
-          int a = l, b = m;
-          while (true) {
-            a = lds_find(p, a);
-            b = lds_find(p, b);
-            if (a == b) break;
-            if (a > b) { int t = a; a = b; b = t; }
-            int old = atomicCAS(&p[a], a, b);
-            if (old == a) break;
-            a = old;
-          }
+          lds_unite(p, l, m);
         }
   }
   __syncthreads();
-  if (inside) {
-    if (fg) {
+  if (!col) return;
+  for (int z = 0; z < CZ && kb + z < bx.nz; ++z) {
+    const int l = tx + CX * (ty + CY * z);
+    const int g = g0 + (kb + z) * plane;
+    int root = -1;
+    if (fgm >> z & 1u) {
       int r = lds_find(p, l);
       int rx = r % CX, ry = (r / CX) % CY, rz = r / (CX * CY);
-      int root = (int)bx.at(blockIdx.x * CX + rx, blockIdx.y * CY + ry, blockIdx.z * CZ + rz);
-      L[g] = root;
-      T[g] = root;
-    } else {
-      L[g] = -1;
-      T[g] = -1;
+      root = (blockIdx.x * CX + rx) + bx.nx * ((blockIdx.y * CY + ry) + bx.ny * (kb + rz));
     }
+    L[g] = root;
+    T[g] = root;
   }
 }
 
-// Pass 2: unite across tile boundaries only (the forward half of the
-// neighbourhood, as in pass 1).  T holds every voxel's tile root and does not
-// change; unions start from tile roots, so only root entries of L move.  A
-// union (l, m) is skipped when the voxel before l along x (or, failing that,
-// along y) lies in the same tiles on both sides and joins the same pair of
-// tile roots: that predecessor's union, or its own predecessor's, performs
-// it.  This removes the repeated unions of one root pair along a tile face.
-__device__ __forceinline__ bool same_pair(const uint8_t* bin, const int* T, int64_t l, int64_t m,
-                                          int64_t e, int a, int b) {
-  return bin[l - e] && bin[m - e] && T[l - e] == a && T[m - e] == b;
+// Pass 2: unions across tile boundaries.  A pair of neighbouring foreground
+// voxels (l, m) in different tiles, with tile offset o = tile(m) - tile(l),
+// is united by l when o is positive in (z, y, x) lexicographic order and by
+// m otherwise, so every cross-tile pair is taken exactly once, and the voxel
+// taking it lies on the +x, +y or +z face of its tile.  One block per tile,
+// one thread per face voxel (+z face, then the rest of the +y face, then the
+// rest of the +x face).  Each thread reads the tile roots of its candidate
+// neighbours in one batch (T = -1 marks background), and the tile-root pairs
+// go through a block-wide hash set in LDS, so each distinct pair of tile
+// roots is united once per tile.  Unions start from the tile roots in T, so
+// only root entries of L move.
+constexpr int NFACE_Z = CX * CY;                // lz = CZ-1
+constexpr int NFACE_Y = CX * (CZ - 1);          // ly = CY-1, lz < CZ-1
+constexpr int NFACE_X = (CY - 1) * (CZ - 1);    // lx = CX-1, ly < CY-1, lz < CZ-1
+constexpr int NFACE = NFACE_Z + NFACE_Y + NFACE_X;  // 841
+constexpr int PAIR_SET = 2048;                  // slots of the per-tile pair set
+constexpr unsigned long long NO_PAIR = ~0ull;
+
+// true when (a, b) was not in the set yet (or the set is full: unite anyway)
+__device__ __forceinline__ bool pair_insert(unsigned long long* set, int a, int b) {
+  const unsigned long long key = ((unsigned long long)(unsigned)a << 32) | (unsigned)b;
+  unsigned h = ((unsigned)a * 0x9E3779B1u) ^ ((unsigned)b * 0x85EBCA77u);
+  for (int probe = 0; probe < PAIR_SET; ++probe) {
+    unsigned slot = (h + probe) & (PAIR_SET - 1);
+    unsigned long long old = atomicCAS(&set[slot], NO_PAIR, key);
+    if (old == NO_PAIR) return true;
+    if (old == key) return false;
+  }
+  return true;
 }
 
-__global__ void ccl_border_kernel(Box bx, const uint8_t* bin, int* L, const int* T, int conn) {
-  int64_t n = bx.size();
-  for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n;
-       l += (int64_t)gridDim.x * blockDim.x) {
-    if (!bin[l]) continue;
-    int64_t t = l / bx.nx;
-    int i = (int)(l - t * bx.nx), j = (int)(t % bx.ny), k = (int)(t / bx.ny);
-    int ti = i / CX, tj = j / CY, tk = k / CZ;
-    bool edge = (i % CX == 0) || (i % CX == CX - 1) || (j % CY == 0) || (j % CY == CY - 1) ||
-                (k % CZ == CZ - 1);
-    if (!edge) continue;
-    int a0 = T[l];
-    for (int dk = 0; dk <= 1; ++dk)
-      for (int dj = (dk ? -1 : 0); dj <= 1; ++dj)
-        for (int di = ((dk || dj) ? -1 : 1); di <= 1; ++di) {
-          if (conn == 6 && (abs(di) + abs(dj) + abs(dk)) != 1) continue;
-          int i2 = i + di, j2 = j + dj, k2 = k + dk;
-          if (i2 < 0 || i2 >= bx.nx || j2 < 0 || j2 >= bx.ny || k2 >= bx.nz) continue;
-          if (i2 / CX == ti && j2 / CY == tj && k2 / CZ == tk) continue;
-          int64_t m = bx.at(i2, j2, k2);
-          if (!bin[m]) continue;
-          int b0 = T[m];
-          if (a0 == b0) continue;
-          if (i % CX != 0 && i2 % CX != 0 && same_pair(bin, T, l, m, 1, a0, b0)) continue;
-          if (j % CY != 0 && j2 % CY != 0 && same_pair(bin, T, l, m, bx.nx, a0, b0)) continue;
-          uf_unite(L, a0, b0);
-        }
+__global__ __launch_bounds__(1024) void ccl_border_kernel(Box bx, int* L, const int* T, int conn) {
+  __shared__ unsigned long long set[PAIR_SET];
+  for (int e = threadIdx.x; e < PAIR_SET; e += blockDim.x) set[e] = NO_PAIR;
+  __syncthreads();
+  const int plane = bx.nx * bx.ny;
+  const int e = threadIdx.x;
+  if (e >= NFACE) return;
+  int lx, ly, lz;
+  if (e < NFACE_Z) { lx = e % CX; ly = e / CX; lz = CZ - 1; }
+  else if (e < NFACE_Z + NFACE_Y) { int f = e - NFACE_Z; lx = f % CX; ly = CY - 1; lz = f / CX; }
+  else { int f = e - NFACE_Z - NFACE_Y; lx = CX - 1; ly = f % (CY - 1); lz = f / (CY - 1); }
+  const int i = blockIdx.x * CX + lx, j = blockIdx.y * CY + ly, k = blockIdx.z * CZ + lz;
+  if (i >= bx.nx || j >= bx.ny || k >= bx.nz) return;
+  const int l = i + bx.nx * j + plane * k;
+  const int a0 = T[l];
+  if (a0 < 0) return;
+  int nb[26];
+#pragma unroll
+  for (int q = 0; q < 26; ++q) {
+    const int o = q < 13 ? q : q + 1;  // skip the centre of the 3x3x3 cube
+    const int di = o % 3 - 1, dj = (o / 3) % 3 - 1, dk = o / 9 - 1;
+    nb[q] = -1;
+    if (conn == 6 && (abs(di) + abs(dj) + abs(dk)) != 1) continue;
+    const int oz = (lz + dk) >= CZ ? 1 : ((lz + dk) < 0 ? -1 : 0);
+    const int oy = (ly + dj) >= CY ? 1 : ((ly + dj) < 0 ? -1 : 0);
+    const int ox = (lx + di) >= CX ? 1 : ((lx + di) < 0 ? -1 : 0);
+    if (!(oz > 0 || (oz == 0 && (oy > 0 || (oy == 0 && ox > 0))))) continue;
+    const int i2 = i + di, j2 = j + dj, k2 = k + dk;
+    if (i2 < 0 || i2 >= bx.nx || j2 < 0 || j2 >= bx.ny || k2 < 0 || k2 >= bx.nz) continue;
+    nb[q] = T[i2 + bx.nx * j2 + plane * k2];
+  }
+  int last = -1;
+#pragma unroll
+  for (int q = 0; q < 26; ++q) {
+    const int b0 = nb[q];
+    if (b0 < 0 || b0 == a0 || b0 == last) continue;
+    last = b0;
+    if (pair_insert(set, a0, b0)) uf_unite(L, a0, b0);
   }
 }
 
-__global__ void ccl_compress_kernel(int* L, int64_t n) {
-  for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n;
-       l += (int64_t)gridDim.x * blockDim.x) {
-    if (L[l] >= 0) L[l] = uf_find(L, (int)l);
+__global__ void ccl_compress_kernel(int* L, int n) {
+  for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < n; l += gridDim.x * blockDim.x) {
+    if (L[l] >= 0) L[l] = uf_find(L, l);
   }
 }
 
@@ -155,11 +206,11 @@ static int grid_for(int64_t n) {
 hipError_t launch_label_components(const Box& bx, const uint8_t* bin, int* L, int* T, int conn,
                                    hipStream_t st) {
   int64_t n = bx.size();
+  if (n >= ((int64_t)1 << 31)) return hipErrorInvalidValue;
   dim3 tiles((bx.nx + CX - 1) / CX, (bx.ny + CY - 1) / CY, (bx.nz + CZ - 1) / CZ);
-  hipLaunchKernelGGL(ccl_local_kernel, tiles, dim3(CX, CY, CZ), 0, st, bx, bin, L, T, conn);
-  hipLaunchKernelGGL(ccl_border_kernel, dim3(grid_for(n)), dim3(256), 0, st, bx, bin, L,
-                     (const int*)T, conn);
-  hipLaunchKernelGGL(ccl_compress_kernel, dim3(grid_for(n)), dim3(256), 0, st, L, n);
+  hipLaunchKernelGGL(ccl_local_kernel, tiles, dim3(CX, CY, 1), 0, st, bx, bin, L, T, conn);
+  hipLaunchKernelGGL(ccl_border_kernel, tiles, dim3(1024), 0, st, bx, L, (const int*)T, conn);
+  hipLaunchKernelGGL(ccl_compress_kernel, dim3(grid_for(n)), dim3(256), 0, st, L, (int)n);
   return hipGetLastError();
 }
 
