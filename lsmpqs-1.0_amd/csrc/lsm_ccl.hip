@@ -38,14 +38,15 @@ __device__ __forceinline__ void uf_unite(int* L, int a, int b) {
   }
 }
 
-// Pass 1: union-find inside one 32 x 16 x 8 tile in LDS (16 KB).  A block
-// of 32 x 16 threads owns the tile; each thread takes the 8 voxels of its z
-// column.  Local indices are ordered like global linear indices (x fastest,
+// Pass 1: union-find inside one 64 x 16 x 16 tile in LDS (64 KB).  A block
+// of 64 x 16 threads owns the tile (one wave per x row); each thread takes
+// the 16 voxels of its z column.  Local indices are ordered like global linear indices (x fastest,
 // then y, then z), so the local root -- the largest local index of the tile
 // component -- is also its largest global index.  Every voxel leaves
 // pointing straight at that root, in L (which pass 2 links further) and in T
 // (which stays: the tile root of every voxel).
-constexpr int CX = 32, CY = 16, CZ = 8, CN = CX * CY * CZ;
+constexpr int CX = 64, CY = 16, CZ = 16, CN = CX * CY * CZ;
+static_assert(CX == 64, "one wave per tile row");
 
 // root of x with path halving (each visited entry is re-pointed at its
 // grandparent, an ancestor, so concurrent finds and links stay consistent)
@@ -84,18 +85,35 @@ __global__ __launch_bounds__(CX * CY) void ccl_local_kernel(Box bx, const uint8_
 #pragma unroll
   for (int z = 0; z < CZ; ++z) {
     const int l = tx + CX * (ty + CY * z);
-    bool fg = col && kb + z < bx.nz && bin[g0 + (kb + z) * plane];
-    p[l] = fg ? l : -1;
+    const bool fg = col && kb + z < bx.nz && bin[g0 + (kb + z) * plane];
     fgm |= fg ? (1u << z) : 0u;
+    // x runs: every foreground voxel starts out pointing at the last voxel
+    // of its run of foreground voxels along the tile row
+    const unsigned long long row = __ballot(fg);
+    const unsigned long long ones = ~(row >> tx);  // first zero bit at or after this voxel
+    const int run = ones ? __ffsll((long long)ones) - 1 : CX - tx;
+    p[l] = fg ? l + run - 1 : -1;
   }
   __syncthreads();
+  // Unions with the forward neighbours of the 26- (6-) neighbourhood.  The
+  // +x neighbour is already joined by the run.  A union (l, l + (di, dj, dk))
+  // with di != 0 is implied when l + di is foreground (that voxel is joined
+  // to l by the run and to the target by the pair with di = 0); one with
+  // di = 0, dj != 0, dk = 1 is implied when l + dj (y) is foreground.
   for (int z = 0; z < CZ; ++z) {
     if (!(fgm >> z & 1u)) continue;
     const int l = tx + CX * (ty + CY * z);
+    const bool xm = tx > 0 && p[l - 1] >= 0, xp = tx < CX - 1 && p[l + 1] >= 0;
+    const bool ym = ty > 0 && p[l - CX] >= 0, yp = ty < CY - 1 && p[l + CX] >= 0;
     for (int dk = 0; dk <= 1; ++dk)
       for (int dj = (dk ? -1 : 0); dj <= 1; ++dj)
         for (int di = ((dk || dj) ? -1 : 1); di <= 1; ++di) {
+          if (dk == 0 && dj == 0) continue;  // +x: the run
           if (conn == 6 && (abs(di) + abs(dj) + abs(dk)) != 1) continue;
+          if (conn != 6) {
+            if ((di == 1 && xp) || (di == -1 && xm)) continue;
+            if (di == 0 && dk == 1 && ((dj == 1 && yp) || (dj == -1 && ym))) continue;
+          }
           int x2 = tx + di, y2 = ty + dj, z2 = z + dk;
           if (x2 < 0 || x2 >= CX || y2 < 0 || y2 >= CY || z2 >= CZ) continue;
           int m = x2 + CX * (y2 + CY * z2);
@@ -133,7 +151,7 @@ __global__ __launch_bounds__(CX * CY) void ccl_local_kernel(Box bx, const uint8_
 constexpr int NFACE_Z = CX * CY;                // lz = CZ-1
 constexpr int NFACE_Y = CX * (CZ - 1);          // ly = CY-1, lz < CZ-1
 constexpr int NFACE_X = (CY - 1) * (CZ - 1);    // lx = CX-1, ly < CY-1, lz < CZ-1
-constexpr int NFACE = NFACE_Z + NFACE_Y + NFACE_X;  // 841
+constexpr int NFACE = NFACE_Z + NFACE_Y + NFACE_X;  // 2209
 constexpr int PAIR_SET = 2048;                  // slots of the per-tile pair set
 constexpr unsigned long long NO_PAIR = ~0ull;
 
@@ -150,13 +168,9 @@ __device__ __forceinline__ bool pair_insert(unsigned long long* set, int a, int 
   return true;
 }
 
-__global__ __launch_bounds__(1024) void ccl_border_kernel(Box bx, int* L, const int* T, int conn) {
-  __shared__ unsigned long long set[PAIR_SET];
-  for (int e = threadIdx.x; e < PAIR_SET; e += blockDim.x) set[e] = NO_PAIR;
-  __syncthreads();
-  const int plane = bx.nx * bx.ny;
-  const int e = threadIdx.x;
-  if (e >= NFACE) return;
+// face job e of tile blockIdx
+__device__ __forceinline__ void border_job(const Box& bx, int* L, const int* T, int conn,
+                                           unsigned long long* set, int plane, int e) {
   int lx, ly, lz;
   if (e < NFACE_Z) { lx = e % CX; ly = e / CX; lz = CZ - 1; }
   else if (e < NFACE_Z + NFACE_Y) { int f = e - NFACE_Z; lx = f % CX; ly = CY - 1; lz = f / CX; }
@@ -189,6 +203,14 @@ __global__ __launch_bounds__(1024) void ccl_border_kernel(Box bx, int* L, const 
     last = b0;
     if (pair_insert(set, a0, b0)) uf_unite(L, a0, b0);
   }
+}
+
+__global__ __launch_bounds__(1024) void ccl_border_kernel(Box bx, int* L, const int* T, int conn) {
+  __shared__ unsigned long long set[PAIR_SET];
+  for (int e = threadIdx.x; e < PAIR_SET; e += blockDim.x) set[e] = NO_PAIR;
+  __syncthreads();
+  const int plane = bx.nx * bx.ny;
+  for (int e = threadIdx.x; e < NFACE; e += blockDim.x) border_job(bx, L, T, conn, set, plane, e);
 }
 
 __global__ void ccl_compress_kernel(int* L, int n) {
