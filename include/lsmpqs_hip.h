@@ -42,6 +42,16 @@ QSSLIB_REAL constCurvModel3d(Options *options, QSS_DataArrays *p, Grid *g, FILE 
 QSSLIB_REAL constCurvModel3dNoVar(Options *options, QSS_DataArrays *p, Grid *g, FILE *fp_out,
                                   QSSLIB_REAL a);
 
+/* ---- file-level drivers (reference: src/level_set_evolution/drain_top.h:28,
+ *      imbibe_top.h:28) ----
+ * Read phi (data), mask and grid files (.gz allowed), run drain_model3d /
+ * imbibe_model3d with the log in options->outfile, write
+ * options->path/data_final.gz when save_data.  Returns 0, or 1 on an input
+ * error, a mismatch of the array and grid dimensions, a 2D grid (only 3D is
+ * supported) or a device error (message in lsmpqs_hip_last_error()). */
+int drainTop(Options *options, char *fname_data_in, char *fname_grid_in, char *fname_mask_in);
+int imbibeTop(Options *options, char *fname_data_in, char *fname_grid_in, char *fname_mask_in);
+
 /* ---- model drivers (reference: src/level_set_evolution/drain_model3d.h:31,
  *      imbibe_model3d.h:32) ----
  * Curvature schedule a += b*dc up to amax (drain) / down to amin (imbibe).
@@ -73,6 +83,26 @@ Options *createOptionsDefault(void);
 void destroyOptions(Options *options);
 void writeDataArrayQSS(QSSLIB_REAL *data, Grid *grid, char *file_name, int zip_status);
 QSSLIB_REAL *readDataArrayQSS(int *grid_dims_ghostbox, char *file_name);
+/* option file "key value" lines (reference: src/utilities/qss_options.h:227,
+   qss_options.c:118-494; keys recognised by their first letters as there) */
+Options *createOptionsFromInputFile(char *file_name);
+void printOptions(Options *options, FILE *fp);
+
+/* ---- grids (reference: src/geometry/qss_grid.h:107-255) ----
+ * accuracy LOW / MEDIUM / HIGH / VERY_HIGH adds 2 / 3 / 5 / 4 ghost layers
+ * and sets the ENO1 / ENO2 / ENO3 / WENO5 fill box.  The binary file is the
+ * reference's field-by-field record (zip_status 1: gzip afterwards). */
+Grid *createGridSetDx(int num_dims, QSSLIB_REAL dx, QSSLIB_REAL *x_lo, QSSLIB_REAL *x_hi,
+                      QSSLIB_SPATIAL_DERIVATIVE_ACCURACY_TYPE accuracy);
+Grid *createGridSetDxDyDz(int num_dims, QSSLIB_REAL dx, QSSLIB_REAL dy, QSSLIB_REAL dz,
+                          QSSLIB_REAL *x_lo, QSSLIB_REAL *x_hi,
+                          QSSLIB_SPATIAL_DERIVATIVE_ACCURACY_TYPE accuracy);
+Grid *createGridSetGridDims(int num_dims, int *grid_dims, QSSLIB_REAL *x_lo, QSSLIB_REAL *x_hi,
+                            QSSLIB_SPATIAL_DERIVATIVE_ACCURACY_TYPE accuracy);
+void destroyGrid(Grid *grid);
+void printGrid(Grid *grid, FILE *fp);
+void writeGridToBinaryFile(Grid *grid, char *file_name, int zip_status);
+Grid *readGridFromBinaryFile(char *file_name);
 
 /* ---- extensions with no counterpart in the reference ---- */
 

@@ -40,6 +40,7 @@ Options* createOptionsDefault(void) {
   o->stop_touch = 1;
   o->eps_coefficient = (QSSLIB_REAL)1.5;
   o->extrapol_id = 0;
+  o->extrapol_func = (void (*)())signedLinearExtrapolationBCqss;
   o->checkpoint = 0;
   o->reservoir_inlet = 1;
   o->theta = 0;

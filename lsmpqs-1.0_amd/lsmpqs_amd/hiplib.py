@@ -35,6 +35,9 @@ EXPORTS = (
     "allocateMemoryForQSSDataArrays", "destroyQSSDataArrays", "lsmpqs_hip_last_error",
     "lsmpqs_set_dt_slabs", "lsmpqs_set_quirks", "lsmpqs_release", "lsmpqs_bench_inner_steps",
     "lsmpqs_download", "lsmpqs_model_prepare", "lsmpqs_session_stats",
+    "drainTop", "imbibeTop", "createOptionsFromInputFile", "printOptions", "createGridSetDx",
+    "createGridSetDxDyDz", "createGridSetGridDims", "destroyGrid", "printGrid",
+    "writeGridToBinaryFile", "readGridFromBinaryFile",
 )
 
 
