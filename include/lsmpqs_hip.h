@@ -69,6 +69,12 @@ void getMainInd_new(Options *o, QSS_DataArrays *p, Grid *g);
 
 /* ---- reinitialization (reference: src/reinitialization/reinitialize_top.h:28) ---- */
 void reinitializeSubcellFix3d(QSSLIB_REAL *data, Grid *g, Options *o);
+/* smoothed-sign reinitialization, TVD "RK2" until options->tmax_r
+   (reference: src/reinitialization/reinitialize_top.h:26, reinitialize_top.c:205-295).
+   Its HJ-ENO3 uses 1/6 where the reference's coefficient `sixth` is never
+   assigned (qss_reinitialization3d.f:109-110); on voxels that take none of
+   those branches the result is the reference's. */
+void reinitializeMedium3d(QSSLIB_REAL *data, Grid *g, Options *o);
 
 /* ---- setup and boundary helpers (reference: src/utilities/qss_general_util.h:28-37) ---- */
 void signedLinearExtrapolationBCqss(QSSLIB_REAL *phi, Grid *g, int bdry_location_idx);

@@ -70,6 +70,31 @@ static void reinit_dev(Session* s, real*& f, const Grid* g, const Options* o) {
   }
 }
 
+// Smoothed-sign reinitialization of f (reference:
+// src/reinitialization/reinitialize_top.c:205-295): "RK2" with dt_r = 0.5 dx
+// until tmax_r, the all-face BC after each stage.  Uses A (stage 1), B (next).
+static void reinit_medium_dev(Session* s, real*& f, const Grid* g, const Options* o) {
+  ReinitArgs ra{};
+  ra.bx = s->bx;
+  ra.dx = g->dx[0]; ra.dy = g->dx[1]; ra.dz = g->dx[2];
+  real t_r = 0, dt_r = real(0.5) * g->dx[0], tmax_r = o->tmax_r;
+  ra.dt = dt_r;
+  BcArgs ba{};
+  ba.bx = s->bx;
+  while (t_r < tmax_r) {
+    ra.in = f; ra.un = f; ra.out = s->A;
+    DCHECK(launch_reinit_smooth(ra, 1, s->st));
+    ba.src = s->A; ba.dst = s->A;
+    DCHECK(launch_bc(ba, false, s->st));
+    ra.in = s->A; ra.un = f; ra.out = s->B;
+    DCHECK(launch_reinit_smooth(ra, 2, s->st));
+    ba.src = s->B; ba.dst = s->B;
+    DCHECK(launch_bc(ba, false, s->st));
+    std::swap(f, s->B);
+    t_r = t_r + dt_r;
+  }
+}
+
 // Trapped-phase masks (reference: src/connectivity/trapComponents.c:156-375)
 static void trap_dev(Session* s, const Grid* g, const Options* o) {
   ensure_labels(s);
@@ -651,6 +676,17 @@ void reinitializeSubcellFix3d(QSSLIB_REAL* data, Grid* g, Options* o) {
   Session* s = session_for((const QSS_DataArrays*)data, g);
   up(s, s->D, data);
   reinit_dev(s, s->D, g, o);
+  down(s, data, s->D);
+  sync(s);
+  API_CATCH()
+}
+
+void reinitializeMedium3d(QSSLIB_REAL* data, Grid* g, Options* o) {
+  API_TRY
+  lsm::set_error("");
+  Session* s = session_for((const QSS_DataArrays*)data, g);
+  up(s, s->D, data);
+  reinit_medium_dev(s, s->D, g, o);
   down(s, data, s->D);
   sync(s);
   API_CATCH()

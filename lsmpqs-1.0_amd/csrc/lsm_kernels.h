@@ -87,6 +87,18 @@ hipError_t launch_coeff_reduce(const Box& bx, const real* b, const real* vx, con
                                const real* vz, real inv_dx, double* partials, double* out,
                                hipStream_t st);
 
+// Smoothed-sign reinitialization: stage 1 out = in + dt L(in), stage 2
+// out = 3/4 un + 1/4 (in + dt L(in)), fill box only.
+struct ReinitArgs {
+  Box bx;
+  const real* in;
+  const real* un;
+  real* out;
+  real dt, dx, dy, dz;
+  int tiles_x, tiles_y, tiles_z;  // set by the launcher
+};
+hipError_t launch_reinit_smooth(const ReinitArgs& a, int stage, hipStream_t st);
+
 // Subcell-fix reinitialization pieces.
 hipError_t launch_reinit_distance(const Box& bx, real* dist0, const real* phi0, real dx, real dy,
                                   real dz, hipStream_t st);

@@ -378,6 +378,125 @@ hipError_t launch_stage(const StageArgs& a, int scheme, bool var, int stage, boo
 }
 
 // ---------------------------------------------------------------------------
+// Smoothed-sign reinitialization, one fused right-hand side + RK stage
+// (right-hand side: qss_reinitialization3d.f:39-470 with use_phi0_for_sgn = 0;
+// stages: reinitialize_top.c:205-295).  HJ-ENO3 one-sided derivatives (the
+// minus sides of y and z scaled by 1/dx, as there; the 1/6 coefficient the
+// reference leaves undefined is 1/6), Godunov selection by the sign of phi,
+// rhs = S (1 - |grad phi|) with S = phi / sqrt(phi^2 + |grad phi|^2 dx_max^2),
+// 0 where |phi| < 1e-5.  Stage 1: out = in + dt rhs; stage 2:
+// out = 3/4 un + 1/4 (in + dt rhs).  Same tile walk as the stage kernel:
+// 64 x 8 columns marched along z, z line in registers, four LDS planes.
+template <int STAGE>
+__global__ __launch_bounds__(TX * TY) void reinit_smooth_kernel(ReinitArgs a) {
+  __shared__ real pl[NSLOT][PY * PX];
+  const Box bx = a.bx;
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
+  const int tile = xcd_tile(blockIdx.x, a.tiles_x * a.tiles_y * a.tiles_z);
+  const int tcol = tile % a.tiles_x, trow = (tile / a.tiles_x) % a.tiles_y;
+  const int tzc = tile / (a.tiles_x * a.tiles_y);
+  const int i0 = bx.ilo + tcol * TX, j0 = bx.jlo + trow * TY;
+  const int i = i0 + tx, j = j0 + ty;
+  const int zc = (bx.khi - bx.klo + a.tiles_z) / a.tiles_z;
+  const int k0 = bx.klo + tzc * zc;
+  const int k1 = min(k0 + zc - 1, bx.khi);
+  if (k0 > k1) return;
+  const bool active = (i <= bx.ihi) && (j <= bx.jhi);
+  const bool inbox = (i < bx.nx) && (j < bx.ny);
+  const int64_t plane = (int64_t)bx.nx * bx.ny;
+  const real* __restrict__ in = a.in;
+  const HaloSlot hs = halo_slot(bx, i0, j0, tid);
+  const int me = (ty + 3) * PX + tx + 3;
+  const real inv_dx = real(1.0) / a.dx, inv_dy = real(1.0) / a.dy, inv_dz = real(1.0) / a.dz;
+  real dx_sq = tmax(tmax(a.dx, a.dy), a.dz);
+  dx_sq = dx_sq * dx_sq;
+
+  const real* col = in + (inbox ? (int64_t)i + (int64_t)bx.nx * j : 0);
+  real w[7];
+#pragma unroll
+  for (int o = 0; o < 7; ++o) w[o] = real(0);
+  real wnext = real(0);
+  if (inbox) {
+#pragma unroll
+    for (int o = 0; o < 6; ++o) w[o] = col[(int64_t)(k0 - 3 + o) * plane];
+    wnext = col[(int64_t)(k0 + 3) * plane];
+  }
+  const real* hp = hs.off >= 0 ? in + hs.off : nullptr;
+  auto halo = [&](int k) { return hp ? hp[(int64_t)k * plane] : real(0); };
+  if (hs.lds >= 0) {
+    pl[0][hs.lds] = halo(k0 - 1);
+    pl[1][hs.lds] = halo(k0);
+  }
+  pl[0][me] = w[2];
+  pl[1][me] = w[3];
+  real hnext = hs.lds >= 0 ? halo(k0 + 1) : real(0);
+  __syncthreads();
+
+  int64_t idx = bx.at(i, j, k0);
+  for (int k = k0; k <= k1; ++k, idx += plane) {
+    const int sc = (k - k0 + 1) & 3, sp = (k - k0 + 2) & 3;
+    w[6] = wnext;
+    if (inbox && k + 4 < bx.nz) wnext = col[(int64_t)(k + 4) * plane];
+    pl[sp][me] = w[4];
+    if (hs.lds >= 0) {
+      pl[sp][hs.lds] = hnext;
+      hnext = halo(k + 2);
+    }
+    real unv = 0;
+    if (active && STAGE == 2) unv = a.un[idx];
+    __syncthreads();
+    if (active) {
+      const real* P = pl[sc];
+      real lx[7], ly[7];
+#pragma unroll
+      for (int o = 0; o < 7; ++o) {
+        lx[o] = P[me - 3 + o];
+        ly[o] = P[me + (o - 3) * PX];
+      }
+      real gp[3], gm[3];
+      eno3(lx, inv_dx, inv_dx, gp[0], gm[0]);
+      eno3(ly, inv_dy, inv_dx, gp[1], gm[1]);
+      eno3(w, inv_dz, inv_dx, gp[2], gm[2]);
+      const real c = lx[3];
+      real star[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        real p_, m_;
+        if (c > real(0)) { p_ = tmax(-gp[d], real(0)); m_ = tmax(gm[d], real(0)); }
+        else { p_ = tmax(gp[d], real(0)); m_ = tmax(-gm[d], real(0)); }
+        star[d] = tmax(p_, m_);
+      }
+      real rhs = real(0);
+      if (fabs(c) >= real(1.e-5)) {
+        real n2 = star[0] * star[0] + star[1] * star[1] + star[2] * star[2];
+        real sgn = c / sqrt(c * c + n2 * dx_sq);
+        rhs = sgn * (real(1.0) - sqrt(n2));
+      }
+      real val;
+      if (STAGE == 1) val = c + a.dt * rhs;
+      else val = real(0.75) * unv + real(0.25) * (c + a.dt * rhs);
+      a.out[idx] = val;
+    }
+#pragma unroll
+    for (int o = 0; o < 6; ++o) w[o] = w[o + 1];
+  }
+}
+
+hipError_t launch_reinit_smooth(const ReinitArgs& a, int stage, hipStream_t st) {
+  const Box& bx = a.bx;
+  int nxf = bx.ihi - bx.ilo + 1, nyf = bx.jhi - bx.jlo + 1, nzf = bx.khi - bx.klo + 1;
+  int gx = (nxf + TX - 1) / TX, gy = (nyf + TY - 1) / TY;
+  int cols = gx * gy;
+  int gz = max(1, min((2048 + cols - 1) / cols, (nzf + 7) / 8));
+  ReinitArgs t = a;
+  t.tiles_x = gx; t.tiles_y = gy; t.tiles_z = gz;
+  dim3 grid(gx * gy * gz), block(TX, TY, 1);
+  if (stage == 1) hipLaunchKernelGGL(reinit_smooth_kernel<1>, grid, block, 0, st, t);
+  else hipLaunchKernelGGL(reinit_smooth_kernel<2>, grid, block, 0, st, t);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Time-step probe: H/dx at (ihi_fb, jhi_fb, last plane of each slab), the
 // voxels whose value the reference's per-thread max_H_over_dX ends up holding
 // (qss_spatial_derivatives3d.f:490; constCurvModel3d.c:105-108,139-149).

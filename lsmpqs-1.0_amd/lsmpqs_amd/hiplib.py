@@ -37,7 +37,7 @@ EXPORTS = (
     "lsmpqs_download", "lsmpqs_model_prepare", "lsmpqs_session_stats",
     "drainTop", "imbibeTop", "createOptionsFromInputFile", "printOptions", "createGridSetDx",
     "createGridSetDxDyDz", "createGridSetGridDims", "destroyGrid", "printGrid",
-    "writeGridToBinaryFile", "readGridFromBinaryFile",
+    "writeGridToBinaryFile", "readGridFromBinaryFile", "reinitializeMedium3d",
 )
 
 

@@ -75,9 +75,19 @@ void orc_qss3dcomputereinitializationeqnrhssubcellfixorder1(
     orc_real *, const orc_real *, ORC_B6, const orc_real *, const orc_real *, ORC_B6,
     const orc_real *, const orc_real *, const orc_real *);
 
+void orc_computereinitializationeqnrhs3d(orc_real *, const orc_real *, const orc_real *, ORC_B6,
+                                         ORC_B6, const orc_real *, const orc_real *,
+                                         const orc_real *, const int *, const orc_real *);
+
 /* driver level (qss_oracle_driver.c) */
 void orc_signedLinearExtrapolationBCqss(orc_real *phi, Grid *g, int bdry_location_idx);
 void orc_reinitializeSubcellFix3d(orc_real *data, Grid *g, Options *o);
+/* smoothed-sign reinitialization; sixth: value of the reference's undefined
+   ENO3 coefficient (see orc_computereinitializationeqnrhs3d).  Pinned against
+   the reference build only on voxels whose ENO3 takes no branch using it; on
+   the others the reference's result is undefined behaviour (parity with the
+   reference unpinned there; closed forms pin the value 1/6). */
+void orc_reinitializeMedium3d(orc_real *data, Grid *g, Options *o, orc_real sixth);
 void orc_findConnectivity3d(QSS_DataArrays *p, Grid *g);
 void orc_getMainInd_new(Options *o, QSS_DataArrays *p, Grid *g);
 void orc_trapComponents_mask(QSS_DataArrays *p, Grid *g, Options *o);

@@ -59,6 +59,31 @@ void orc_reinitializeSubcellFix3d(R *data, Grid *g, Options *o)
   free(dist0); free(copy); free(rhs); free(next);
 }
 
+/* reference: src/reinitialization/reinitialize_top.c:205-295 -- TVD "RK2"
+   (stage 2 = 3/4 u + 1/4 (u1 + dt L)) of the smoothed-sign equation with
+   dt_r = 0.5 dx until tmax_r, both stages followed by the all-face BC */
+void orc_reinitializeMedium3d(R *data, Grid *g, Options *o, R sixth)
+{
+  long n = npts(g), l;
+  R *data0 = ralloc(g), *rhs = ralloc(g), *next = ralloc(g), *stage1 = ralloc(g);
+  R t_r = 0, dt_r = (R)0.5 * g->dx[0], tmax_r = o->tmax_r;
+  int use0 = 0;
+  memcpy(data0, data, (size_t)n * sizeof(R));
+  while (t_r < tmax_r) {
+    orc_computereinitializationeqnrhs3d(rhs, data, data0, GB6(g), FB6(g), &g->dx[0], &g->dx[1],
+                                        &g->dx[2], &use0, &sixth);
+    orc_qss3drk1step(stage1, GB6(g), data, GB6(g), rhs, GB6(g), FB6(g), &dt_r);
+    orc_signedLinearExtrapolationBCqss(stage1, g, 9);
+    orc_computereinitializationeqnrhs3d(rhs, stage1, data0, GB6(g), FB6(g), &g->dx[0], &g->dx[1],
+                                        &g->dx[2], &use0, &sixth);
+    orc_qss3dtvdrk2stage2(next, GB6(g), stage1, GB6(g), data, GB6(g), rhs, GB6(g), FB6(g), &dt_r);
+    orc_signedLinearExtrapolationBCqss(next, g, 9);
+    for (l = 0; l < n; l++) data[l] = next[l];
+    t_r = t_r + dt_r;
+  }
+  free(data0); free(rhs); free(next); free(stage1);
+}
+
 /* Union-find over the whole ghost box with the 26-neighbourhood; every
    component is labelled with its largest linear index, which is the root the
    reference's union rule produces (reference:

@@ -19,6 +19,11 @@
 
 typedef QSSLIB_REAL R;
 
+/* square root in the working precision: the reference's Fortran takes the
+   square root of a default real in that real's precision, so the FP32 build
+   must not round through double */
+static R wsqrt(R x) { return sizeof(R) == 4 ? (R)sqrtf((float)x) : (R)sqrt((double)x); }
+
 /* ghost-box addressing shared by all kernels */
 typedef struct { int ilo, jlo, klo, nx, nxy; long n; } box_t;
 
@@ -63,6 +68,39 @@ static void eno2_1d(const R *p, R inv_d, R *plus, R *minus)
 static void eno3_1d(const R *p, R inv_plus, R inv_minus, R *plus, R *minus)
 {
   const R half = (R)0.5, third = (R)(1.0 / 3.0), sixth = (R)(1.0 / 6.0);
+  R D1 = p[3] - p[2], D1p = p[4] - p[3], D1m = p[2] - p[1];
+  R D1mm = p[1] - p[0], D1pp = p[5] - p[4], D1ppp = p[6] - p[5];
+  R D2 = D1p - D1, D2p = D1pp - D1p, D2m = D1 - D1m;
+  R D2pp = D1ppp - D1pp, D2mm = D1m - D1mm;
+  R D3 = D2 - D2m, D3p = D2p - D2, D3pp = D2pp - D2p, D3m = D2m - D2mm;
+  R v;
+  v = D1p;
+  if (fabs(D2) < fabs(D2p)) {
+    v = v - half * D2;
+    if (fabs(D3) < fabs(D3p)) v = v - sixth * D3; else v = v - sixth * D3p;
+  } else {
+    v = v - half * D2p;
+    if (fabs(D3p) < fabs(D3pp)) v = v + third * D3p; else v = v + third * D3pp;
+  }
+  *plus = v * inv_plus;
+  v = D1;
+  if (fabs(D2m) < fabs(D2)) {
+    v = v + half * D2m;
+    if (fabs(D3m) < fabs(D3)) v = v + third * D3m; else v = v + third * D3;
+  } else {
+    v = v + half * D2;
+    if (fabs(D3) < fabs(D3p)) v = v - sixth * D3; else v = v - sixth * D3p;
+  }
+  *minus = v * inv_minus;
+}
+
+/* The same ENO3 with the 1/6 coefficient as a parameter: the reference's
+   smoothed-sign reinitialization declares `sixth` without a value
+   (qss_reinitialization3d.f:109-110), so what it multiplies is not fixed by
+   the source. */
+static void eno3_1d_sixth(const R *p, R inv_plus, R inv_minus, R sixth, R *plus, R *minus)
+{
+  const R half = (R)0.5, third = (R)(1.0 / 3.0);
   R D1 = p[3] - p[2], D1p = p[4] - p[3], D1m = p[2] - p[1];
   R D1mm = p[1] - p[0], D1pp = p[5] - p[4], D1ppp = p[6] - p[5];
   R D2 = D1p - D1, D2p = D1pp - D1p, D2m = D1 - D1m;
@@ -151,7 +189,7 @@ static R h_over_dx(R vn, R xp, R xm, R yp, R ym, R zp, R zm,
   R xc = rmax(fabs(xp), fabs(xm));
   R yc = rmax(fabs(yp), fabs(ym));
   R zc = rmax(fabs(zp), fabs(zm));
-  R ng = sqrt(xc * xc + yc * yc + zc * zc + max_dx_sq);
+  R ng = wsqrt(xc * xc + yc * yc + zc * zc + max_dx_sq);
   return fabs(vn) / ng * (xc * inv_dx + yc * inv_dy + zc * inv_dz);
 }
 
@@ -248,7 +286,7 @@ static void rhs_body(const R *phi, const R *vn_arr, R vn_s, const R *b_arr, R b_
         vn = vn_arr ? AT(vn_arr, bx, i, j, k) : vn_s;
         if (fabs(vn) >= (R)1.e-5) {
           R s = godunov_sq(vn, d[0], d[1], d[2], d[3], d[4], d[5]);
-          temp_norm = norm_sign * vn * sqrt(s);
+          temp_norm = norm_sign * vn * wsqrt(s);
         }
         H = h_over_dx(vn, d[0], d[1], d[2], d[3], d[4], d[5], dx, dy, dz,
                       inv_dx, inv_dy, inv_dz);
@@ -574,9 +612,9 @@ static void coeff_common(const R *mask, const R *mx, const R *my, const R *mz, b
   orc_qssheaviside(h2, &m, &eps_var);
   *absgrad = AT(mx, bx, i, j, k) * AT(mx, bx, i, j, k) + AT(my, bx, i, j, k) * AT(my, bx, i, j, k)
            + AT(mz, bx, i, j, k) * AT(mz, bx, i, j, k);
-  *absgrad = sqrt(*absgrad);
+  *absgrad = wsqrt(*absgrad);
   x = m * m + *absgrad * dx * *absgrad * dx;
-  x = sqrt(x);
+  x = wsqrt(x);
   *s = m / x;
 }
 
@@ -684,12 +722,58 @@ void orc_qss3dcomputereinitializationeqnrhssubcellfixorder1(
           }
           if (fabs(c) > (R)1.e-5) {
             R n2 = star[0] * star[0] + star[1] * star[1] + star[2] * star[2];
-            AT(rhs, bx, i, j, k) = sgn0 - sgn0 * sqrt(n2);
+            AT(rhs, bx, i, j, k) = sgn0 - sgn0 * wsqrt(n2);
           } else {
             AT(rhs, bx, i, j, k) = 0;
           }
         } else {
           AT(rhs, bx, i, j, k) = AT(dist0, bx, i, j, k) - sgn0 * fabs(c) / max_dx;
+        }
+      }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Smoothed-sign reinitialization right-hand side
+   (reference: src/reinitialization/qss_reinitialization3d.f:39-470):
+   HJ-ENO3 one-sided derivatives (minus sides of y and z scaled by 1/dx, as
+   there), Godunov selection by the sign of phi (or phi0), then
+   S (1 - |grad phi|) with S = phi / sqrt(phi^2 + |grad phi|^2 dx_max^2)
+   (with phi0: phi0 / sqrt(phi0^2 + dx_max^2)); 0 where |phi| < 1e-5.
+   `sixth` is the value the reference's undefined coefficient takes. */
+void orc_computereinitializationeqnrhs3d(R *rhs, const R *phi, const R *phi0, GBARGS, FBARGS,
+                                         const R *dx, const R *dy, const R *dz,
+                                         const int *use_phi0_for_sgn, const R *sixth)
+{
+  box_t bx = GB;
+  R inv_dx = (R)1.0 / *dx, inv_dy = (R)1.0 / *dy, inv_dz = (R)1.0 / *dz;
+  R dx_sq = rmax3(*dx, *dy, *dz);
+  dx_sq = dx_sq * dx_sq;
+  int i, j, k, o, d;
+  for (k = *klo_fb; k <= *khi_fb; k++)
+    for (j = *jlo_fb; j <= *jhi_fb; j++)
+      for (i = *ilo_fb; i <= *ihi_fb; i++) {
+        R lx[7], ly[7], lz[7], gp[3], gm[3], star[3];
+        for (o = 0; o < 7; o++) {
+          lx[o] = AT(phi, bx, i + o - 3, j, k);
+          ly[o] = AT(phi, bx, i, j + o - 3, k);
+          lz[o] = AT(phi, bx, i, j, k + o - 3);
+        }
+        eno3_1d_sixth(lx, inv_dx, inv_dx, *sixth, &gp[0], &gm[0]);
+        eno3_1d_sixth(ly, inv_dy, inv_dx, *sixth, &gp[1], &gm[1]);
+        eno3_1d_sixth(lz, inv_dz, inv_dx, *sixth, &gp[2], &gm[2]);
+        const int use0 = *use_phi0_for_sgn == 1;
+        R c = use0 ? AT(phi0, bx, i, j, k) : AT(phi, bx, i, j, k);
+        for (d = 0; d < 3; d++) {
+          if (c > 0) { gp[d] = rmax(-gp[d], 0); gm[d] = rmax(gm[d], 0); }
+          else       { gp[d] = rmax(gp[d], 0);  gm[d] = rmax(-gm[d], 0); }
+          star[d] = rmax(gp[d], gm[d]);
+        }
+        if (fabs(c) >= (R)1.e-5) {
+          R n2 = star[0] * star[0] + star[1] * star[1] + star[2] * star[2];
+          R sgn = use0 ? c / wsqrt(c * c + dx_sq) : c / wsqrt(c * c + n2 * dx_sq);
+          AT(rhs, bx, i, j, k) = sgn * ((R)1.0 - wsqrt(n2));
+        } else {
+          AT(rhs, bx, i, j, k) = 0;
         }
       }
 }
