@@ -129,6 +129,32 @@ int lsmpqs_set_dt_slabs(int nslabs);
    reciprocal per reconstruction, shared z faces) and FMA-contracted curvature. */
 unsigned lsmpqs_set_quirks(unsigned mask);
 
+/* Batched drain_model3d (drain != 0) / imbibe_model3d of n independent boxes
+   (subvolumes of any size, each with its own QSS_DataArrays, Grid and
+   Options, hence its own curvature schedule).  No counterpart in the
+   reference, which runs one box per process.  Each box runs the single-box
+   driver on a host thread of its own; the inner time steps of all boxes
+   waiting for one run together: a batched time-step probe, per RK stage one
+   persistent stage kernel fed by a work queue of 64 x 8 x zc voxel bricks of
+   all boxes, and a batched ghost extrapolation, captured once as a HIP graph
+   and replayed.  zc and the number of persistent workgroups are picked on
+   the first step by timing.  Results equal the boxes' individual runs bit
+   for bit.  All boxes must share the spatial scheme, the time order and the
+   contact-angle path (theta > 0 or not).  fp_out: one log per box or NULL;
+   no data_step_N files are written.  Returns 0, or -2 with the message in
+   lsmpqs_hip_last_error(). */
+typedef struct {
+  int boxes;
+  long long batched_steps;    /* batched inner steps (graph launches) */
+  long long box_steps;        /* inner steps summed over the boxes */
+  int brick_depth;            /* z planes per brick chosen by the tuner */
+  int workgroups;             /* persistent workgroups chosen by the tuner */
+  double batched_ms;          /* GPU time of the batched steps */
+  double autotune_best_ms;    /* fastest first-stage launch seen while tuning */
+} lsmpqs_batch_stats;
+int lsmpqs_batch_model3d(int n, Options **options, QSS_DataArrays **p, Grid **g, FILE **fp_out,
+                         int drain, lsmpqs_batch_stats *stats);
+
 /* free the device state kept for record p */
 void lsmpqs_release(QSS_DataArrays *p);
 

@@ -1,0 +1,330 @@
+// lsm_batch.hip -- many independent boxes (subvolumes of uneven size, each
+// with its own options and curvature schedule) drained or imbibed together.
+//
+// Each box runs the ordinary model driver on a host thread of its own, with
+// its own device session and stream, so every outer-step operation
+// (trapping, reinitialization, reductions, schedule) is the single-box code.
+// Its inner time steps are handed to the scheduler on the calling thread
+// instead: once every live box is waiting for an inner step, one batched
+// step runs for all of them -- a batched time-step probe, then per RK stage
+// one persistent stage kernel that takes 64 x 8 x zc bricks of all boxes
+// from a work queue, and one batched ghost extrapolation.  That sequence is
+// captured once as a HIP graph; each step only uploads the per-box records
+// (pointers, geometry, brick and ghost prefix sums) and replays the graph.
+// The brick depth zc and the number of persistent workgroups are picked on
+// the first step by timing the first stage over the actual boxes.  Every
+// voxel sees exactly the arithmetic of the single-box path, so a batched run
+// reproduces the boxes' individual runs bit for bit.
+#include <math.h>
+#include <stdio.h>
+#include <algorithm>
+#include <condition_variable>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+#include "lsm_batch.h"
+#include "lsmpqs_hip.h"
+
+namespace lsm {
+namespace {
+
+#define BCHECK(x)                                                                \
+  do {                                                                           \
+    hipError_t e_ = (x);                                                         \
+    if (e_ != hipSuccess)                                                        \
+      throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr int kCounters = 4;
+
+struct Request {
+  Session* s;
+  StepCfg c;
+  real dt = 0;
+  bool done = false;
+};
+
+class Batch final : public BatchHook {
+ public:
+  explicit Batch(int nbox) : live_(nbox), nmax_(nbox) {}
+  ~Batch() override { release(); }
+
+  // box thread: one inner step of box s
+  real step(Session* s, const StepCfg& c) override {
+    BCHECK(hipStreamSynchronize(s->st));
+    Request r{s, c};
+    std::unique_lock<std::mutex> lk(mu_);
+    if (!failure_.empty()) throw std::runtime_error(failure_);
+    parked_.push_back(&r);
+    cv_sched_.notify_one();
+    cv_box_.wait(lk, [&] { return r.done; });
+    if (!failure_.empty()) throw std::runtime_error(failure_);
+    return r.dt;
+  }
+
+  void box_finished() {
+    std::lock_guard<std::mutex> lk(mu_);
+    live_ -= 1;
+    cv_sched_.notify_one();
+  }
+
+  // calling thread: serve rounds until every box has finished
+  void run() {
+    for (;;) {
+      std::vector<Request*> round;
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_sched_.wait(lk, [&] { return (int)parked_.size() == live_; });
+        if (live_ == 0) break;
+        round.swap(parked_);
+      }
+      try {
+        launch(round);
+      } catch (const std::exception& e) {
+        std::lock_guard<std::mutex> lk(mu_);
+        failure_ = std::string("batched step: ") + e.what();
+      }
+      std::lock_guard<std::mutex> lk(mu_);
+      for (Request* r : round) r->done = true;
+      cv_box_.notify_all();
+    }
+  }
+
+  lsmpqs_batch_stats stats{};
+
+ private:
+  void configure(const Request& first) {
+    scheme_ = first.c.scheme;
+    var_ = first.c.var;
+    order_ = first.c.order;
+    fast_ = sizeof(real) == 4 && !(quirks() & EXACT_FP32);
+    nstage_ = order_ <= 1 ? 1 : (order_ == 2 ? 2 : 3);
+    BCHECK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+    BCHECK(hipMalloc((void**)&d_recs_, sizeof(BatchRec) * nmax_));
+    BCHECK(hipMalloc((void**)&d_count_, sizeof(int) * 4));
+    BCHECK(hipMalloc((void**)&d_brick_, sizeof(int) * (nmax_ + 1)));
+    BCHECK(hipMalloc((void**)&d_ghost_, sizeof(int64_t) * (nmax_ + 1)));
+    BCHECK(hipMalloc((void**)&d_counters_, sizeof(int) * kCounters));
+    BCHECK(hipMalloc((void**)&d_dts_, sizeof(real) * nmax_));
+    BCHECK(hipHostMalloc((void**)&h_recs_, sizeof(BatchRec) * nmax_, hipHostMallocDefault));
+    BCHECK(hipHostMalloc((void**)&h_count_, sizeof(int) * 4, hipHostMallocDefault));
+    BCHECK(hipHostMalloc((void**)&h_brick_, sizeof(int) * (nmax_ + 1), hipHostMallocDefault));
+    BCHECK(hipHostMalloc((void**)&h_ghost_, sizeof(int64_t) * (nmax_ + 1), hipHostMallocDefault));
+    BCHECK(hipHostMalloc((void**)&h_dts_, sizeof(real) * nmax_, hipHostMallocDefault));
+    BCHECK(hipEventCreate(&ev0_));
+    BCHECK(hipEventCreate(&ev1_));
+    view_.recs = d_recs_;
+    view_.count = d_count_;
+    view_.brick_prefix = d_brick_;
+    view_.ghost_prefix = d_ghost_;
+    view_.counters = d_counters_;
+    view_.dts = d_dts_;
+    int dev = 0, cus = 256;
+    BCHECK(hipGetDevice(&dev));
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      cus = 256;
+    cus_ = cus;
+    bc_grid_ = 4 * cus_;
+  }
+
+  void release() {
+    if (exec_) (void)hipGraphExecDestroy(exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    void* d[] = {d_recs_, d_count_, d_brick_, d_ghost_, d_counters_, d_dts_};
+    for (void* p : d)
+      if (p) (void)hipFree(p);
+    void* h[] = {h_recs_, h_count_, h_brick_, h_ghost_, h_dts_};
+    for (void* p : h)
+      if (p) (void)hipHostFree(p);
+    if (ev0_) (void)hipEventDestroy(ev0_);
+    if (ev1_) (void)hipEventDestroy(ev1_);
+    if (st_) (void)hipStreamDestroy(st_);
+  }
+
+  // records of the round's boxes for brick depth zc, copied to the device
+  void upload(const std::vector<Request*>& round, int zc) {
+    const int n = (int)round.size();
+    h_count_[0] = n;
+    h_brick_[0] = 0;
+    h_ghost_[0] = 0;
+    for (int i = 0; i < n; ++i) {
+      StepArgs a = make_step_args(round[i]->s, round[i]->c);
+      BatchRec& r = h_recs_[i];
+      r.probe = a.probe;
+      int bricks = 0;
+      for (int k = 0; k < 3; ++k) {
+        r.stage[k] = a.stage[k];
+        r.bc[k] = a.bc[k];
+        bricks = stage_bricks(r.stage[k], zc);
+      }
+      h_brick_[i + 1] = h_brick_[i] + bricks;
+      h_ghost_[i + 1] = h_ghost_[i] + ghost_count(a.probe.bx);
+    }
+    BCHECK(hipMemcpyAsync(d_recs_, h_recs_, sizeof(BatchRec) * n, hipMemcpyHostToDevice, st_));
+    BCHECK(hipMemcpyAsync(d_count_, h_count_, sizeof(int), hipMemcpyHostToDevice, st_));
+    BCHECK(hipMemcpyAsync(d_brick_, h_brick_, sizeof(int) * (n + 1), hipMemcpyHostToDevice, st_));
+    BCHECK(hipMemcpyAsync(d_ghost_, h_ghost_, sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, st_));
+  }
+
+  void enqueue_step() {
+    BCHECK(launch_batch_reset(d_counters_, kCounters, st_));
+    BCHECK(launch_probe_batch(view_, scheme_, nmax_, st_));
+    BCHECK(hipMemcpyAsync(h_dts_, d_dts_, sizeof(real) * nmax_, hipMemcpyDeviceToHost, st_));
+    for (int k = 0; k < nstage_; ++k) {
+      const bool fin = k == nstage_ - 1;
+      BCHECK(launch_stage_batch(view_, k, scheme_, var_, k + 1, fin, fast_, grid_, st_));
+      BCHECK(launch_bc_batch(view_, k, fin, bc_grid_, st_));
+    }
+  }
+
+  // brick depth and workgroup count: fastest first stage over this round
+  void autotune(const std::vector<Request*>& round) {
+    const int depths[] = {8, 16, 32, 64};
+    const int per_cu[] = {1, 2, 4};
+    float best = 1e30f;
+    upload(round, depths[0]);
+    BCHECK(launch_probe_batch(view_, scheme_, nmax_, st_));  // stage 1 reads this step's dt
+    for (int zc : depths) {
+      upload(round, zc);
+      for (int pc : per_cu) {
+        const int grid = pc * cus_;
+        float t = 1e30f;
+        for (int rep = 0; rep < 2; ++rep) {
+          BCHECK(launch_batch_reset(d_counters_, kCounters, st_));
+          BCHECK(hipEventRecord(ev0_, st_));
+          BCHECK(launch_stage_batch(view_, 0, scheme_, var_, 1, nstage_ == 1, fast_, grid, st_));
+          BCHECK(hipEventRecord(ev1_, st_));
+          BCHECK(hipEventSynchronize(ev1_));
+          float ms = 0;
+          BCHECK(hipEventElapsedTime(&ms, ev0_, ev1_));
+          t = std::min(t, ms);
+        }
+        if (t < best) {
+          best = t;
+          zc_ = zc;
+          grid_ = grid;
+        }
+      }
+    }
+    // order 1 wrote the imposed result into B; the real step recomputes it
+    stats.brick_depth = zc_;
+    stats.workgroups = grid_;
+    stats.autotune_best_ms = best;
+  }
+
+  void capture() {
+    BCHECK(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
+    try {
+      enqueue_step();
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(st_, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    BCHECK(hipStreamEndCapture(st_, &graph_));
+    BCHECK(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
+  }
+
+  void launch(const std::vector<Request*>& round) {
+    if (!st_) configure(*round[0]);
+    for (Request* r : round)
+      if (r->c.scheme != scheme_ || r->c.var != var_ || r->c.order != order_)
+        throw std::runtime_error(
+            "all boxes of a batch need the same spatial scheme, time order and contact-angle path");
+    if (zc_ == 0) autotune(round);
+    upload(round, zc_);
+    if (!exec_) capture();
+    BCHECK(hipEventRecord(ev0_, st_));
+    BCHECK(hipGraphLaunch(exec_, st_));
+    BCHECK(hipEventRecord(ev1_, st_));
+    BCHECK(hipStreamSynchronize(st_));
+    float ms = 0;
+    BCHECK(hipEventElapsedTime(&ms, ev0_, ev1_));
+    stats.batched_steps += 1;
+    stats.batched_ms += ms;
+    stats.box_steps += (long long)round.size();
+    for (size_t i = 0; i < round.size(); ++i) round[i]->dt = h_dts_[i];
+  }
+
+  std::mutex mu_;
+  std::condition_variable cv_sched_, cv_box_;
+  std::vector<Request*> parked_;
+  int live_;
+  std::string failure_;
+
+  int nmax_;
+  int scheme_ = 0, order_ = 0, nstage_ = 0;
+  bool var_ = false, fast_ = false;
+  int cus_ = 256, zc_ = 0, grid_ = 0, bc_grid_ = 1024;
+  hipStream_t st_ = nullptr;
+  hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t exec_ = nullptr;
+  BatchView view_{};
+  BatchRec* d_recs_ = nullptr;
+  int *d_count_ = nullptr, *d_brick_ = nullptr, *d_counters_ = nullptr;
+  int64_t* d_ghost_ = nullptr;
+  real* d_dts_ = nullptr;
+  BatchRec* h_recs_ = nullptr;
+  int *h_count_ = nullptr, *h_brick_ = nullptr;
+  int64_t* h_ghost_ = nullptr;
+  real* h_dts_ = nullptr;
+};
+
+}  // namespace
+}  // namespace lsm
+
+extern "C" int lsmpqs_batch_model3d(int n, Options** options, QSS_DataArrays** p, Grid** g,
+                                    FILE** fp_out, int drain, lsmpqs_batch_stats* stats) {
+  using namespace lsm;
+  set_error("");
+  if (n < 1 || !options || !p || !g) {
+    set_error("lsmpqs_batch_model3d: n >= 1 records with options, arrays and grids required");
+    return -2;
+  }
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < i; ++j)
+      if (p[i] == p[j] || options[i] == options[j]) {
+        set_error("lsmpqs_batch_model3d: every box needs its own QSS_DataArrays and Options");
+        return -2;
+      }
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    set_error("lsmpqs_batch_model3d: no HIP device");
+    return -2;
+  }
+  Batch batch(n);
+  std::vector<std::string> errors(n);
+  std::vector<std::thread> threads;
+  threads.reserve(n);
+  for (int i = 0; i < n; ++i) {
+    threads.emplace_back([&, i] {
+      try {
+        if (hipSetDevice(dev) != hipSuccess) throw std::runtime_error("hipSetDevice failed");
+        batch_hook() = &batch;
+        model_run(options[i], p[i], g[i], fp_out ? fp_out[i] : nullptr, drain != 0);
+      } catch (const std::exception& e) {
+        errors[i] = e.what();
+      }
+      batch_hook() = nullptr;
+      batch.box_finished();
+    });
+  }
+  batch.run();
+  for (auto& t : threads) t.join();
+  if (stats) {
+    *stats = batch.stats;
+    stats->boxes = n;
+  }
+  for (int i = 0; i < n; ++i)
+    if (!errors[i].empty()) {
+      std::string m = "box " + std::to_string(i) + ": " + errors[i];
+      set_error(m);
+      fprintf(stderr, "lsmpqs_hip: %s\n", m.c_str());
+      return -2;
+    }
+  return 0;
+}

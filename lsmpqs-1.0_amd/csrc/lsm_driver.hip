@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <stdexcept>
 #include <vector>
+#include "lsm_batch.h"
 #include "lsm_session.h"
 #include "lsmpqs_hip.h"
 
@@ -175,16 +176,6 @@ static void build_stale(Session* s, const Grid* g) {
 // kernel followed by the ghost extrapolation), mask imposition fused into the
 // last stage.  Returns nothing; dt lands in s->dt (device) and s->h_dt
 // (pinned, valid after ev_dt).
-struct StepCfg {
-  int scheme;
-  bool var;
-  int order;
-  real a_const, b_const;
-  real norm_sign;
-  real bmax, umax, overlap;
-  int conn, slabs, quirk_eno3;
-  real dx, dy, dz;
-};
 
 static void stage_timed(Session* s, const StageArgs& a, const StepCfg& c, int stage, bool fin) {
   if (s->profile) DCHECK(hipEventRecord(s->stage_ev[2 * stage], s->st));
@@ -194,15 +185,13 @@ static void stage_timed(Session* s, const StageArgs& a, const StepCfg& c, int st
   }
 }
 
-static void inner_step(Session* s, const StepCfg& c) {
+StepArgs make_step_args(Session* s, const StepCfg& c) {
   const Box& bx = s->bx;
-  ProbeArgs pa{};
+  StepArgs a{};
+  ProbeArgs& pa = a.probe;
   pa.bx = bx; pa.phi = s->phi; pa.vn = c.var ? s->vn : nullptr; pa.a_const = c.a_const;
   pa.dx = c.dx; pa.dy = c.dy; pa.dz = c.dz; pa.eno3_quirk = c.quirk_eno3; pa.nslabs = c.slabs;
   pa.bmax_over_dx = c.bmax; pa.umax_over_dx = c.umax; pa.dt_out = s->dt;
-  DCHECK(launch_probe(pa, c.scheme, s->st));
-  DCHECK(hipMemcpyAsync(s->h_dt, s->dt, sizeof(real), hipMemcpyDeviceToHost, s->st));
-  DCHECK(hipEventRecord(s->ev_dt, s->st));
 
   StageArgs sa{};
   sa.bx = bx; sa.dt = s->dt;
@@ -216,35 +205,40 @@ static void inner_step(Session* s, const StepCfg& c) {
   sa.fast = (sizeof(real) == 4 && !(quirks() & EXACT_FP32)) ? 1 : 0;
   BcArgs bc{};
   bc.bx = bx;
-
+  a.order = c.order;
+  for (int i = 0; i < 3; ++i) { a.stage[i] = sa; a.bc[i] = bc; }
   if (c.order <= 1) {
-    // phi_new = max(phi + dt L(phi), masks) into B; face layers into A
-    sa.in = s->phi; sa.un = s->phi; sa.out = s->B; sa.pre = s->A;
-    stage_timed(s, sa, c, 1, true);
-    bc.src = s->A; bc.dst = s->B; bc.floor = s->floor;
-    DCHECK(launch_bc(bc, true, s->st));
-    std::swap(s->phi, s->B);
-    return;
+    // phi_new = max(phi + dt L(phi), floor) into B; face layers into A
+    a.stage[0].in = s->phi; a.stage[0].un = s->phi; a.stage[0].out = s->B; a.stage[0].pre = s->A;
+    a.bc[0].src = s->A; a.bc[0].dst = s->B; a.bc[0].floor = s->floor;
+    return a;
   }
-  sa.in = s->phi; sa.un = s->phi; sa.out = s->A; sa.pre = nullptr;
-  stage_timed(s, sa, c, 1, false);
-  bc.src = s->A; bc.dst = s->A;
-  DCHECK(launch_bc(bc, false, s->st));
+  a.stage[0].in = s->phi; a.stage[0].un = s->phi; a.stage[0].out = s->A; a.stage[0].pre = nullptr;
+  a.bc[0].src = s->A; a.bc[0].dst = s->A;
   if (c.order == 2) {
-    sa.in = s->A; sa.un = s->phi; sa.out = s->phi; sa.pre = s->B;
-    stage_timed(s, sa, c, 2, true);
-    bc.src = s->B;
+    a.stage[1].in = s->A; a.stage[1].un = s->phi; a.stage[1].out = s->phi; a.stage[1].pre = s->B;
+    a.bc[1].src = s->B; a.bc[1].dst = s->phi; a.bc[1].floor = s->floor;
   } else {
-    sa.in = s->A; sa.un = s->phi; sa.out = s->B; sa.pre = nullptr;
-    stage_timed(s, sa, c, 2, false);
-    bc.src = s->B; bc.dst = s->B;
-    DCHECK(launch_bc(bc, false, s->st));
-    sa.in = s->B; sa.un = s->phi; sa.out = s->phi; sa.pre = s->A;
-    stage_timed(s, sa, c, 3, true);
-    bc.src = s->A;
+    a.stage[1].in = s->A; a.stage[1].un = s->phi; a.stage[1].out = s->B; a.stage[1].pre = nullptr;
+    a.bc[1].src = s->B; a.bc[1].dst = s->B;
+    a.stage[2].in = s->B; a.stage[2].un = s->phi; a.stage[2].out = s->phi; a.stage[2].pre = s->A;
+    a.bc[2].src = s->A; a.bc[2].dst = s->phi; a.bc[2].floor = s->floor;
   }
-  bc.dst = s->phi; bc.floor = s->floor;
-  DCHECK(launch_bc(bc, true, s->st));
+  return a;
+}
+
+static void inner_step(Session* s, const StepCfg& c) {
+  StepArgs a = make_step_args(s, c);
+  DCHECK(launch_probe(a.probe, c.scheme, s->st));
+  DCHECK(hipMemcpyAsync(s->h_dt, s->dt, sizeof(real), hipMemcpyDeviceToHost, s->st));
+  DCHECK(hipEventRecord(s->ev_dt, s->st));
+  const int nst = c.order <= 1 ? 1 : (c.order == 2 ? 2 : 3);
+  for (int i = 0; i < nst; ++i) {
+    const bool fin = i == nst - 1;
+    stage_timed(s, a.stage[i], c, i + 1, fin);
+    DCHECK(launch_bc(a.bc[i], fin, s->st));
+  }
+  if (c.order <= 1) std::swap(s->phi, s->B);
 }
 
 // Imposition floor from the current masks; called whenever mask, mask_w,
@@ -287,10 +281,16 @@ static void profile_collect(Session* s, int order) {
 // step n+1 (or stops).  Returns dt_sub; *last_dt gets the final dt.
 static real inner_loop(Session* s, const StepCfg& c, real tplot, real* last_dt) {
   real dt_sub = 0, dt = 0;
+  BatchHook* hook = batch_hook();
   while (dt_sub < tplot) {
-    inner_step(s, c);
-    DCHECK(hipEventSynchronize(s->ev_dt));
-    dt = *s->h_dt;
+    if (hook) {
+      dt = hook->step(s, c);
+      if (c.order <= 1) std::swap(s->phi, s->B);
+    } else {
+      inner_step(s, c);
+      DCHECK(hipEventSynchronize(s->ev_dt));
+      dt = *s->h_dt;
+    }
     if (!(dt > real(0)) || !isfinite((double)dt))
       throw std::runtime_error("non-positive or non-finite time step");
     profile_collect(s, c.order);
@@ -414,7 +414,15 @@ static void reservoir_box(Session* s, const Grid* g) {
   DCHECK(launch_box(s->phi_extra, s->bx, g->x_lo_ghostbox, g->dx, corner, side, -1, s->st));
 }
 
+BatchHook*& batch_hook() {
+  static thread_local BatchHook* h = nullptr;
+  return h;
+}
+
+// data_step_N files of the model drivers: off with LSMPQS_NO_STEP_FILES=1
+// and in batched runs (the boxes would share the file names)
 static bool step_files_enabled() {
+  if (batch_hook()) return false;
   const char* e = getenv("LSMPQS_NO_STEP_FILES");
   return !(e && *e && atoi(e) != 0);
 }
@@ -469,7 +477,7 @@ static void model_coeffs(Session* s, Options* o, const Grid* g, bool var, bool d
   }
 }
 
-static void model_dev(Options* o, QSS_DataArrays* p, Grid* g, FILE* fp, bool drain) {
+void model_run(Options* o, QSS_DataArrays* p, Grid* g, FILE* fp, bool drain) {
   Session* s = session_for(p, g);
   s->tm = Timers();
   bool var = (o->theta > 0) || o->use_var_theta;
@@ -613,14 +621,14 @@ QSSLIB_REAL constCurvModel3dNoVar(Options* o, QSS_DataArrays* p, Grid* g, FILE* 
 void drain_model3d(Options* o, QSS_DataArrays* p, Grid* g, FILE* fp) {
   API_TRY
   lsm::set_error("");
-  model_dev(o, p, g, fp, true);
+  model_run(o, p, g, fp, true);
   API_CATCH()
 }
 
 void imbibe_model3d(Options* o, QSS_DataArrays* p, Grid* g, FILE* fp) {
   API_TRY
   lsm::set_error("");
-  model_dev(o, p, g, fp, false);
+  model_run(o, p, g, fp, false);
   API_CATCH()
 }
 

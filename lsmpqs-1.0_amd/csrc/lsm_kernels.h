@@ -63,6 +63,32 @@ struct BcArgs {
 };
 hipError_t launch_bc(const BcArgs& a, bool final_mask, hipStream_t st);
 
+// Batched inner steps of many boxes.  One record per box; the launches read
+// the record count and prefix sums from device memory, so one captured graph
+// serves every step whatever the set of boxes.
+struct BatchRec {
+  ProbeArgs probe;
+  StageArgs stage[3];
+  BcArgs bc[3];
+};
+struct BatchView {
+  const BatchRec* recs;
+  const int* count;              // [0]: number of records n
+  const int* brick_prefix;       // n+1 entries: first brick of each record
+  const int64_t* ghost_prefix;   // n+1 entries: first ghost voxel of each record
+  int* counters;                 // work-queue counters, one per stage launch
+  real* dts;                     // dt of each record (written by the probe)
+};
+hipError_t launch_batch_reset(int* counters, int ncounters, hipStream_t st);
+hipError_t launch_probe_batch(const BatchView& v, int scheme, int nmax, hipStream_t st);
+// persistent: `grid` workgroups take bricks from counters[si] until none is left
+hipError_t launch_stage_batch(const BatchView& v, int si, int scheme, bool var, int stage,
+                              bool final_stage, bool fast, int grid, hipStream_t st);
+hipError_t launch_bc_batch(const BatchView& v, int si, bool final_mask, int grid, hipStream_t st);
+// bricks and ghost voxels of one box for a z-chunk length zc (sets a.tiles_*)
+int stage_bricks(StageArgs& a, int zc);
+int64_t ghost_count(const Box& bx);
+
 // Elementwise over the whole ghost box.
 hipError_t launch_impose(real* dst, const real* m, const real* src, real overlap, int64_t n,
                          hipStream_t st);                     // dst = max(src, m - overlap)

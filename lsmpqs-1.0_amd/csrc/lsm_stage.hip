@@ -145,15 +145,17 @@ struct PointIn {
 // reconstruction shared between neighbours, FMA contraction in the
 // curvature, hardware square root.  Otherwise every expression follows the
 // reference's order.
+// One brick: the 64 x 8 column `tile` (x fastest, then y, then z chunk) of
+// the decomposition in a.tiles_*.  Called by the whole block; returns at the
+// same point for every thread.
 template <int S, bool VAR, int STAGE, bool FINAL, bool FAST>
-__global__ __launch_bounds__(TX * TY) void stage_kernel(StageArgs a) {
+__device__ __forceinline__ void stage_tile(const StageArgs& a, int tile) {
   constexpr bool FW = FAST && S == WENO5;
   __shared__ real pl[NSLOT][PY * PX];
   __shared__ real xp[FW ? 2 : 1][FW ? TY : 1][TX + 1];  // phi_x^+ by face, double-buffered
   __shared__ real yp[FW ? 2 : 1][FW ? TY + 1 : 1][TX];  // phi_y^+ by face
   const Box bx = a.bx;
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
-  const int tile = xcd_tile(blockIdx.x, a.tiles_x * a.tiles_y * a.tiles_z);
   const int tcol = tile % a.tiles_x, trow = (tile / a.tiles_x) % a.tiles_y;
   const int tzc = tile / (a.tiles_x * a.tiles_y);
   const int i0 = bx.ilo + tcol * TX, j0 = bx.jlo + trow * TY;
@@ -321,6 +323,11 @@ __global__ __launch_bounds__(TX * TY) void stage_kernel(StageArgs a) {
 #pragma unroll
     for (int o = 0; o < 6; ++o) w[o] = w[o + 1];
   }
+}
+
+template <int S, bool VAR, int STAGE, bool FINAL, bool FAST>
+__global__ __launch_bounds__(TX * TY) void stage_kernel(StageArgs a) {
+  stage_tile<S, VAR, STAGE, FINAL, FAST>(a, xcd_tile(blockIdx.x, a.tiles_x * a.tiles_y * a.tiles_z));
 }
 
 template <int S, bool VAR, int STAGE, bool FAST>
@@ -500,8 +507,9 @@ hipError_t launch_reinit_smooth(const ReinitArgs& a, int stage, hipStream_t st) 
 // Time-step probe: H/dx at (ihi_fb, jhi_fb, last plane of each slab), the
 // voxels whose value the reference's per-thread max_H_over_dX ends up holding
 // (qss_spatial_derivatives3d.f:490; constCurvModel3d.c:105-108,139-149).
+// H/dx over the slab-end voxels reduced by the block; returns dt in thread 0
 template <int S>
-__global__ void probe_kernel(ProbeArgs a) {
+__device__ real probe_block(const ProbeArgs& a) {
   __shared__ real hs[1024];
   const Box bx = a.bx;
   int t = threadIdx.x;
@@ -520,12 +528,20 @@ __global__ void probe_kernel(ProbeArgs a) {
   }
   hs[t] = h;
   __syncthreads();
+  real dt = real(0);
   if (t == 0) {
     real cur = real(-1);
     for (int s = 0; s < a.nslabs; ++s)
       if (hs[s] > cur) cur = hs[s];
-    *a.dt_out = real(0.5) / (cur + a.bmax_over_dx + a.umax_over_dx);
+    dt = real(0.5) / (cur + a.bmax_over_dx + a.umax_over_dx);
   }
+  return dt;
+}
+
+template <int S>
+__global__ void probe_kernel(ProbeArgs a) {
+  real dt = probe_block<S>(a);
+  if (threadIdx.x == 0) *a.dt_out = dt;
 }
 
 hipError_t launch_probe(const ProbeArgs& a, int scheme, hipStream_t st) {
@@ -571,14 +587,14 @@ __device__ __forceinline__ real bc_z(const real* f, const Box& bx, int i, int j,
   return bc_y(f, bx, i, j, k);
 }
 
+// ghost voxel g of the three regions (z ghost planes, then y ghost rows of
+// the fill planes, then x ghost columns of the fill rows)
 template <bool FINAL>
-__global__ void bc_kernel(BcArgs a, int64_t nA, int64_t nB, int64_t nC) {
+__device__ __forceinline__ void bc_one(const BcArgs& a, int64_t g, int64_t nA, int64_t nB) {
   const Box bx = a.bx;
-  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t stride = (int64_t)gridDim.x * blockDim.x;
   int nxf = bx.ihi - bx.ilo + 1, nyf = bx.jhi - bx.jlo + 1;
   int gxn = bx.nx - nxf, gyn = bx.ny - nyf;
-  for (; g < nA + nB + nC; g += stride) {
+  {
     int i, j, k;
     if (g < nA) {
       int64_t plane = (int64_t)bx.nx * bx.ny;
@@ -612,17 +628,164 @@ __global__ void bc_kernel(BcArgs a, int64_t nA, int64_t nB, int64_t nC) {
   }
 }
 
-hipError_t launch_bc(const BcArgs& a, bool final_mask, hipStream_t st) {
-  const Box& bx = a.bx;
+template <bool FINAL>
+__global__ void bc_kernel(BcArgs a, int64_t nA, int64_t nB, int64_t nC) {
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < nA + nB + nC; g += stride)
+    bc_one<FINAL>(a, g, nA, nB);
+}
+
+__host__ __device__ static inline void ghost_regions(const Box& bx, int64_t& nA, int64_t& nB,
+                                                     int64_t& nC) {
   int nxf = bx.ihi - bx.ilo + 1, nyf = bx.jhi - bx.jlo + 1, nzf = bx.khi - bx.klo + 1;
-  int64_t nA = (int64_t)(bx.nz - nzf) * bx.nx * bx.ny;
-  int64_t nB = (int64_t)nzf * (bx.ny - nyf) * bx.nx;
-  int64_t nC = (int64_t)nzf * nyf * (bx.nx - nxf);
+  nA = (int64_t)(bx.nz - nzf) * bx.nx * bx.ny;
+  nB = (int64_t)nzf * (bx.ny - nyf) * bx.nx;
+  nC = (int64_t)nzf * nyf * (bx.nx - nxf);
+}
+int64_t ghost_count(const Box& bx) {
+  int64_t nA, nB, nC;
+  ghost_regions(bx, nA, nB, nC);
+  return nA + nB + nC;
+}
+
+hipError_t launch_bc(const BcArgs& a, bool final_mask, hipStream_t st) {
+  int64_t nA, nB, nC;
+  ghost_regions(a.bx, nA, nB, nC);
   int64_t tot = nA + nB + nC;
   int blocks = (int)std::min<int64_t>((tot + 255) / 256, 8192);
   if (blocks < 1) blocks = 1;
   if (final_mask) hipLaunchKernelGGL(bc_kernel<true>, dim3(blocks), dim3(256), 0, st, a, nA, nB, nC);
   else hipLaunchKernelGGL(bc_kernel<false>, dim3(blocks), dim3(256), 0, st, a, nA, nB, nC);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Batched inner steps (records, counts and prefix sums in device memory).
+
+// largest r < n with prefix[r] <= b
+template <typename I>
+__device__ __forceinline__ int find_rec(const I* prefix, int n, I b) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= b) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void batch_reset_kernel(int* counters, int n) {
+  if ((int)threadIdx.x < n) counters[threadIdx.x] = 0;
+}
+hipError_t launch_batch_reset(int* counters, int ncounters, hipStream_t st) {
+  hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, st, counters, ncounters);
+  return hipGetLastError();
+}
+
+template <int S>
+__global__ void probe_batch_kernel(BatchView v) {
+  const int r = blockIdx.x;
+  if (r >= v.count[0]) return;
+  const ProbeArgs a = v.recs[r].probe;
+  real dt = probe_block<S>(a);
+  if (threadIdx.x == 0) {
+    *a.dt_out = dt;
+    v.dts[r] = dt;
+  }
+}
+hipError_t launch_probe_batch(const BatchView& v, int scheme, int nmax, hipStream_t st) {
+  dim3 grid(nmax), block(1024);
+  switch (scheme) {
+  case 2: hipLaunchKernelGGL(probe_batch_kernel<ENO2>, grid, block, 0, st, v); break;
+  case 3: hipLaunchKernelGGL(probe_batch_kernel<ENO3>, grid, block, 0, st, v); break;
+  case 5: hipLaunchKernelGGL(probe_batch_kernel<WENO5>, grid, block, 0, st, v); break;
+  default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// Persistent stage kernel: each workgroup takes the next brick of the queue
+// (bricks of all records in record order) until the queue is empty.
+template <int S, bool VAR, int STAGE, bool FINAL, bool FAST>
+__global__ __launch_bounds__(TX * TY) void stage_batch_kernel(BatchView v, int si) {
+  __shared__ int next;
+  const int n = v.count[0], total = v.brick_prefix[n];
+  const int tid = threadIdx.y * TX + threadIdx.x;
+  for (;;) {
+    if (tid == 0) next = atomicAdd(&v.counters[si], 1);
+    __syncthreads();
+    const int b = next;
+    __syncthreads();
+    if (b >= total) break;
+    const int r = find_rec(v.brick_prefix, n, b);
+    const StageArgs a = v.recs[r].stage[si];
+    stage_tile<S, VAR, STAGE, FINAL, FAST>(a, b - v.brick_prefix[r]);
+  }
+}
+
+template <int S, bool VAR, bool FAST>
+static hipError_t stage_batch_s(const BatchView& v, int si, int stage, bool fin, dim3 g, hipStream_t st) {
+  dim3 b(TX, TY, 1);
+#define LSM_SB(ST, FI) hipLaunchKernelGGL((stage_batch_kernel<S, VAR, ST, FI, FAST>), g, b, 0, st, v, si)
+  switch (stage * 2 + (fin ? 1 : 0)) {
+  case 2: LSM_SB(1, false); break;
+  case 3: LSM_SB(1, true); break;
+  case 4: LSM_SB(2, false); break;
+  case 5: LSM_SB(2, true); break;
+  case 6: LSM_SB(3, false); break;
+  default: LSM_SB(3, true); break;
+  }
+#undef LSM_SB
+  return hipGetLastError();
+}
+
+template <bool FAST>
+static hipError_t stage_batch_f(const BatchView& v, int si, int scheme, bool var, int stage,
+                                bool fin, dim3 g, hipStream_t st) {
+  switch (scheme) {
+  case 2: return var ? stage_batch_s<ENO2, true, FAST>(v, si, stage, fin, g, st)
+                     : stage_batch_s<ENO2, false, FAST>(v, si, stage, fin, g, st);
+  case 3: return var ? stage_batch_s<ENO3, true, FAST>(v, si, stage, fin, g, st)
+                     : stage_batch_s<ENO3, false, FAST>(v, si, stage, fin, g, st);
+  case 5: return var ? stage_batch_s<WENO5, true, FAST>(v, si, stage, fin, g, st)
+                     : stage_batch_s<WENO5, false, FAST>(v, si, stage, fin, g, st);
+  default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_stage_batch(const BatchView& v, int si, int scheme, bool var, int stage,
+                              bool fin, bool fast, int grid, hipStream_t st) {
+  if (grid < 1) return hipErrorInvalidValue;
+  if constexpr (std::is_same<real, float>::value) {
+    if (fast) return stage_batch_f<true>(v, si, scheme, var, stage, fin, dim3(grid), st);
+  }
+  return stage_batch_f<false>(v, si, scheme, var, stage, fin, dim3(grid), st);
+}
+
+int stage_bricks(StageArgs& a, int zc) {
+  const Box& bx = a.bx;
+  int nxf = bx.ihi - bx.ilo + 1, nyf = bx.jhi - bx.jlo + 1, nzf = bx.khi - bx.klo + 1;
+  a.tiles_x = (nxf + TX - 1) / TX;
+  a.tiles_y = (nyf + TY - 1) / TY;
+  a.tiles_z = std::max(1, (nzf + zc - 1) / zc);
+  return a.tiles_x * a.tiles_y * a.tiles_z;
+}
+
+template <bool FINAL>
+__global__ void bc_batch_kernel(BatchView v, int si) {
+  const int n = v.count[0];
+  const int64_t total = v.ghost_prefix[n];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += stride) {
+    const int r = find_rec(v.ghost_prefix, n, g);
+    const BcArgs& a = v.recs[r].bc[si];
+    int64_t nA, nB, nC;
+    ghost_regions(a.bx, nA, nB, nC);
+    bc_one<FINAL>(a, g - v.ghost_prefix[r], nA, nB);
+  }
+}
+hipError_t launch_bc_batch(const BatchView& v, int si, bool final_mask, int grid, hipStream_t st) {
+  if (final_mask) hipLaunchKernelGGL(bc_batch_kernel<true>, dim3(grid), dim3(256), 0, st, v, si);
+  else hipLaunchKernelGGL(bc_batch_kernel<false>, dim3(grid), dim3(256), 0, st, v, si);
   return hipGetLastError();
 }
 

@@ -25,6 +25,12 @@ class Timing(C.Structure):
                 ("outer_steps", C.c_longlong), ("inner_loop_ms", C.c_double)]
 
 
+class BatchStats(C.Structure):
+    _fields_ = [("boxes", C.c_int), ("batched_steps", C.c_longlong), ("box_steps", C.c_longlong),
+                ("brick_depth", C.c_int), ("workgroups", C.c_int), ("batched_ms", C.c_double),
+                ("autotune_best_ms", C.c_double)]
+
+
 # every exported entry point declared in include/lsmpqs_hip.h and qss_data_arrays.h
 EXPORTS = (
     "constCurvModel3d", "constCurvModel3dNoVar", "drain_model3d", "imbibe_model3d",
@@ -38,6 +44,7 @@ EXPORTS = (
     "drainTop", "imbibeTop", "createOptionsFromInputFile", "printOptions", "createGridSetDx",
     "createGridSetDxDyDz", "createGridSetGridDims", "destroyGrid", "printGrid",
     "writeGridToBinaryFile", "readGridFromBinaryFile", "reinitializeMedium3d",
+    "lsmpqs_batch_model3d",
 )
 
 
