@@ -155,6 +155,47 @@ typedef struct {
 int lsmpqs_batch_model3d(int n, Options **options, QSS_DataArrays **p, Grid **g, FILE **fp_out,
                          int drain, lsmpqs_batch_stats *stats);
 
+/* Gradient of a saturation loss through the level-set time steps.  No
+   counterpart in the reference (which has no derivatives); the forward map is
+   the reference's variable-coefficient constCurvModel3d inner step
+   (src/level_set_evolution/constCurvModel3d.c:127-230) with
+   setVarNorm3dTheta's per-voxel contact angle
+   (src/spatial_derivatives/qss_spatial_derivatives3d.f:94-170; the
+   reference's drain_model3d.c:124-131 never reaches it, its test
+   ~(use_var_theta) being nonzero for 0 and 1) at a fixed time step: nsteps TVD-RK steps of order options->order_time_accur with
+   the scheme options->order_space_accur (2, 3 or 5), the library's ghost
+   extrapolation and the imposition max(phi, mask - overlap) after each
+   step, then
+     S_w  = 1 - V(phi < 0) / V(mask < 0)   (smoothed Heaviside, width
+                                            options->eps_coefficient dx)
+     loss = (S_w - target_sw)^2.
+   Reference quirks are off in this map (exact normal-term sign convention of
+   the scheme, no stale normal terms, no phi(i+3,j,j) read).  theta: per-voxel
+   contact angle (radians) on the ghost box; p->phi the initial state (ghost
+   values used as given), p->mask the pore mask.  On return p->phi holds the
+   final state, dloss_dtheta the gradient on the fill box and 0 on ghosts,
+   args the loss, S_w, the time step and d loss / d b (dloss_dtheta NULL:
+   loss only, no reverse pass, dloss_db = 0).  The
+   reverse pass transposes the linearisation of every stage with the
+   forward's discrete choices (ENO stencils, upwind sides, Godunov branch,
+   extrapolation signs, imposition) held fixed; states are stored every
+   checkpoint_every steps and recomputed in between.  Returns 0, or -2 with
+   the message in lsmpqs_hip_last_error(). */
+typedef struct {
+  int nsteps;                 /* in: time steps, >= 1 */
+  int checkpoint_every;       /* in: steps between stored states, >= 1 */
+  double dt;                  /* in: fixed time step; <= 0: CFL rule on the initial state */
+  double a0;                  /* in: pressure coefficient */
+  double b;                   /* in: curvature coefficient */
+  double target_sw;           /* in: target wetting saturation */
+  double loss;                /* out */
+  double sw;                  /* out: final wetting saturation */
+  double dt_used;             /* out */
+  double dloss_db;            /* out: d loss / d b */
+} lsmpqs_adjoint_args;
+int lsmpqs_adjoint_run(Options *options, QSS_DataArrays *p, Grid *g, const QSSLIB_REAL *theta,
+                       QSSLIB_REAL *dloss_dtheta, lsmpqs_adjoint_args *args);
+
 /* free the device state kept for record p */
 void lsmpqs_release(QSS_DataArrays *p);
 

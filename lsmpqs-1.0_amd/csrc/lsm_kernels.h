@@ -89,6 +89,44 @@ hipError_t launch_bc_batch(const BatchView& v, int si, bool final_mask, int grid
 int stage_bricks(StageArgs& a, int zc);
 int64_t ghost_count(const Box& bx);
 
+// Ghost voxel g of a box -> (i, j, k), in three regions: the z ghost planes
+// (nA voxels), the y ghost rows of the fill planes (nB), the x ghost columns
+// of the fill rows.
+__host__ __device__ inline void ghost_ijk(const Box& bx, int64_t g, int64_t nA, int64_t nB, int& i,
+                                          int& j, int& k) {
+  int nxf = bx.ihi - bx.ilo + 1, nyf = bx.jhi - bx.jlo + 1;
+  int gxn = bx.nx - nxf, gyn = bx.ny - nyf;
+  if (g < nA) {
+    int64_t plane = (int64_t)bx.nx * bx.ny;
+    int kk = (int)(g / plane);
+    int64_t r = g - kk * plane;
+    j = (int)(r / bx.nx);
+    i = (int)(r - (int64_t)j * bx.nx);
+    k = kk < bx.klo ? kk : bx.khi + 1 + (kk - bx.klo);
+  } else if (g < nA + nB) {
+    int64_t h = g - nA;
+    i = (int)(h % bx.nx);
+    int64_t t = h / bx.nx;
+    int jj = (int)(t % gyn);
+    k = bx.klo + (int)(t / gyn);
+    j = jj < bx.jlo ? jj : bx.jhi + 1 + (jj - bx.jlo);
+  } else {
+    int64_t h = g - nA - nB;
+    int ii = (int)(h % gxn);
+    int64_t t = h / gxn;
+    j = bx.jlo + (int)(t % nyf);
+    k = bx.klo + (int)(t / nyf);
+    i = ii < bx.ilo ? ii : bx.ihi + 1 + (ii - bx.ilo);
+  }
+}
+__host__ __device__ inline void ghost_regions(const Box& bx, int64_t& nA, int64_t& nB,
+                                              int64_t& nC) {
+  int nxf = bx.ihi - bx.ilo + 1, nyf = bx.jhi - bx.jlo + 1, nzf = bx.khi - bx.klo + 1;
+  nA = (int64_t)(bx.nz - nzf) * bx.nx * bx.ny;
+  nB = (int64_t)nzf * (bx.ny - nyf) * bx.nx;
+  nC = (int64_t)nzf * nyf * (bx.nx - nxf);
+}
+
 // Elementwise over the whole ghost box.
 hipError_t launch_impose(real* dst, const real* m, const real* src, real overlap, int64_t n,
                          hipStream_t st);                     // dst = max(src, m - overlap)

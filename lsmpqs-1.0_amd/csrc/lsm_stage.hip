@@ -592,40 +592,15 @@ __device__ __forceinline__ real bc_z(const real* f, const Box& bx, int i, int j,
 template <bool FINAL>
 __device__ __forceinline__ void bc_one(const BcArgs& a, int64_t g, int64_t nA, int64_t nB) {
   const Box bx = a.bx;
-  int nxf = bx.ihi - bx.ilo + 1, nyf = bx.jhi - bx.jlo + 1;
-  int gxn = bx.nx - nxf, gyn = bx.ny - nyf;
-  {
-    int i, j, k;
-    if (g < nA) {
-      int64_t plane = (int64_t)bx.nx * bx.ny;
-      int kk = (int)(g / plane);
-      int64_t r = g - kk * plane;
-      j = (int)(r / bx.nx);
-      i = (int)(r - (int64_t)j * bx.nx);
-      k = kk < bx.klo ? kk : bx.khi + 1 + (kk - bx.klo);
-    } else if (g < nA + nB) {
-      int64_t h = g - nA;
-      i = (int)(h % bx.nx);
-      int64_t t = h / bx.nx;
-      int jj = (int)(t % gyn);
-      k = bx.klo + (int)(t / gyn);
-      j = jj < bx.jlo ? jj : bx.jhi + 1 + (jj - bx.jlo);
-    } else {
-      int64_t h = g - nA - nB;
-      int ii = (int)(h % gxn);
-      int64_t t = h / gxn;
-      j = bx.jlo + (int)(t % nyf);
-      k = bx.klo + (int)(t / nyf);
-      i = ii < bx.ilo ? ii : bx.ihi + 1 + (ii - bx.ilo);
-    }
-    real v = bc_z(a.src, bx, i, j, k);
-    int64_t idx = bx.at(i, j, k);
-    if (FINAL) {
-      real f = a.floor[idx];
-      v = (f > v) ? f : v;
-    }
-    a.dst[idx] = v;
+  int i, j, k;
+  ghost_ijk(bx, g, nA, nB, i, j, k);
+  real v = bc_z(a.src, bx, i, j, k);
+  int64_t idx = bx.at(i, j, k);
+  if (FINAL) {
+    real f = a.floor[idx];
+    v = (f > v) ? f : v;
   }
+  a.dst[idx] = v;
 }
 
 template <bool FINAL>
@@ -635,13 +610,6 @@ __global__ void bc_kernel(BcArgs a, int64_t nA, int64_t nB, int64_t nC) {
     bc_one<FINAL>(a, g, nA, nB);
 }
 
-__host__ __device__ static inline void ghost_regions(const Box& bx, int64_t& nA, int64_t& nB,
-                                                     int64_t& nC) {
-  int nxf = bx.ihi - bx.ilo + 1, nyf = bx.jhi - bx.jlo + 1, nzf = bx.khi - bx.klo + 1;
-  nA = (int64_t)(bx.nz - nzf) * bx.nx * bx.ny;
-  nB = (int64_t)nzf * (bx.ny - nyf) * bx.nx;
-  nC = (int64_t)nzf * nyf * (bx.nx - nxf);
-}
 int64_t ghost_count(const Box& bx) {
   int64_t nA, nB, nC;
   ghost_regions(bx, nA, nB, nC);

@@ -6,6 +6,7 @@
 // contiguous bytes.
 #include <float.h>
 #include <algorithm>
+#include "lsm_coeff.h"
 #include "lsm_kernels.h"
 
 namespace lsm {
@@ -91,22 +92,7 @@ hipError_t launch_fill(real* dst, real v, int64_t n, hipStream_t st) {
 
 // ---------------------------------------------------------------------------
 // Fill-box iteration helper: flat fill index -> (i,j,k)
-struct FillIter {
-  Box bx;
-  int nxf, nyf;
-  int64_t n;
-  __host__ __device__ FillIter(const Box& b) : bx(b) {
-    nxf = b.ihi - b.ilo + 1;
-    nyf = b.jhi - b.jlo + 1;
-    n = (int64_t)nxf * nyf * (b.khi - b.klo + 1);
-  }
-  __device__ void ijk(int64_t f, int& i, int& j, int& k) const {
-    int64_t t = f / nxf;
-    i = bx.ilo + (int)(f - t * nxf);
-    j = bx.jlo + (int)(t % nyf);
-    k = bx.klo + (int)(t / nyf);
-  }
-};
+
 
 constexpr int RED_BLOCKS = 2048;
 int reduce_partials() { return 2 * RED_BLOCKS; }
@@ -357,17 +343,6 @@ hipError_t launch_central_grad(const Box& bx, real* gx, real* gy, real* gz, cons
   return hipGetLastError();
 }
 
-__device__ __forceinline__ void coeff_common(real m, real mx, real my, real mz, real dx, real& h1,
-                                             real& h2, real& ag, real& s) {
-  real eps_var = real(1.5) * dx;
-  h1 = heaviside(-m, eps_var);
-  h2 = heaviside(m, eps_var);
-  ag = mx * mx + my * my + mz * mz;
-  ag = sqrt(ag);
-  real x = m * m + ag * dx * ag * dx;
-  x = sqrt(x);
-  s = m / x;
-}
 
 __global__ void var_norm_kernel(Box bx, real* vn, const real* mask, const real* gx, const real* gy,
                                 const real* gz, real a0, real cos_theta, real dx) {

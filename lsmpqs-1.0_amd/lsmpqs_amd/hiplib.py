@@ -44,7 +44,7 @@ EXPORTS = (
     "drainTop", "imbibeTop", "createOptionsFromInputFile", "printOptions", "createGridSetDx",
     "createGridSetDxDyDz", "createGridSetGridDims", "destroyGrid", "printGrid",
     "writeGridToBinaryFile", "readGridFromBinaryFile", "reinitializeMedium3d",
-    "lsmpqs_batch_model3d",
+    "lsmpqs_batch_model3d", "lsmpqs_adjoint_run",
 )
 
 

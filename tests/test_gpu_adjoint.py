@@ -1,0 +1,177 @@
+"""Gradient of the saturation loss through the level-set time steps
+(lsmpqs_adjoint_run) on the GPU.
+
+The reverse pass is checked against central finite differences of the same
+forward map in FP64 along 16 seeded random directions of the contact-angle
+field (and along b), for each spatial scheme and Runge-Kutta order, to
+1e-3 of the largest directional derivative (directions whose probes straddle
+a switch of the scheme are detected and left out, at most 4 of 16); the
+FP32 library's gradient against the FP64 one; the result against the
+checkpoint interval; and the torch.autograd wrapper against the raw entry
+point, with a fit of a uniform contact angle to a target saturation.  The
+forward map is the variable-coefficient inner step of the reference's
+constCurvModel3d (src/level_set_evolution/constCurvModel3d.c:127-230) with
+setVarNorm3dTheta's per-voxel angle
+(src/spatial_derivatives/qss_spatial_derivatives3d.f:94-170)."""
+import numpy as np
+import pytest
+
+from lsmpqs_amd.abi import default_options
+from lsmpqs_amd.adjoint import saturation_loss, ThetaLoss
+from lsm_cases import case
+
+N = 20
+STEPS = 12
+A0, B = 0.05, 0.04
+TARGET = 0.5
+FD_H = 3e-7
+FD_TOL = 1e-3
+
+
+def problem(dt, space, time, seed=7):
+    g, mask, phi0 = case(N, dt, seed=3100)
+    o = default_options(dt)
+    o.order_space_accur = space
+    o.order_time_accur = time
+    rng = np.random.default_rng(seed)
+    theta = (0.4 + 0.6 * rng.random(phi0.shape)).astype(dt)
+    return g, o, np.ascontiguousarray(phi0), np.ascontiguousarray(mask), theta
+
+
+def fill_of(g):
+    return (slice(g.klo_fb, g.khi_fb + 1), slice(g.jlo_fb, g.jhi_fb + 1),
+            slice(g.ilo_fb, g.ihi_fb + 1))
+
+
+def smooth_between(lm, l0, lp, h):
+    """The forward map is piecewise smooth: the reference's scheme switches
+    discontinuously (ENO stencils at |D2| ties of opposite sign, the
+    curvature cut at |grad phi|^2 = 1e-5, the normal-term cut at
+    |vel_n| = 1e-5, the sign of the ghost extrapolation at phi = 0).  A
+    central difference whose two probes straddle such a switch measures the
+    jump, not the derivative; its one-sided differences then disagree by
+    jump / h.  True where they agree to 5 %."""
+    fwd, bwd = (lp - l0) / h, (l0 - lm) / h
+    return abs(fwd - bwd) <= 0.05 * max(abs(fwd), abs(bwd), 1e-30)
+
+
+def loss(g, o, phi0, mask, theta, b=B, dt=0.0):
+    return saturation_loss(phi0, mask, theta, g, o, STEPS, A0, b, TARGET, dt=dt,
+                           gradient=False).loss
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("space,time", [(2, 1), (2, 2), (3, 3), (5, 3), (5, 2)])
+def test_gradient_matches_finite_differences_fp64(space, time):
+    print(f"case: adjoint FD fp64 scheme {space} RK{time} n={N} steps={STEPS}", flush=True)
+    g, o, phi0, mask, theta = problem(np.float64, space, time)
+    r = saturation_loss(phi0, mask, theta, g, o, STEPS, A0, B, TARGET, checkpoint_every=4)
+    assert np.isfinite(r.loss) and r.dt > 0
+    assert r.loss == loss(g, o, phi0, mask, theta, dt=r.dt)  # loss-only run, same map
+    f = fill_of(g)
+    ghost = np.ones(theta.shape, dtype=bool)
+    ghost[f] = False
+    assert not r.dtheta[ghost].any()
+    assert np.abs(r.dtheta).max() > 0
+    rng = np.random.default_rng(100 + 10 * space + time)
+    l0 = r.loss
+    got, want, skipped = [], [], 0
+    for _ in range(16):
+        v = np.zeros_like(theta)
+        v[f] = rng.standard_normal(v[f].shape)
+        lp = loss(g, o, phi0, mask, theta + FD_H * v, dt=r.dt)
+        lm = loss(g, o, phi0, mask, theta - FD_H * v, dt=r.dt)
+        if not smooth_between(lm, l0, lp, FD_H):
+            skipped += 1
+            continue
+        want.append((lp - lm) / (2 * FD_H))
+        got.append(float(np.sum(r.dtheta * v)))
+    print(f"directions crossing a switch of the scheme: {skipped} of 16", flush=True)
+    assert skipped <= 4
+    got, want = np.array(got), np.array(want)
+    scale = np.abs(want).max()
+    assert scale > 0
+    print(f"adjoint vs central differences: max error {np.abs(got - want).max() / scale:.2e} "
+          f"of the largest directional derivative", flush=True)
+    assert np.abs(got - want).max() <= FD_TOL * scale, (got, want)
+    # b
+    for hb in (1e-6, 3e-7, 1e-7):
+        lp = loss(g, o, phi0, mask, theta, B + hb, dt=r.dt)
+        lm = loss(g, o, phi0, mask, theta, B - hb, dt=r.dt)
+        if smooth_between(lm, l0, lp, hb):
+            break
+    else:
+        pytest.fail("every b step crosses a switch of the scheme")
+    fd_b = (lp - lm) / (2 * hb)
+    assert abs(r.dloss_db - fd_b) <= FD_TOL * max(abs(fd_b), 1e-12), (r.dloss_db, fd_b)
+
+
+@pytest.mark.gpu
+def test_checkpoint_interval_does_not_change_the_gradient():
+    print("case: adjoint checkpoints fp64 WENO5 RK3", flush=True)
+    g, o, phi0, mask, theta = problem(np.float64, 5, 3)
+    rs = [saturation_loss(phi0, mask, theta, g, o, STEPS, A0, B, TARGET, checkpoint_every=k)
+          for k in (1, 5, STEPS)]
+    for r in rs[1:]:
+        assert r.loss == rs[0].loss
+        assert np.array_equal(r.phi, rs[0].phi)
+        # the transposed stencils accumulate with atomics: order-dependent rounding only
+        assert np.abs(r.dtheta - rs[0].dtheta).max() <= 1e-10 * np.abs(rs[0].dtheta).max()
+        assert abs(r.dloss_db - rs[0].dloss_db) <= 1e-10 * abs(rs[0].dloss_db)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("space,time", [(3, 3), (5, 3)])
+def test_fp32_gradient_matches_fp64(space, time):
+    print(f"case: adjoint fp32 vs fp64 scheme {space} RK{time}", flush=True)
+    g64, o64, phi64, mask64, th64 = problem(np.float64, space, time)
+    g32, o32, phi32, mask32, th32 = problem(np.float32, space, time)
+    r64 = saturation_loss(phi64, mask64, th64, g64, o64, STEPS, A0, B, TARGET)
+    r32 = saturation_loss(phi32, mask32, th32, g32, o32, STEPS, A0, B, TARGET, dt=r64.dt)
+    assert abs(r32.sw - r64.sw) <= 1e-5
+    d64, d32 = r64.dtheta, r32.dtheta.astype(np.float64)
+    rel = np.linalg.norm(d32 - d64) / np.linalg.norm(d64)
+    print(f"fp32 vs fp64 gradient: relative L2 {rel:.3e}", flush=True)
+    assert rel <= 1e-4
+    assert abs(r32.dloss_db - r64.dloss_db) <= 1e-4 * abs(r64.dloss_db)
+
+
+@pytest.mark.gpu
+def test_autograd_wrapper_and_uniform_angle_fit():
+    import torch
+    print("case: adjoint autograd + fit fp64 ENO3 RK3", flush=True)
+    g, o, phi0, mask, theta = problem(np.float64, 3, 3)
+    prob = {"phi0": phi0, "mask": mask, "grid": g, "options": o, "nsteps": STEPS, "a0": A0,
+            "target_sw": TARGET}
+    th = torch.tensor(theta, requires_grad=True)
+    b = torch.tensor(B, dtype=torch.float64, requires_grad=True)
+    L = ThetaLoss.apply(th, b, prob)
+    L.backward()
+    r = saturation_loss(phi0, mask, theta, g, o, STEPS, A0, B, TARGET)
+    assert float(L.detach()) == r.loss
+    # two runs: the atomics of the transposed stencils round in different orders
+    assert np.abs(th.grad.numpy() - r.dtheta).max() <= 1e-10 * np.abs(r.dtheta).max()
+    assert abs(float(b.grad) - r.dloss_db) <= 1e-10 * abs(r.dloss_db)
+    # Gauss-Newton on one uniform angle: loss = res^2, d res / d t = (dL/dt) / (2 res)
+    prob["dt"] = r.dt
+    t = torch.tensor(0.6, dtype=torch.float64, requires_grad=True)
+    field = torch.ones(phi0.shape, dtype=torch.float64)
+    bb = torch.tensor(B, dtype=torch.float64)
+    # a target the map reaches: the saturation at angle 0.9
+    prob["target_sw"] = saturation_loss(phi0, mask, np.full_like(theta, 0.9), g, o, STEPS, A0, B,
+                                        TARGET, dt=r.dt, gradient=False).sw
+    L0 = ThetaLoss.apply(t * field, bb, prob)
+    losses = []
+    for _ in range(4):
+        t.grad = None
+        L = ThetaLoss.apply(t * field, bb, prob)
+        L.backward()
+        res = prob["last"].sw - prob["target_sw"]
+        losses.append(float(L))
+        if float(t.grad) == 0.0:
+            break
+        with torch.no_grad():
+            t -= res / (float(t.grad) / (2 * res))
+    print(f"fit: loss {losses}, angle {float(t):.4f}", flush=True)
+    assert float(L0) >= 0
+    assert losses[-1] <= 1e-2 * losses[0]
