@@ -165,12 +165,17 @@ int lsmpqs_batch_model3d(int n, Options **options, QSS_DataArrays **p, Grid **g,
    ~(use_var_theta) being nonzero for 0 and 1) at a fixed time step: nsteps TVD-RK steps of order options->order_time_accur with
    the scheme options->order_space_accur (2, 3 or 5), the library's ghost
    extrapolation and the imposition max(phi, mask - overlap) after each
-   step, then
+   step and the far-field cap phi = min(phi, phi_cap) (no counterpart in
+   the reference, whose inner steps are few between reinitializations: over
+   many steps the signed extrapolation grows phi without bound where the
+   mask's advection velocity enters the box through a face, in solid far
+   from the interface), then
      S_w  = 1 - V(phi < 0) / V(mask < 0)   (smoothed Heaviside, width
                                             options->eps_coefficient dx)
      loss = (S_w - target_sw)^2.
-   Reference quirks are off in this map (exact normal-term sign convention of
-   the scheme, no stale normal terms, no phi(i+3,j,j) read).  theta: per-voxel
+   Reference quirks are off in this map (normal term -vel_n |grad phi| for
+   every scheme, no stale normal terms, no phi(i+3,j,j) read); the FP32
+   library uses its fast WENO5 and curvature arithmetic.  theta: per-voxel
    contact angle (radians) on the ghost box; p->phi the initial state (ghost
    values used as given), p->mask the pore mask.  On return p->phi holds the
    final state, dloss_dtheta the gradient on the fill box and 0 on ghosts,
@@ -188,10 +193,12 @@ typedef struct {
   double a0;                  /* in: pressure coefficient */
   double b;                   /* in: curvature coefficient */
   double target_sw;           /* in: target wetting saturation */
+  double phi_cap;             /* in: far-field cap; <= 0: the largest initial phi */
   double loss;                /* out */
   double sw;                  /* out: final wetting saturation */
   double dt_used;             /* out */
   double dloss_db;            /* out: d loss / d b */
+  double cap_used;            /* out: the cap applied */
 } lsmpqs_adjoint_args;
 int lsmpqs_adjoint_run(Options *options, QSS_DataArrays *p, Grid *g, const QSSLIB_REAL *theta,
                        QSSLIB_REAL *dloss_dtheta, lsmpqs_adjoint_args *args);

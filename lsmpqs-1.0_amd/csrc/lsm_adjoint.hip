@@ -88,6 +88,8 @@ template <int N> __device__ Dual<N> sqrt(const Dual<N>& a) {
   for (int q = 0; q < N; ++q) r.d[q] = a.d[q] * f;
   return r;
 }
+// the fast reconstruction's hardware reciprocal has the derivative of 1/x
+template <int N> __device__ Dual<N> fast_rcp(const Dual<N>& a) { return Dual<N>(real(1)) / a; }
 }  // namespace ad
 using ad::Dual;
 
@@ -168,11 +170,31 @@ __device__ real curv_partials(const real* v, real fx, real fy, real fz, real* w)
   return curv;
 }
 
+// One-sided derivatives of one axis line l[0..6] = phi(-3..+3) as the stage
+// kernel forms them: FW (single precision, WENO5) the fast face
+// reconstructions, phi^- from the window phi(-3..+2) and phi^+ from
+// phi(-2..+3); otherwise the scheme's reference-order formulas.
+template <typename T, int S, bool FW>
+__device__ __forceinline__ void line_derivs(const T* l, real h, real inv_axis, real inv_x, T& plus,
+                                            T& minus) {
+  if constexpr (FW && S == WENO5) {
+    const FaceScale<real> f = face_scale(h);
+    FaceScale<T> sc;
+    sc.inv = T(f.inv);
+    sc.thr = T(f.thr);
+    sc.c1 = T(f.c1);
+    minus = weno5_face(l, sc).minus_right;
+    plus = weno5_face(l + 1, sc).plus_left;
+  } else {
+    axis_derivs<T, S>(l, T(inv_axis), T(inv_x), plus, minus);
+  }
+}
+
 // adj[line voxel o] += lam (fp d plus/d phi_o + fm d minus/d phi_o) along one
 // axis line (stride st through voxel idx)
-template <int S>
+template <int S, bool FW>
 __device__ __forceinline__ void scatter_axis(const real* phi, real* adj, int64_t idx, int64_t st,
-                                             real inv_axis, real inv_x, real fp, real fm,
+                                             real h, real inv_axis, real inv_x, real fp, real fm,
                                              real lam) {
   if (fp == real(0) && fm == real(0)) return;
   Dual<7> l[7];
@@ -182,7 +204,7 @@ __device__ __forceinline__ void scatter_axis(const real* phi, real* adj, int64_t
     l[o].d[o] = 1;
   }
   Dual<7> plus, minus;
-  axis_derivs<Dual<7>, S>(l, Dual<7>(inv_axis), Dual<7>(inv_x), plus, minus);
+  line_derivs<Dual<7>, S, FW>(l, h, inv_axis, inv_x, plus, minus);
 #pragma unroll
   for (int o = 0; o < 7; ++o) {
     const real w = fp * plus.d[o] + fm * minus.d[o];
@@ -190,7 +212,7 @@ __device__ __forceinline__ void scatter_axis(const real* phi, real* adj, int64_t
   }
 }
 
-template <int S>
+template <int S, bool FW>
 __global__ __launch_bounds__(256) void adj_rhs_kernel(AdjRhsArgs a) {
   const Box bx = a.bx;
   FillIter it(bx);
@@ -212,9 +234,9 @@ __global__ __launch_bounds__(256) void adj_rhs_kernel(AdjRhsArgs a) {
       ly[o] = a.phi[idx + (o - 3) * sy];
       lz[o] = a.phi[idx + (o - 3) * sz];
     }
-    axis_derivs<real, S>(lx, ix, ix, d[0], d[1]);
-    axis_derivs<real, S>(ly, iy, ix, d[2], d[3]);
-    axis_derivs<real, S>(lz, iz, ix, d[4], d[5]);
+    line_derivs<real, S, FW>(lx, a.dx, ix, ix, d[0], d[1]);
+    line_derivs<real, S, FW>(ly, a.dy, iy, ix, d[2], d[3]);
+    line_derivs<real, S, FW>(lz, a.dz, iz, ix, d[4], d[5]);
     // partials of the right-hand side w.r.t. them and vel_n
     const real vn = a.vn[idx], bb = a.b[idx];
     const real vxv = a.vx[idx], vyv = a.vy[idx], vzv = a.vz[idx];
@@ -247,9 +269,9 @@ __global__ __launch_bounds__(256) void adj_rhs_kernel(AdjRhsArgs a) {
         atomicAdd(&a.adj[bx.at(i + kCurvOff[q][0], j + kCurvOff[q][1], k + kCurvOff[q][2])], lam * w);
     }
     // one-sided derivatives, one axis line at a time
-    scatter_axis<S>(a.phi, a.adj, idx, 1, ix, ix, fd[0], fd[1], lam);
-    scatter_axis<S>(a.phi, a.adj, idx, sy, iy, ix, fd[2], fd[3], lam);
-    scatter_axis<S>(a.phi, a.adj, idx, sz, iz, ix, fd[4], fd[5], lam);
+    scatter_axis<S, FW>(a.phi, a.adj, idx, 1, a.dx, ix, ix, fd[0], fd[1], lam);
+    scatter_axis<S, FW>(a.phi, a.adj, idx, sy, a.dy, iy, ix, fd[2], fd[3], lam);
+    scatter_axis<S, FW>(a.phi, a.adj, idx, sz, a.dz, iz, ix, fd[4], fd[5], lam);
   }
 }
 
@@ -343,6 +365,24 @@ __global__ void adj_impose_kernel(Box bx, real* lam, const real* out, const real
     if (out[idx] == floor[idx]) lam[idx] = 0;
   }
 }
+// Far-field cap after every step, u = min(u, cap) over the ghost box, and its
+// transpose (lam = 0 where the cap was taken).  Without reinitialization
+// between steps, the signed extrapolation at a face where the mask's
+// advection velocity points into the box feeds a downwind difference, and
+// phi grows without bound there (in solid, far from the interface); the cap
+// keeps such voxels finite without touching the interface region.
+__global__ void adj_cap_kernel(real* u, real cap, int64_t n) {
+  for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n;
+       l += (int64_t)gridDim.x * blockDim.x) {
+    const real v = u[l];
+    u[l] = v > cap ? cap : v;
+  }
+}
+__global__ void adj_cap_rev_kernel(real* lam, const real* out, real cap, int64_t n) {
+  for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n;
+       l += (int64_t)gridDim.x * blockDim.x)
+    if (out[l] == cap) lam[l] = 0;
+}
 // d loss / d phi on the fill box: 2 (S_w - target) dV H'(phi) / V_pore, with
 // H the smoothed volume fraction of the volume reduction
 // (reference: src/utilities/qss_util3d.f:288-358)
@@ -419,6 +459,10 @@ namespace {
   } while (0)
 
 constexpr int kBlocks = 2048;
+// Single precision runs the stage kernels' fast WENO5 and curvature (the
+// reference-order single-precision WENO5 forms 0/0 on flat windows); the
+// reverse pass differentiates the same reconstruction.
+constexpr bool kFast = sizeof(real) == 4;
 
 struct Buffers {
   std::vector<real*> all;
@@ -437,7 +481,7 @@ struct Adjoint {
   Session* s;
   StepCfg c;
   int nstage;
-  real dt;
+  real dt, cap;
   real *u1, *u2, *u3, *pre;  // stage results of one step
   real *lam, *lamn, *lams[2];
   real *g_vn, *g_b, *dvn;
@@ -451,7 +495,7 @@ struct Adjoint {
   StepArgs args(const real* u, real* out) {
     StepArgs a = make_step_args(s, c);
     for (int k = 0; k < 3; ++k) {
-      a.stage[k].fast = 0;
+      a.stage[k].fast = kFast ? 1 : 0;
       a.stage[k].nstale = 0;
       a.stage[k].eno3_quirk = 0;
       a.stage[k].un = u;
@@ -478,6 +522,8 @@ struct Adjoint {
       ACHECK(launch_stage(a.stage[k], c.scheme, true, k + 1, fin, s->st));
       ACHECK(launch_bc(a.bc[k], fin, s->st));
     }
+    hipLaunchKernelGGL(adj_cap_kernel, dim3(kBlocks), dim3(256), 0, s->st, out, cap, s->n);
+    ACHECK(hipGetLastError());
   }
 
   void rhs_transpose(const real* phi, const real* l, real coef, real* adj) {
@@ -487,9 +533,12 @@ struct Adjoint {
     r.vn = s->vn; r.b = s->b; r.vx = s->vx; r.vy = s->vy; r.vz = s->vz;
     r.dx = c.dx; r.dy = c.dy; r.dz = c.dz; r.norm_sign = c.norm_sign;
     switch (c.scheme) {
-    case 2: hipLaunchKernelGGL(adj_rhs_kernel<ENO2>, dim3(kBlocks), dim3(256), 0, s->st, r); break;
-    case 3: hipLaunchKernelGGL(adj_rhs_kernel<ENO3>, dim3(kBlocks), dim3(256), 0, s->st, r); break;
-    default: hipLaunchKernelGGL(adj_rhs_kernel<WENO5>, dim3(kBlocks), dim3(256), 0, s->st, r); break;
+    case 2: hipLaunchKernelGGL((adj_rhs_kernel<ENO2, false>), dim3(kBlocks), dim3(256), 0, s->st, r); break;
+    case 3: hipLaunchKernelGGL((adj_rhs_kernel<ENO3, false>), dim3(kBlocks), dim3(256), 0, s->st, r); break;
+    default:
+      if (kFast) hipLaunchKernelGGL((adj_rhs_kernel<WENO5, true>), dim3(kBlocks), dim3(256), 0, s->st, r);
+      else hipLaunchKernelGGL((adj_rhs_kernel<WENO5, false>), dim3(kBlocks), dim3(256), 0, s->st, r);
+      break;
     }
     ACHECK(hipGetLastError());
   }
@@ -516,6 +565,7 @@ struct Adjoint {
     zero(lamn);
     for (real* l : lams) zero(l);
     // imposition of the last stage, then its ghost values
+    hipLaunchKernelGGL(adj_cap_rev_kernel, dim3(kBlocks), dim3(256), 0, s->st, lam, u3, cap, s->n);
     hipLaunchKernelGGL(adj_impose_kernel, dim3(kBlocks), dim3(256), 0, s->st, s->bx, lam, u3, s->floor);
     ACHECK(hipGetLastError());
     bc_transpose(lam, pre, true);
@@ -604,6 +654,14 @@ extern "C" int lsmpqs_adjoint_run(Options* o, QSS_DataArrays* p, Grid* g, const 
     }
     if (!(dt > 0) || !isfinite((double)dt)) throw std::runtime_error("invalid time step");
     A.dt = dt;
+    if (args->phi_cap > 0) {
+      A.cap = (real)args->phi_cap;
+    } else {
+      real m = p->phi[0];
+      for (int64_t l = 1; l < n; ++l) m = p->phi[l] > m ? p->phi[l] : m;
+      A.cap = m;
+    }
+    args->cap_used = (double)A.cap;
     ACHECK(hipMemcpyAsync(s->dt, &dt, sizeof(real), hipMemcpyHostToDevice, s->st));
     // forward with checkpoints every k steps
     const int N = args->nsteps, K = std::min(args->checkpoint_every, args->nsteps);

@@ -24,8 +24,8 @@ from .abi import types_for
 class AdjointArgs(C.Structure):
     _fields_ = [("nsteps", C.c_int), ("checkpoint_every", C.c_int), ("dt", C.c_double),
                 ("a0", C.c_double), ("b", C.c_double), ("target_sw", C.c_double),
-                ("loss", C.c_double), ("sw", C.c_double), ("dt_used", C.c_double),
-                ("dloss_db", C.c_double)]
+                ("phi_cap", C.c_double), ("loss", C.c_double), ("sw", C.c_double),
+                ("dt_used", C.c_double), ("dloss_db", C.c_double), ("cap_used", C.c_double)]
 
 
 def _lib(dtype):
@@ -45,18 +45,20 @@ class AdjointResult:
         self.sw = args.sw
         self.dt = args.dt_used
         self.dloss_db = args.dloss_db
+        self.cap = args.cap_used
         self.dtheta = dtheta
         self.phi = phi
 
 
 def saturation_loss(phi0, mask, theta, grid, options, nsteps, a0, b, target_sw, dt=0.0,
-                    checkpoint_every=None, gradient=True):
+                    checkpoint_every=None, gradient=True, phi_cap=0.0):
     """Run the forward map and its reverse pass.
 
     phi0, mask, theta: arrays over the ghost box (shape nz+2g, ny+2g, nx+2g)
     of the library's QSSLIB_REAL; options: the Options record (scheme from
     order_space_accur, RK order from order_time_accur, overlap,
-    eps_coefficient); dt <= 0 takes the CFL rule on phi0.  Raises
+    eps_coefficient); dt <= 0 takes the CFL rule on phi0; phi_cap <= 0 caps
+    phi at the largest value of phi0 after every step.  Raises
     hiplib.LsmpqsError on a library error."""
     dtype = np.dtype(phi0.dtype)
     lib = _lib(dtype)
@@ -74,6 +76,7 @@ def saturation_loss(phi0, mask, theta, grid, options, nsteps, a0, b, target_sw, 
     a.nsteps = int(nsteps)
     a.checkpoint_every = int(checkpoint_every or max(1, int(round(np.sqrt(nsteps)))))
     a.dt, a.a0, a.b, a.target_sw = float(dt), float(a0), float(b), float(target_sw)
+    a.phi_cap = float(phi_cap)
     rc = lib.lsmpqs_adjoint_run(C.addressof(options), C.addressof(rec), C.addressof(grid),
                                 theta.ctypes.data,
                                 dtheta.ctypes.data if gradient else None, C.byref(a))

@@ -99,3 +99,25 @@ def pack_case(dims, radius, porosity, seed, dtype=np.float64, clip=6.0):
     phi0 = initial_phi(g, mask)
     info = {"spheres": int(len(r)), "porosity": achieved, "seed": int(seed)}
     return g, mask.astype(dtype), phi0.astype(dtype), info
+
+
+def gaussian_random_field(shape, sigma, corr_len, seed):
+    """Stationary Gaussian field of standard deviation sigma whose correlation
+    falls as exp(-r^2 / (2 corr_len^2)) (r in voxels): seeded white noise
+    filtered with a periodic Gaussian of width corr_len / sqrt(2), rescaled to
+    unit sample deviation, times sigma."""
+    from scipy.ndimage import gaussian_filter
+    w = np.random.default_rng(seed).standard_normal(shape)
+    f = gaussian_filter(w, corr_len / np.sqrt(2.0), mode="wrap")
+    f -= f.mean()
+    return sigma * f / f.std()
+
+
+def contact_angle_case(n=128, radius=8.0, porosity=0.4, seed=1005, mean_deg=40.0,
+                       sigma_deg=5.0, corr_len=10.0, dtype=np.float64):
+    """(grid, mask, phi0, theta): an n^3 pack with a per-voxel contact angle
+    (radians) mean_deg + a Gaussian random field (sigma_deg, corr_len voxels),
+    both seeded by `seed`."""
+    g, mask, phi0, _ = pack_case((n, n, n), radius, porosity, seed, dtype)
+    deg = mean_deg + gaussian_random_field(mask.shape, sigma_deg, corr_len, seed)
+    return g, mask, phi0, (np.deg2rad(deg)).astype(dtype)
