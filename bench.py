@@ -34,10 +34,11 @@ THETA_DEG = 30.0
 
 
 def lean_arrays(dtype, mask, phi0):
-    """QSS_DataArrays record with only phi and mask (the rest NULL)."""
+    """QSS_DataArrays record with only phi and mask (the rest NULL), on copies:
+    the model drivers write both back (mask reinitialized), as the reference's do."""
     _, _, DataArrays = types_for(dtype)
     rec = DataArrays()
-    keep = [np.ascontiguousarray(phi0, dtype=dtype), np.ascontiguousarray(mask, dtype=dtype)]
+    keep = [np.array(phi0, dtype=dtype, order="C"), np.array(mask, dtype=dtype, order="C")]
     rec.phi = keep[0].ctypes.data
     rec.mask = keep[1].ctypes.data
     return rec, keep
@@ -157,6 +158,120 @@ def curve_leg(dtype, g64, mask, phi0, curv_steps, tmax, a, b):
             "nw_voxels": int((keep[0] < 0).sum())}
 
 
+def adjoint_leg(dtype, n=128, steps=200, checkpoint_every=20):
+    """Gradient of (S_w - 0.45)^2 with respect to a per-voxel contact-angle
+    field (40 deg + Gaussian random field, sigma 5 deg, correlation length
+    10; n^3 pack of radius 8, seed 1005) and b, through `steps` WENO5 TVD-RK3
+    steps at a0/b = 0.15, b = 0.04: host wall time of the forward map alone
+    and of forward + checkpointed reverse pass (lsmpqs_adjoint_run)."""
+    from lsmpqs_amd.adjoint import saturation_loss
+    from lsmpqs_amd.inputs import contact_angle_case
+    g, mask, phi0, theta = contact_angle_case(n, dtype=dtype)
+    o = default_options(dtype)
+    o.order_space_accur, o.order_time_accur = 5, 3
+    b = 0.04
+    a0, target = 0.15 * b, 0.45
+    saturation_loss(phi0, mask, theta, g, o, 2, a0, b, target, gradient=False)  # warm-up
+    w0 = time.perf_counter()
+    rf = saturation_loss(phi0, mask, theta, g, o, steps, a0, b, target, gradient=False)
+    tf = time.perf_counter() - w0
+    w0 = time.perf_counter()
+    rg = saturation_loss(phi0, mask, theta, g, o, steps, a0, b, target,
+                         checkpoint_every=checkpoint_every)
+    tg = time.perf_counter() - w0
+    nvox = n ** 3
+    return {"grid": n, "steps": steps, "checkpoint_every": checkpoint_every,
+            "forward_s": round(tf, 4), "gradient_s": round(tg, 4),
+            "gradient_over_forward": round(tg / tf, 2),
+            "forward_voxel_updates_per_s": round(nvox * steps / tf, 1),
+            "sw": round(rg.sw, 8), "loss": rg.loss, "dloss_db": rg.dloss_db,
+            "same_loss_as_forward": rf.loss == rg.loss}
+
+
+def batch_leg(tmax=20.0, seed=1004, sequential=True):
+    """64 subvolumes (dims U{48..192} per axis) of one seeded 768^3 pack,
+    each with its own 15-step curvature schedule (kappa from U[0.03,0.08] to
+    U[0.2,0.4], b = 0.04), FP32 HJ-ENO3 + TVD-RK3, theta 0, trapping on,
+    ENO3 read quirk off (undefined for non-cubic boxes): one batched
+    drainage run (lsmpqs_batch_model3d), and for comparison the same boxes
+    one after another through drain_model3d."""
+    from lsmpqs_amd.inputs import subvolume_batch
+    dt_ = np.float32
+    lib = hiplib.load(dt_)
+    boxes = subvolume_batch(seed=seed, dtype=dt_)
+    old = lib.lsmpqs_set_quirks(0)
+    lib.lsmpqs_set_quirks(old & ~1)
+    lib.lsmpqs_set_dt_slabs(16)
+    os.environ["LSMPQS_NO_STEP_FILES"] = "1"
+
+    def records():
+        out = []
+        for bx in boxes:
+            rec, keep = lean_arrays(dt_, bx["mask"], bx["phi0"])
+            o = default_options(dt_)
+            o.order_space_accur, o.order_time_accur = 3, 3
+            o.theta = 0
+            ks, ke = bx["kappa"]
+            o.b = bx["b"]
+            o.a = ks * o.b
+            o.dc = (ke - ks) / 14
+            o.amax = ke * o.b + 1e-3 * o.b * o.dc
+            o.tmax = tmax
+            o.check_connectivity = 1
+            g = make_grid(tuple(bx["dims"]), dt_)
+            out.append((g, o, rec, keep))
+        return out
+
+    def voxel_updates(items):
+        tot = 0
+        for g, o, rec, _ in items:
+            st = hiplib.Timing()
+            lib.lsmpqs_session_stats(C.byref(rec), C.byref(g), C.byref(st))
+            tot += int(st.inner_steps) * g.grid_dims[0] * g.grid_dims[1] * g.grid_dims[2]
+        return tot
+
+    libc = C.CDLL("libc.so.6")
+    libc.fopen.restype = C.c_void_p
+    libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
+    devnull = C.c_void_p(libc.fopen(b"/dev/null", b"w"))  # the drivers' per-step log lines
+    try:
+        items = records()
+        n = len(items)
+        opts = (C.c_void_p * n)(*[C.addressof(o) for _, o, _, _ in items])
+        recs = (C.c_void_p * n)(*[C.addressof(r) for _, _, r, _ in items])
+        grids = (C.c_void_p * n)(*[C.addressof(g) for g, _, _, _ in items])
+        st = hiplib.BatchStats()
+        w0 = time.perf_counter()
+        rc = lib.lsmpqs_batch_model3d(n, opts, recs, grids, None, 1, C.byref(st))
+        wall = time.perf_counter() - w0
+        if rc != 0:
+            raise hiplib.LsmpqsError(hiplib.last_error(lib))
+        vu = voxel_updates(items)
+        for _, _, rec, _ in items:
+            lib.lsmpqs_release(C.byref(rec))
+        out = {"boxes": n, "voxels": int(sum(np.prod(b["dims"]) for b in boxes)),
+               "batched_wall_s": round(wall, 3), "batched_gpu_s": round(st.batched_ms * 1e-3, 3),
+               "batched_steps": int(st.batched_steps), "box_steps": int(st.box_steps),
+               "brick_depth": int(st.brick_depth), "workgroups": int(st.workgroups),
+               "voxel_updates": vu,
+               "voxel_updates_per_s_batched_gpu": round(vu / (st.batched_ms * 1e-3), 1)}
+        if sequential:
+            items2 = records()
+            w0 = time.perf_counter()
+            for g, o, rec, _ in items2:
+                lib.drain_model3d(C.byref(o), C.byref(rec), C.byref(g), devnull)
+                hiplib.check(lib)
+            out["sequential_wall_s"] = round(time.perf_counter() - w0, 3)
+            out["same_result"] = all(np.array_equal(a[3][0], b[3][0])
+                                     for a, b in zip(items, items2))
+            for _, _, rec, _ in items2:
+                lib.lsmpqs_release(C.byref(rec))
+            out["speedup_over_sequential"] = round(out["sequential_wall_s"] / wall, 2)
+        return out
+    finally:
+        lib.lsmpqs_set_quirks(old)
+
+
 def cpu_leg(n, steps=8):
     """The reference build (oracle/_ref, OpenMP, FP64) on a bounded sample:
     one outer step of constCurvModel3d at n^3 whose pseudo-time window holds
@@ -260,6 +375,10 @@ def main():
     ap.add_argument("--precisions", default="f32,f64")
     ap.add_argument("--calib-copy", type=int, default=0, help="MiB copied once before the run")
     ap.add_argument("--curve-only", action="store_true", help="only the drainage-curve leg")
+    ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint leg")
+    ap.add_argument("--batch-only", action="store_true", help="only the batched-subvolume leg")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--batch", action="store_true", help="also run the batched-subvolume leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -267,6 +386,14 @@ def main():
         sys.exit(2)
     if args.calib_copy:
         calibration_copy(args.calib_copy)
+    if args.adjoint_only:
+        out = {k: adjoint_leg(np.float32 if k == "f32" else np.float64)
+               for k in args.precisions.split(",")}
+        print(json.dumps({"adjoint": out}), flush=True)
+        return
+    if args.batch_only:
+        print(json.dumps({"batch": batch_leg()}), flush=True)
+        return
     if args.curve_only:
         gc = make_grid((args.curve_n,) * 3, np.float64)
         mc, _, _, _ = sphere_pack_mask(gc, (6, 12), 0.36, 1002)
@@ -307,6 +434,14 @@ def main():
         del mc, pc
     v32 = r32["nvox"] * args.steps / (r32["elapsed_ms"] * 1e-3)
     v64 = r64["nvox"] * args.steps / (r64["elapsed_ms"] * 1e-3)
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = {"f32": adjoint_leg(np.float32), "f64": adjoint_leg(np.float64)}
+        print(f"# adjoint: {adjoint}", file=sys.stderr, flush=True)
+    batch = None
+    if args.batch:
+        batch = batch_leg()
+        print(f"# batch: {batch}", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_leg(args.cpu_n)
     line = {
         "metric": "voxel-updates/s per RK3 step (WENO5, theta 30, trapped masks), FP32 vs FP64, 1 MI355X",
@@ -329,6 +464,8 @@ def main():
                  "roofline": stage_roofline(r64, 8, "f64", n)},
         "fp32_over_fp64": round(v32 / v64, 3),
         "drainage_curve": curve,
+        "adjoint": adjoint,
+        "batch": batch,
         "stage_avg_ms": {"f32": [round(x, 4) for x in r32["stage_avg_ms"]],
                          "f64": [round(x, 4) for x in r64["stage_avg_ms"]]},
         "host_wall_s": {"f32": round(r32["wall_s"], 4), "f64": round(r64["wall_s"], 4)},

@@ -213,9 +213,18 @@ __global__ __launch_bounds__(1024) void ccl_border_kernel(Box bx, int* L, const 
   for (int e = threadIdx.x; e < NFACE; e += blockDim.x) border_job(bx, L, T, conn, set, plane, e);
 }
 
+// Final labels.  Each thread writes only its own entry: a find with path
+// halving here could re-point an entry another thread has already set to
+// its root at a non-root ancestor, after that thread's last write.  Reading
+// a concurrently finalised entry yields its parent or its root, both on the
+// way to the same root.
 __global__ void ccl_compress_kernel(int* L, int n) {
+  volatile int* V = L;
   for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < n; l += gridDim.x * blockDim.x) {
-    if (L[l] >= 0) L[l] = uf_find(L, l);
+    int r = V[l];
+    if (r < 0) continue;
+    for (int q = V[r]; q != r; q = V[r]) r = q;
+    V[l] = r;
   }
 }
 

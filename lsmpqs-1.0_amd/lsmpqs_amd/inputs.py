@@ -121,3 +121,75 @@ def contact_angle_case(n=128, radius=8.0, porosity=0.4, seed=1005, mean_deg=40.0
     g, mask, phi0, _ = pack_case((n, n, n), radius, porosity, seed, dtype)
     deg = mean_deg + gaussian_random_field(mask.shape, sigma_deg, corr_len, seed)
     return g, mask, phi0, (np.deg2rad(deg)).astype(dtype)
+
+
+def _stamp(mask, axes, c, r, clip, dx):
+    """mask = max(mask, r - |x - c|) over the voxels within r + clip of c."""
+    reach = r + clip
+    sl = []
+    for a, axis in enumerate(axes):
+        i0 = max(0, int(np.floor((c[a] - reach - axis[0]) / dx)))
+        i1 = min(len(axis), int(np.ceil((c[a] + reach - axis[0]) / dx)) + 1)
+        if i1 <= i0:
+            return
+        sl.append((i0, i1))
+    X = axes[0][sl[0][0]:sl[0][1]] - c[0]
+    Y = axes[1][sl[1][0]:sl[1][1]] - c[1]
+    Z = axes[2][sl[2][0]:sl[2][1]] - c[2]
+    d = r - np.sqrt(Z[:, None, None] ** 2 + Y[None, :, None] ** 2 + X[None, None, :] ** 2)
+    view = mask[sl[2][0]:sl[2][1], sl[1][0]:sl[1][1], sl[0][0]:sl[0][1]]
+    np.maximum(view, d, out=view)
+
+
+def boolean_pack(extent, radius, porosity, rng):
+    """Centres (m, 3) uniform in [0, extent)^3 and radii of a Boolean sphere
+    model whose expected porosity is `porosity`: m = -ln(porosity) V / E[4/3 pi r^3].
+    radius: a number or an (rmin, rmax) pair (radii uniform in it)."""
+    rmin, rmax = (float(radius), float(radius)) if np.isscalar(radius) else map(float, radius)
+    er3 = (rmax ** 4 - rmin ** 4) / (4 * (rmax - rmin)) if rmax > rmin else rmin ** 3
+    m = int(round(-np.log(porosity) * float(extent) ** 3 / (4.0 / 3.0 * np.pi * er3)))
+    centres = rng.uniform(0.0, float(extent), size=(m, 3))
+    radii = rng.uniform(rmin, rmax, size=m) if rmax > rmin else np.full(m, rmin)
+    return centres, radii
+
+
+def subvolume_mask(grid, origin, centres, radii, clip=6.0):
+    """Mask of the subvolume of a pack whose fill box starts at parent
+    coordinate `origin` (x, y, z): the union signed distance of the spheres
+    that reach its ghost box, clipped at -clip, float64."""
+    xs, ys, zs = ghost_coords(grid)
+    dx = float(grid.dx[0])
+    shift = np.array([float(grid.x_lo[a]) - origin[a] for a in range(3)])
+    c = centres + shift                      # sphere centres in the grid's coordinates
+    lo = np.array([xs[0], ys[0], zs[0]])
+    hi = np.array([xs[-1], ys[-1], zs[-1]])
+    reach = radii + clip
+    keep = np.all((c + reach[:, None] >= lo) & (c - reach[:, None] <= hi), axis=1)
+    N = [grid.grid_dims_ghostbox[a] for a in range(3)]
+    mask = np.full((N[2], N[1], N[0]), -clip, dtype=np.float64)
+    for cc, r in zip(c[keep], radii[keep]):
+        _stamp(mask, (xs, ys, zs), cc, r, clip, dx)
+    return mask
+
+
+def subvolume_batch(nbox=64, parent=768, lo=48, hi=192, radius=(6.0, 12.0), porosity=0.36,
+                    seed=1004, dtype=np.float32, b=0.04, kappa_start=(0.03, 0.08),
+                    kappa_end=(0.2, 0.4), clip=6.0):
+    """Subvolumes cut from one seeded parent^3 Boolean pack: per box its grid
+    (dims uniform in {lo..hi} per axis), mask, initial phi (the non-wetting
+    sphere at its own x-low face) and curvature schedule (kappa = a0/b from
+    U(kappa_start) to U(kappa_end)).  Returns a list of dicts."""
+    from .abi import make_grid
+    rng = np.random.default_rng(seed)
+    centres, radii = boolean_pack(parent, radius, porosity, rng)
+    out = []
+    for _ in range(nbox):
+        dims = [int(v) for v in rng.integers(lo, hi + 1, size=3)]
+        origin = [int(rng.integers(0, parent - d + 1)) for d in dims]
+        g = make_grid(dims, dtype=dtype)
+        mask = subvolume_mask(g, origin, centres, radii, clip)
+        phi0 = initial_phi(g, mask)
+        ks, ke = rng.uniform(*kappa_start), rng.uniform(*kappa_end)
+        out.append({"grid": g, "dims": dims, "origin": origin, "mask": mask.astype(dtype),
+                    "phi0": phi0.astype(dtype), "kappa": (ks, ke), "b": b})
+    return out

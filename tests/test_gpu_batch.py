@@ -70,3 +70,71 @@ def test_batched_boxes_match_single_runs(dt, drain):
     for (g1, o1, A1), (g2, o2, A2) in zip(single, batched):
         for f in ("phi", "mask_w", "mask_nw"):
             assert np.array_equal(A1[f], A2[f]), f
+
+
+@pytest.mark.gpu
+def test_sixty_four_subvolumes_of_a_768_pack():
+    """The full batched case: 64 subvolumes (dims uniform in {48..192} per
+    axis) of one seeded 768^3 pack (radius U[6,12], porosity 0.36, seed 1004),
+    each with its own 15-step curvature schedule (kappa from U[0.03,0.08] to
+    U[0.2,0.4], b = 0.04), FP32 HJ-ENO3 + TVD-RK3, trapping on, ENO3 read
+    quirk off.  The batched drainage equals the 64 single-box runs bit for bit
+    (phi and both trapped-phase masks)."""
+    from lsmpqs_amd.inputs import subvolume_batch
+    from lsmpqs_amd.abi import default_options, make_grid, types_for
+    dt = np.float32
+    print("case: batch 64 subvolumes of a 768^3 pack, ENO3 RK3 fp32", flush=True)
+    lib = hiplib.load(dt)
+    boxes = subvolume_batch(seed=1004, dtype=dt)
+    _, _, DataArrays = types_for(dt)
+
+    def records():
+        out = []
+        for bx in boxes:
+            # copies: drain_model3d writes phi and the reinitialized mask back
+            keep = {"phi": bx["phi0"].copy(), "mask": bx["mask"].copy(),
+                    "mask_w": np.zeros_like(bx["mask"]), "mask_nw": np.zeros_like(bx["mask"])}
+            rec = DataArrays()
+            for f, arr in keep.items():
+                setattr(rec, f, arr.ctypes.data)
+            o = default_options(dt)
+            o.order_space_accur, o.order_time_accur = 3, 3
+            ks, ke = bx["kappa"]
+            o.b = bx["b"]
+            o.a = ks * o.b
+            o.dc = (ke - ks) / 14
+            o.amax = ke * o.b + 1e-3 * o.b * o.dc
+            o.tmax = 20.0
+            out.append((make_grid(tuple(bx["dims"]), dt), o, rec, keep))
+        return out
+
+    old = lib.lsmpqs_set_quirks(0)
+    lib.lsmpqs_set_quirks(old & ~1)
+    lib.lsmpqs_set_dt_slabs(16)
+    os.environ["LSMPQS_NO_STEP_FILES"] = "1"
+    try:
+        single, batched = records(), records()
+        for g, o, rec, _ in single:
+            lib.drain_model3d(C.byref(o), C.byref(rec), C.byref(g), DEVNULL)
+            hiplib.check(lib)
+            lib.lsmpqs_release(C.byref(rec))
+        n = len(batched)
+        opts = (C.c_void_p * n)(*[C.addressof(o) for _, o, _, _ in batched])
+        recs = (C.c_void_p * n)(*[C.addressof(r) for _, _, r, _ in batched])
+        grids = (C.c_void_p * n)(*[C.addressof(g) for g, _, _, _ in batched])
+        st = hiplib.BatchStats()
+        rc = lib.lsmpqs_batch_model3d(n, opts, recs, grids, None, 1, C.byref(st))
+        assert rc == 0, hiplib.last_error(lib)
+        for _, _, rec, _ in batched:
+            lib.lsmpqs_release(C.byref(rec))
+    finally:
+        lib.lsmpqs_set_quirks(old)
+    print(f"batched steps {st.batched_steps}, box steps {st.box_steps}, brick depth "
+          f"{st.brick_depth}, workgroups {st.workgroups}", flush=True)
+    assert st.boxes == 64 and st.box_steps > st.batched_steps
+    moved = 0
+    for (_, _, _, k1), (_, _, _, k2), bx in zip(single, batched, boxes):
+        for f in ("phi", "mask_w", "mask_nw"):
+            assert np.array_equal(k1[f], k2[f]), f
+        moved += int(np.any(k1["phi"] != bx["phi0"]))
+    assert moved == 64

@@ -175,3 +175,40 @@ def test_fp32_exact_order_matches_fp32_oracle():
     sh = saturation(Ah["phi"], Ah["mask"], g32)
     print(f"  saturation fp32-oracle {s32:.8f} fp32-gpu-exact {sh:.8f}")
     assert abs(sh - s32) < 1e-5
+
+
+def _canonical_labels(binv):
+    """Largest linear index of each 26-connected component (0 on background),
+    from scipy's labelling."""
+    from scipy import ndimage
+    lab, n = ndimage.label(binv, structure=np.ones((3, 3, 3)))
+    flat = lab.ravel()
+    top = np.full(n + 1, -1, dtype=np.int64)
+    np.maximum.at(top, flat, np.arange(flat.size))
+    return np.where(flat > 0, top[flat], 0).reshape(binv.shape).astype(np.int32)
+
+
+@pytest.mark.parametrize("shape", [(96, 187, 90), (178, 107, 122)])
+def test_labels_of_blob_fields_are_stable(shape):
+    """Labels of thresholded smooth random fields (long, branching components
+    across many tiles), five times each, against scipy's partition with the
+    reference's root rule.  The final compression pass once let a finished
+    entry be re-pointed at a non-root ancestor by another thread's path
+    halving; every run must now give the exact labels."""
+    from scipy.ndimage import gaussian_filter
+    from lsmpqs_amd.abi import make_grid, Arrays
+    print(f"case labels blobs {shape}", flush=True)
+    lib = hiplib.load(np.float32)
+    g = make_grid(shape, np.float32)
+    rng = np.random.default_rng(sum(shape))
+    for thr in (0.0, 0.4):
+        A = Arrays(g, np.float32)
+        f = gaussian_filter(rng.standard_normal(A.shape), 2.0)
+        A["phi_bin"] = (f / f.std() > thr).astype(np.uint8)
+        want = _canonical_labels(A["phi_bin"])
+        for rep in range(5):
+            A["connectivity"] = -7
+            lib.findConnectivity3d(A.ptr(), C.byref(g))
+            hiplib.check(lib)
+            assert np.array_equal(A["connectivity"], want), (thr, rep)
+        lib.lsmpqs_release(A.ptr())
