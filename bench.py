@@ -378,7 +378,7 @@ def main():
     ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint leg")
     ap.add_argument("--batch-only", action="store_true", help="only the batched-subvolume leg")
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--batch", action="store_true", help="also run the batched-subvolume leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-subvolume leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -439,7 +439,7 @@ def main():
         adjoint = {"f32": adjoint_leg(np.float32), "f64": adjoint_leg(np.float64)}
         print(f"# adjoint: {adjoint}", file=sys.stderr, flush=True)
     batch = None
-    if args.batch:
+    if not args.no_batch:
         batch = batch_leg()
         print(f"# batch: {batch}", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_leg(args.cpu_n)

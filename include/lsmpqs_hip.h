@@ -165,11 +165,12 @@ int lsmpqs_batch_model3d(int n, Options **options, QSS_DataArrays **p, Grid **g,
    ~(use_var_theta) being nonzero for 0 and 1) at a fixed time step: nsteps TVD-RK steps of order options->order_time_accur with
    the scheme options->order_space_accur (2, 3 or 5), the library's ghost
    extrapolation and the imposition max(phi, mask - overlap) after each
-   step and the far-field cap phi = min(phi, phi_cap) (no counterpart in
-   the reference, whose inner steps are few between reinitializations: over
-   many steps the signed extrapolation grows phi without bound where the
-   mask's advection velocity enters the box through a face, in solid far
-   from the interface), then
+   step and the cap phi = min(phi, phi_cap) (no counterpart in the
+   reference, whose inner steps are few between reinitializations: over
+   many steps the signed extrapolation grows phi exponentially where the
+   mask's advection velocity enters the box through a face, in solid away
+   from the interface; the default 6 dx, twice the WENO5 stencil reach,
+   leaves the stencils of the |phi| < 3 dx band alone), then
      S_w  = 1 - V(phi < 0) / V(mask < 0)   (smoothed Heaviside, width
                                             options->eps_coefficient dx)
      loss = (S_w - target_sw)^2.
@@ -193,7 +194,7 @@ typedef struct {
   double a0;                  /* in: pressure coefficient */
   double b;                   /* in: curvature coefficient */
   double target_sw;           /* in: target wetting saturation */
-  double phi_cap;             /* in: far-field cap; <= 0: the largest initial phi */
+  double phi_cap;             /* in: cap on phi after every step; <= 0: 6 dx */
   double loss;                /* out */
   double sw;                  /* out: final wetting saturation */
   double dt_used;             /* out */

@@ -365,12 +365,13 @@ __global__ void adj_impose_kernel(Box bx, real* lam, const real* out, const real
     if (out[idx] == floor[idx]) lam[idx] = 0;
   }
 }
-// Far-field cap after every step, u = min(u, cap) over the ghost box, and its
+// Cap after every step, u = min(u, cap) over the ghost box, and its
 // transpose (lam = 0 where the cap was taken).  Without reinitialization
 // between steps, the signed extrapolation at a face where the mask's
 // advection velocity points into the box feeds a downwind difference, and
-// phi grows without bound there (in solid, far from the interface); the cap
-// keeps such voxels finite without touching the interface region.
+// phi grows exponentially there (in solid, away from the interface).  A cap
+// of twice the WENO5 stencil reach leaves every stencil of the |phi| < 3dx
+// band of a distance-like field untouched and bounds that growth.
 __global__ void adj_cap_kernel(real* u, real cap, int64_t n) {
   for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n;
        l += (int64_t)gridDim.x * blockDim.x) {
@@ -654,13 +655,7 @@ extern "C" int lsmpqs_adjoint_run(Options* o, QSS_DataArrays* p, Grid* g, const 
     }
     if (!(dt > 0) || !isfinite((double)dt)) throw std::runtime_error("invalid time step");
     A.dt = dt;
-    if (args->phi_cap > 0) {
-      A.cap = (real)args->phi_cap;
-    } else {
-      real m = p->phi[0];
-      for (int64_t l = 1; l < n; ++l) m = p->phi[l] > m ? p->phi[l] : m;
-      A.cap = m;
-    }
+    A.cap = args->phi_cap > 0 ? (real)args->phi_cap : real(6) * g->dx[0];
     args->cap_used = (double)A.cap;
     ACHECK(hipMemcpyAsync(s->dt, &dt, sizeof(real), hipMemcpyHostToDevice, s->st));
     // forward with checkpoints every k steps

@@ -57,8 +57,8 @@ def saturation_loss(phi0, mask, theta, grid, options, nsteps, a0, b, target_sw, 
     phi0, mask, theta: arrays over the ghost box (shape nz+2g, ny+2g, nx+2g)
     of the library's QSSLIB_REAL; options: the Options record (scheme from
     order_space_accur, RK order from order_time_accur, overlap,
-    eps_coefficient); dt <= 0 takes the CFL rule on phi0; phi_cap <= 0 caps
-    phi at the largest value of phi0 after every step.  Raises
+    eps_coefficient); dt <= 0 takes the CFL rule on phi0; phi is capped at
+    phi_cap after every step (<= 0: 6 dx).  Raises
     hiplib.LsmpqsError on a library error."""
     dtype = np.dtype(phi0.dtype)
     lib = _lib(dtype)

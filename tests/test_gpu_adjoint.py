@@ -240,6 +240,6 @@ def test_contact_angle_field_128():
     derr = np.abs(d32 - d64).max() / np.abs(d64).max()
     print(f"fp32 directional derivatives vs fp64: max error {derr:.2e}", flush=True)
     assert abs(r32.sw - r64.sw) <= 1e-4
-    assert rel <= 1e-2
+    assert rel <= 2e-3
     assert derr <= FD_TOL
-    assert abs(r32.dloss_db - r64.dloss_db) <= 1e-2 * abs(r64.dloss_db)
+    assert abs(r32.dloss_db - r64.dloss_db) <= 1e-3 * abs(r64.dloss_db)
