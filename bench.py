@@ -132,7 +132,7 @@ def stage_roofline(res, itemsize, key, n):
     return out
 
 
-def curve_leg(dtype, g64, mask, phi0, curv_steps, tmax, a, b):
+def curve_leg(dtype, g64, mask, phi0, curv_steps, tmax, a, b, step_files=False):
     """Drainage-curve time: drain_model3d end to end on the GPU (set-up,
     curv_steps curvature steps, each running outer steps until the reference's
     stopping tests fire or pseudo-time tmax; step files off), host wall time
@@ -145,10 +145,17 @@ def curve_leg(dtype, g64, mask, phi0, curv_steps, tmax, a, b):
     o.amax = a + b * o.dc * (curv_steps - 0.5)
     o.tmax = tmax
     lib.lsmpqs_set_dt_slabs(16)
-    os.environ["LSMPQS_NO_STEP_FILES"] = "1"
-    w0 = time.perf_counter()
-    lib.drain_model3d(C.byref(o), C.byref(rec), C.byref(g), None)
-    wall = time.perf_counter() - w0
+    os.environ["LSMPQS_NO_STEP_FILES"] = "0" if step_files else "1"
+    cwd = os.getcwd()
+    if step_files:  # data_step_N.gz of every curvature step, into a scratch directory
+        import tempfile
+        os.chdir(tempfile.mkdtemp(prefix="lsm_steps_"))
+    try:
+        w0 = time.perf_counter()
+        lib.drain_model3d(C.byref(o), C.byref(rec), C.byref(g), None)
+        wall = time.perf_counter() - w0
+    finally:
+        os.chdir(cwd)
     hiplib.check(lib)
     st = hiplib.Timing()
     lib.lsmpqs_session_stats(C.byref(rec), C.byref(g), C.byref(st))
@@ -272,6 +279,88 @@ def batch_leg(tmax=20.0, seed=1004, sequential=True):
         lib.lsmpqs_set_quirks(old)
 
 
+def precision_leg(n=512, curv_steps=40, tmax=20.0, seed=1003):
+    """FP32 against FP64 over one drainage curve on the GPU: n^3 pack (radius
+    U[10,20], porosity 0.35), WENO5 + TVD-RK3, theta 39.999 deg (both
+    precisions then take overlap 0.3 dx; at exactly 40 deg the reference's
+    threshold test rounds differently in the two precisions), trapping on,
+    curv_steps curvature steps of 0.25 b from a = 0.02 (b = 0.04), pseudo-time
+    tmax per step.  Reports wall time per precision and, on the final states,
+    the saturation difference, phi L-infinity in the |phi| < 3 dx band of the
+    FP64 state, the voxels whose phase (phi < 0) differs, and the trapped-mask
+    differences."""
+    g64 = make_grid((n, n, n), np.float64)
+    mask, _, radii, por = sphere_pack_mask(g64, (10, 20), 0.35, seed)
+    phi0 = initial_phi(g64, mask)
+    res, states = {}, {}
+    for key, dt_ in (("f32", np.float32), ("f64", np.float64)):
+        lib = hiplib.load(dt_)
+        g = make_grid((n, n, n), dt_)
+        _, _, DataArrays = types_for(dt_)
+        keep = {"phi": np.array(phi0, dtype=dt_), "mask": np.array(mask, dtype=dt_),
+                "mask_w": np.zeros(mask.shape, dt_), "mask_nw": np.zeros(mask.shape, dt_)}
+        rec = DataArrays()
+        for f, arr in keep.items():
+            setattr(rec, f, arr.ctypes.data)
+        o = options(dt_, 0.02, 0.04)
+        o.theta = 39.999 * 3.14159 / 180
+        o.dc = 0.25
+        o.amax = o.a + o.b * o.dc * (curv_steps - 0.5)
+        o.tmax = tmax
+        lib.lsmpqs_set_dt_slabs(16)
+        os.environ["LSMPQS_NO_STEP_FILES"] = "1"
+        w0 = time.perf_counter()
+        lib.drain_model3d(C.byref(o), C.byref(rec), C.byref(g), None)
+        wall = time.perf_counter() - w0
+        hiplib.check(lib)
+        st = hiplib.Timing()
+        lib.lsmpqs_session_stats(C.byref(rec), C.byref(g), C.byref(st))
+        lib.lsmpqs_release(C.byref(rec))
+        res[key] = {"wall_s": round(wall, 3), "outer_steps": int(st.outer_steps),
+                    "inner_steps": int(st.inner_steps),
+                    "inner_loop_s": round(st.inner_loop_ms * 1e-3, 3), "overlap": float(o.overlap)}
+        states[key] = keep
+        print(f"# precision {key}: {res[key]}", file=sys.stderr, flush=True)
+    fb = (slice(3, n + 3),) * 3
+    p32 = states["f32"]["phi"][fb].astype(np.float64)
+    p64 = states["f64"]["phi"][fb]
+    m = mask[fb]
+
+    def sat(p):
+        q = p / 1.5
+        v = np.where(p < -1.5, 1.0, np.where(p < 1.5, 0.5 * (1 - q - np.sin(np.pi * q) / np.pi), 0.0))
+        qm = m / 1.5
+        vm = np.where(m < -1.5, 1.0, np.where(m < 1.5, 0.5 * (1 - qm - np.sin(np.pi * qm) / np.pi), 0.0))
+        return float(v.sum() / vm.sum())
+
+    band = np.abs(p64) < 3.0
+    neg64 = p64 < 0
+    iface = np.zeros_like(neg64)
+    for ax in range(3):
+        d = neg64 != np.roll(neg64, 1, axis=ax)
+        iface |= d | np.roll(d, -1, axis=ax)
+    flips = int(((p32 < 0) != neg64).sum())
+    trap_diff = ((states["f32"]["mask_nw"][fb] > 0) != (states["f64"]["mask_nw"][fb] > 0)) | \
+                ((states["f32"]["mask_w"][fb] > 0) != (states["f64"]["mask_w"][fb] > 0))
+    dev = np.abs(p32 - p64)
+    same_trap = band & ~trap_diff
+    out = {"grid": n, "spheres": int(len(radii)), "porosity": round(float(por), 4),
+           "curvature_steps": curv_steps, "tmax": tmax, "f32": res["f32"], "f64": res["f64"],
+           "fp64_over_fp32_wall": round(res["f64"]["wall_s"] / res["f32"]["wall_s"], 3),
+           "sat_nw_f32": sat(p32), "sat_nw_f64": sat(p64),
+           "sat_abs_diff": abs(sat(p32) - sat(p64)),
+           "phi_linf_band": float(dev[band].max()),
+           "phi_linf_band_same_trapping": float(dev[same_trap].max()) if same_trap.any() else 0.0,
+           "phi_p999_band": float(np.quantile(dev[band], 0.999)),
+           "band_voxels": int(band.sum()),
+           "phase_mismatch_voxels": flips, "interface_voxels": int(iface.sum()),
+           "trapped_nw_mismatch": int(((states["f32"]["mask_nw"][fb] > 0)
+                                       != (states["f64"]["mask_nw"][fb] > 0)).sum()),
+           "trapped_w_mismatch": int(((states["f32"]["mask_w"][fb] > 0)
+                                      != (states["f64"]["mask_w"][fb] > 0)).sum())}
+    return out
+
+
 def cpu_leg(n, steps=8):
     """The reference build (oracle/_ref, OpenMP, FP64) on a bounded sample:
     one outer step of constCurvModel3d at n^3 whose pseudo-time window holds
@@ -375,10 +464,15 @@ def main():
     ap.add_argument("--precisions", default="f32,f64")
     ap.add_argument("--calib-copy", type=int, default=0, help="MiB copied once before the run")
     ap.add_argument("--curve-only", action="store_true", help="only the drainage-curve leg")
+    ap.add_argument("--curve-step-files", action="store_true",
+                    help="drainage-curve leg with data_step_N.gz files written")
     ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint leg")
     ap.add_argument("--batch-only", action="store_true", help="only the batched-subvolume leg")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-subvolume leg")
+    ap.add_argument("--precision-only", action="store_true",
+                    help="only the FP32-vs-FP64 drainage-curve comparison (--n, --curve-steps, "
+                         "--curve-tmax)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -391,6 +485,10 @@ def main():
                for k in args.precisions.split(",")}
         print(json.dumps({"adjoint": out}), flush=True)
         return
+    if args.precision_only:
+        print(json.dumps({"precision": precision_leg(args.n, args.curve_steps, args.curve_tmax)}),
+              flush=True)
+        return
     if args.batch_only:
         print(json.dumps({"batch": batch_leg()}), flush=True)
         return
@@ -401,7 +499,8 @@ def main():
         out = {}
         for key in args.precisions.split(","):
             dt_ = np.float32 if key == "f32" else np.float64
-            out[key] = curve_leg(dt_, gc, mc, pc, args.curve_steps, args.curve_tmax, 0.02, 0.04)
+            out[key] = curve_leg(dt_, gc, mc, pc, args.curve_steps, args.curve_tmax, 0.02, 0.04,
+                                 args.curve_step_files)
         print(json.dumps({"drainage_curve": out, "grid": args.curve_n}), flush=True)
         return
     precs = args.precisions.split(",")

@@ -14,7 +14,9 @@
 #include <algorithm>
 #include <stdexcept>
 #include <vector>
+#include <memory>
 #include "lsm_batch.h"
+#include "lsm_gz.h"
 #include "lsm_session.h"
 #include "lsmpqs_hip.h"
 
@@ -484,6 +486,7 @@ void model_run(Options* o, QSS_DataArrays* p, Grid* g, FILE* fp, bool drain) {
   model_setup(s, o, p, g, drain, var);
   real a0 = o->a, da = o->b * o->dc;
   int step = o->init_step + 1;
+  std::unique_ptr<StepWriter> writer;
   while (drain ? (a0 < o->amax) : (a0 > o->amin)) {
     printf("\nStep %d: a = %f\n", step, (double)a0);
     if (fp) fprintf(fp, "\nStep %d: a = %f\n", step, (double)a0);
@@ -495,14 +498,15 @@ void model_run(Options* o, QSS_DataArrays* p, Grid* g, FILE* fp, bool drain) {
     if (o->reservoir_inlet) DCHECK(launch_min(s->phi, s->phi_extra, s->n, s->st));
     DCHECK(launch_impose(s->phi, s->mask, s->phi, 0, s->n, s->st));
     if (step_files_enabled()) {
+      // written by a worker thread from a snapshot while the next step runs
       char fname[256];
       snprintf(fname, sizeof fname, "data_step_%d", step);
-      down(s, p->phi, s->phi);
-      sync(s);
-      writeDataArrayQSS(p->phi, g, fname, 1);
+      if (!writer) writer.reset(new StepWriter(bytes(s), g->grid_dims_ghostbox));
+      writer->submit(s->st, s->phi, fname);
     }
     step += 1;
   }
+  if (writer) writer->finish();
   down(s, p->phi, s->phi);
   down(s, p->mask, s->mask);
   down(s, p->mask_w, s->mask_w);

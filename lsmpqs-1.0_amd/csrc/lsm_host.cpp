@@ -3,6 +3,8 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <string>
+#include "lsm_gz.h"
 #include "lsmpqs_hip.h"
 
 extern "C" {
@@ -111,17 +113,22 @@ void destroyQSSDataArrays(QSS_DataArrays* p) {
 // Headered array file: int[3] ghost-box dims, then QSSLIB_REAL[N]; zip_status
 // 1 compresses with gzip afterwards (reference: src/utilities/qss_data_arrays.c:214-228,
 // src/utilities/qss_file.c:50-64)
+// The gzip file is written directly, deflated on several threads
+// (lsm::write_array_gz), and the uncompressed name removed, which leaves the
+// files the reference's write + `gzip -f` leaves.
 void writeDataArrayQSS(QSSLIB_REAL* data, Grid* g, char* file_name, int zip_status) {
+  if (zip_status == 1) {
+    if (!lsm::write_array_gz(std::string(file_name) + ".gz", g->grid_dims_ghostbox, data,
+                             sizeof(QSSLIB_REAL) * (size_t)g->num_gridpts))
+      fprintf(stderr, "lsmpqs_hip: cannot write %s.gz\n", file_name);
+    remove(file_name);
+    return;
+  }
   FILE* fp = fopen(file_name, "wb");
   if (!fp) return;
   fwrite(g->grid_dims_ghostbox, sizeof(int), 3, fp);
   fwrite(data, sizeof(QSSLIB_REAL), (size_t)g->num_gridpts, fp);
   fclose(fp);
-  if (zip_status == 1) {
-    char cmd[600];
-    snprintf(cmd, sizeof cmd, "gzip -f '%s'", file_name);
-    if (system(cmd) != 0) fprintf(stderr, "lsmpqs_hip: gzip failed for %s\n", file_name);
-  }
 }
 
 // reference: src/utilities/qss_data_arrays.c:347-382 (a .gz name is gunzipped

@@ -199,3 +199,34 @@ def test_outfile_sets_path(dtype, tmp_path):
     assert o.contents.outfile == b"results/deep/log.txt"
     assert o.contents.path == b"results/deep/"
     ours.destroyOptions(o)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("dims", [(7, 5, 3), (262, 262, 40)])
+def test_gzip_array_files_read_back(dtype, dims, tmp_path):
+    """writeDataArrayQSS(..., 1) deflates on several threads into a gzip file of
+    independent members (more than one above 8 MB); gunzip, Python's gzip and
+    readDataArrayQSS read back the reference layout (int32 dims, values) and
+    no uncompressed file is left, as after the reference's write + gzip -f
+    (reference: src/utilities/qss_data_arrays.c:214-228)."""
+    lib = hiplib.load(dtype)
+    Grid, _, _ = types_for(dtype)
+    g = Grid()
+    for a in range(3):
+        g.grid_dims_ghostbox[a] = dims[a]
+    g.num_gridpts = int(np.prod(dims))
+    data = np.random.default_rng(5).standard_normal(g.num_gridpts).astype(dtype)
+    name = str(tmp_path / "arr")
+    lib.writeDataArrayQSS(data.ctypes.data_as(C.c_void_p), C.byref(g), name.encode(), 1)
+    assert os.path.exists(name + ".gz") and not os.path.exists(name)
+    raw = gzip.open(name + ".gz").read()
+    assert np.array_equal(np.frombuffer(raw[:12], np.int32), np.array(dims, np.int32))
+    assert np.array_equal(np.frombuffer(raw[12:], dtype), data)
+    back_dims = (C.c_int * 3)()
+    lib.readDataArrayQSS.restype = C.c_void_p
+    ptr = lib.readDataArrayQSS(back_dims, (name + ".gz").encode())
+    assert ptr and list(back_dims) == list(dims)
+    got = np.ctypeslib.as_array(C.cast(ptr, C.POINTER(C.c_float if dtype == np.float32
+                                                      else C.c_double)), (g.num_gridpts,))
+    assert np.array_equal(got, data)
+    C.CDLL("libc.so.6").free(C.c_void_p(ptr))
