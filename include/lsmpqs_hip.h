@@ -88,6 +88,8 @@ void setThetaOverlap(QSS_DataArrays *p, Grid *g, Options *options);
 Options *createOptionsDefault(void);
 void destroyOptions(Options *options);
 void writeDataArrayQSS(QSSLIB_REAL *data, Grid *grid, char *file_name, int zip_status);
+/* headerless QSSLIB_REAL[num_gridpts] (reference: src/utilities/qss_data_arrays.h:151) */
+void writeDataArrayRaw(QSSLIB_REAL *data, Grid *grid, char *file_name, int zip_status);
 QSSLIB_REAL *readDataArrayQSS(int *grid_dims_ghostbox, char *file_name);
 /* option file "key value" lines (reference: src/utilities/qss_options.h:227,
    qss_options.c:118-494; keys recognised by their first letters as there) */
@@ -203,6 +205,32 @@ typedef struct {
 } lsmpqs_adjoint_args;
 int lsmpqs_adjoint_run(Options *options, QSS_DataArrays *p, Grid *g, const QSSLIB_REAL *theta,
                        QSSLIB_REAL *dloss_dtheta, lsmpqs_adjoint_args *args);
+
+/* ---- pre- and post-processing (the computations of the reference's
+ *      src/applications programs; files stay with the caller) ---- */
+/* createMaskUBC (src/applications/createMaskUBC.c:54-116): a segmented image
+   (nx*ny*nz bytes, x fastest, 255 = grain) becomes the grid createMaskUBC
+   builds (createGridSetGridDims, x_lo = 3 options->dx, accuracy
+   options->accuracy_id, which must give 3 ghost layers) and the mask +dx in
+   grain, -dx in pore, 0 on ghosts, reinitialized by the subcell-fix iteration
+   on the GPU until options->tmax_r.  *grid_out (destroyGrid) and *mask_out
+   (free) are allocated here.  Returns 0 or -2. */
+int lsmpqs_mask_from_image(const unsigned char *image, int nx, int ny, int nz, Options *options,
+                           Grid **grid_out, QSSLIB_REAL **mask_out);
+/* createInitFromMask (src/applications/createInitFromMask.c:75-100): the
+   non-wetting sphere (radius 0.7 L centred 0.49 L before the x-low face; with
+   options->center_inlet radius 90 dx at the centre, x centre taken from the y
+   bounds as the reference does), then max(mask, phi), into data (ghost box).
+   Returns 0 or -2. */
+int lsmpqs_init_from_mask(const QSSLIB_REAL *mask, Grid *g, Options *options, QSSLIB_REAL *data);
+/* makeDiffSteps (src/applications/makeDiffSteps.c:70-140): diff =
+   max(-step_a, step_b - dx/2) over the ghost box; its negative voxels plus
+   the inlet/outlet ghost slabs are labelled on the GPU (26-neighbourhood); if
+   the inlet and outlet voxels at mid j, k share a component, connected_only
+   (may be NULL) receives diff with the sign flipped where diff <= 0 outside
+   it, and 1 is returned ("percolates"); 0 when they do not, -2 on error. */
+int lsmpqs_diff_steps(const QSSLIB_REAL *step_a, const QSSLIB_REAL *step_b, Grid *g,
+                      QSSLIB_REAL *diff, QSSLIB_REAL *connected_only);
 
 /* free the device state kept for record p */
 void lsmpqs_release(QSS_DataArrays *p);

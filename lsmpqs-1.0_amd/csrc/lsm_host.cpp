@@ -131,6 +131,22 @@ void writeDataArrayQSS(QSSLIB_REAL* data, Grid* g, char* file_name, int zip_stat
   fclose(fp);
 }
 
+// Headerless array file: QSSLIB_REAL[N] over the ghost box; zip_status 1
+// gzips it (reference: src/utilities/qss_data_arrays.c:231-242)
+void writeDataArrayRaw(QSSLIB_REAL* data, Grid* g, char* file_name, int zip_status) {
+  const size_t bytes = sizeof(QSSLIB_REAL) * (size_t)g->num_gridpts;
+  if (zip_status == 1) {
+    if (!lsm::gz_write(std::string(file_name) + ".gz", {{data, bytes}}, 0))
+      fprintf(stderr, "lsmpqs_hip: cannot write %s.gz\n", file_name);
+    remove(file_name);
+    return;
+  }
+  FILE* fp = fopen(file_name, "wb");
+  if (!fp) return;
+  fwrite(data, 1, bytes, fp);
+  fclose(fp);
+}
+
 // reference: src/utilities/qss_data_arrays.c:347-382 (a .gz name is gunzipped
 // in place and compressed again afterwards)
 QSSLIB_REAL* readDataArrayQSS(int* dims, char* file_name) {

@@ -44,7 +44,8 @@ EXPORTS = (
     "drainTop", "imbibeTop", "createOptionsFromInputFile", "printOptions", "createGridSetDx",
     "createGridSetDxDyDz", "createGridSetGridDims", "destroyGrid", "printGrid",
     "writeGridToBinaryFile", "readGridFromBinaryFile", "reinitializeMedium3d",
-    "lsmpqs_batch_model3d", "lsmpqs_adjoint_run",
+    "lsmpqs_batch_model3d", "lsmpqs_adjoint_run", "writeDataArrayRaw", "lsmpqs_mask_from_image",
+    "lsmpqs_init_from_mask", "lsmpqs_diff_steps",
 )
 
 

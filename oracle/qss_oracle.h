@@ -99,6 +99,14 @@ orc_real orc_constCurvModel3dNoVar(Options *o, QSS_DataArrays *p, Grid *g, FILE 
 void orc_drain_model3d(Options *o, QSS_DataArrays *p, Grid *g, FILE *fp, int nthreads);
 void orc_imbibe_model3d(Options *o, QSS_DataArrays *p, Grid *g, FILE *fp, int nthreads);
 void orc_model_setup3d(Options *o, QSS_DataArrays *p, Grid *g, int drain);
+/* pre- and post-processing computations (reference: src/applications/
+ * createMaskUBC.c, createInitFromMask.c, makeDiffSteps.c) */
+void orc_mask_from_image(const unsigned char *image, int nx_vox, int ny_vox, int nz_vox,
+                         Grid *g, Options *o, orc_real *mask);
+void orc_init_from_mask(const orc_real *mask, Grid *g, Options *o, orc_real *data);
+int orc_diff_steps(const orc_real *a, const orc_real *b, Grid *g, orc_real *diff,
+                   orc_real *connected_only);
+
 void orc_createSphere(orc_real *phi, orc_real cx, orc_real cy, orc_real cz, orc_real r,
                       int inside_flag, Grid *g);
 void orc_createBox(orc_real *phi, orc_real x0, orc_real y0, orc_real z0,

@@ -507,3 +507,101 @@ void orc_imbibe_model3d(Options *o, QSS_DataArrays *p, Grid *g, FILE *fp, int nt
 {
   model_loop(o, p, g, fp, nthreads, 0);
 }
+
+/* ------------------------------------------------------------------------
+ * Pre- and post-processing applications (their computations, without the
+ * file handling of their main programs).
+ */
+
+/* createMaskUBC's mask: +dx on image voxels equal to 255 (grain), -dx on
+ * the others (pore), 0 on ghosts, then the subcell-fix reinitialization;
+ * g is the grid createMaskUBC builds (createGridSetGridDims with x_lo = 3 dx).
+ * reference: src/applications/createMaskUBC.c:87-116,
+ * src/utilities/qss_general_util.c:261-306 */
+void orc_mask_from_image(const unsigned char *image, int nx_vox, int ny_vox, int nz_vox,
+                         Grid *g, Options *o, R *mask)
+{
+  int nx_gb = g->grid_dims_ghostbox[0], nxy_gb = nx_gb * g->grid_dims_ghostbox[1];
+  int i, j, k;
+  long l;
+  (void)nz_vox;
+  for (l = 0; l < npts(g); l++) mask[l] = 0;
+  for (k = g->klo_fb; k <= g->khi_fb; k++)
+    for (j = g->jlo_fb; j <= g->jhi_fb; j++)
+      for (i = g->ilo_fb; i <= g->ihi_fb; i++) {
+        long idx = (long)(i - 3) + (long)(j - 3) * nx_vox + (long)(k - 3) * nx_vox * ny_vox;
+        long gb = i + (long)j * nx_gb + (long)k * nxy_gb;
+        mask[gb] = image[idx] == 255 ? o->dx : -o->dx;
+      }
+  orc_reinitializeSubcellFix3d(mask, g, o);
+}
+
+/* createInitFromMask's initial phi: the non-wetting sphere (radius 0.7 L
+ * centred 0.49 L before the x-low face, or radius 90 dx at the centre with
+ * center_inlet, whose x centre the reference takes from the y bounds),
+ * then max(mask, phi).  reference: src/applications/createInitFromMask.c:75-100 */
+void orc_init_from_mask(const R *mask, Grid *g, Options *o, R *data)
+{
+  R radius, cx, cy, cz;
+  long l;
+  if (o->center_inlet) {
+    radius = 90 * g->dx[0];
+    cy = 0.5 * (g->x_lo[1] + g->x_hi[1]);
+    cx = 0.5 * (g->x_lo[1] + g->x_hi[1]);
+    cz = 0.5 * (g->x_lo[2] + g->x_hi[2]);
+  } else {
+    radius = 0.7 * (g->x_hi[0] - g->x_lo[0]);
+    cy = 0.5 * (g->x_lo[1] + g->x_hi[1]);
+    cx = (g->x_lo[0]) - radius * 0.7;
+    cz = 0.5 * (g->x_lo[2] + g->x_hi[2]);
+  }
+  orc_createSphere(data, cx, cy, cz, radius, -1, g);
+  for (l = 0; l < npts(g); l++) data[l] = (mask[l] > data[l]) ? mask[l] : data[l];
+}
+
+/* makeDiffSteps' percolation lens of two saturation steps a < b:
+ * diff = max(-a, b - dx/2); the voxels with diff < 0, plus the inlet and
+ * outlet ghost slabs, are labelled (26-neighbourhood); if the inlet and
+ * outlet voxels at mid j, k share a label, connected_only = diff with the
+ * sign flipped where diff <= 0 outside that component, and 1 is returned,
+ * else 0.  reference: src/applications/makeDiffSteps.c:70-140 */
+int orc_diff_steps(const R *a, const R *b, Grid *g, R *diff, R *connected_only)
+{
+  long n = npts(g), l;
+  int nx_gb = g->grid_dims_ghostbox[0], nxy_gb = nx_gb * g->grid_dims_ghostbox[1];
+  int i, j, k, inlet, outlet, main_val, perc;
+  QSS_DataArrays q;
+  memset(&q, 0, sizeof q);
+  q.phi_bin = (unsigned char *)malloc((size_t)n);
+  q.connectivity = (int *)malloc((size_t)n * sizeof(int));
+  for (l = 0; l < n; l++) {
+    R m = -a[l];
+    R p = b[l];
+    p -= 0.5 * g->dx[0];
+    diff[l] = (m > p) ? m : p;
+    q.phi_bin[l] = diff[l] < 0 ? 1 : 0;
+  }
+  for (k = g->klo_fb; k <= g->khi_fb; k++)
+    for (j = g->jlo_fb; j <= g->jhi_fb; j++) {
+      q.phi_bin[(g->ilo_fb - 1) + (long)j * nx_gb + (long)k * nxy_gb] = 1;
+      q.phi_bin[(g->ihi_fb + 1) + (long)j * nx_gb + (long)k * nxy_gb] = 1;
+    }
+  orc_findConnectivity3d(&q, g);
+  j = 0.5 * (g->jlo_fb + g->jhi_fb);
+  k = 0.5 * (g->klo_fb + g->khi_fb);
+  inlet = (g->ilo_fb - 1) + j * nx_gb + k * nxy_gb;
+  outlet = (g->ihi_fb + 1) + j * nx_gb + k * nxy_gb;
+  perc = q.connectivity[inlet] == q.connectivity[outlet];
+  if (perc) {
+    main_val = q.connectivity[inlet];
+    for (l = 0; l < n; l++) {
+      R v = diff[l];
+      if ((q.connectivity[l] != main_val) & (v <= 0)) v *= -1;
+      connected_only[l] = v;
+    }
+  }
+  free(q.phi_bin);
+  free(q.connectivity);
+  (void)i;
+  return perc;
+}
