@@ -22,6 +22,7 @@
 #include <stdexcept>
 #include <string>
 #include <vector>
+#include "lsm_adjoint.h"
 #include "lsm_batch.h"
 #include "lsm_coeff.h"
 #include "lsmpqs_hip.h"
@@ -197,6 +198,19 @@ __device__ __forceinline__ void scatter_axis(const real* phi, real* adj, int64_t
                                              real h, real inv_axis, real inv_x, real fp, real fm,
                                              real lam) {
   if (fp == real(0) && fm == real(0)) return;
+  if constexpr (FW && S == WENO5) {
+    // reverse mode through the two face reconstructions
+    real l[7], dl[7] = {0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int o = 0; o < 7; ++o) l[o] = phi[idx + (o - 3) * st];
+    const FaceScale<real> sc = face_scale(h);
+    weno5_face_adj(l, sc, fm, real(0), dl);      // phi^- from phi(-3..+2)
+    weno5_face_adj(l + 1, sc, real(0), fp, dl + 1);  // phi^+ from phi(-2..+3)
+#pragma unroll
+    for (int o = 0; o < 7; ++o)
+      if (dl[o] != real(0)) atomicAdd(&adj[idx + (o - 3) * st], lam * dl[o]);
+    return;
+  }
   Dual<7> l[7];
 #pragma unroll
   for (int o = 0; o < 7; ++o) {

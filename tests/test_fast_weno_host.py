@@ -22,3 +22,21 @@ def test_fast_weno_face_is_finite_and_accurate():
     nonfinite, worst = int(out[0]), float(out[1])
     assert nonfinite == 0
     assert worst < 2e-6
+
+
+@pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not installed")
+def test_weno_face_adjoint_matches_differences():
+    """weno5_face_adj, the reverse-mode derivative the FP32 adjoint uses for the
+    fast WENO5 faces, against central differences of weno5_face (double), and
+    its float instantiation against its double one; plus the flat-window
+    branch, whose derivative is the middle candidate's stencil."""
+    exe = os.path.join(tempfile.mkdtemp(prefix="lsm_wa_"), "weno_adj_check")
+    src = os.path.join(ROOT, "tests", "native", "weno_adj_check.cpp")
+    subprocess.run([HIPCC, "-O1", "-std=c++17", "-x", "hip", "--offload-arch=gfx950",
+                    "-I", os.path.join(ROOT, "lsmpqs-1.0_amd", "csrc"), "-I", os.path.join(ROOT, "include"),
+                    "-DQSSLIB_REAL=float", src, "-o", exe], check=True, capture_output=True)
+    out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split()
+    fd, f32, flat = (float(x) for x in out)
+    assert fd < 1e-6
+    assert f32 < 1e-4
+    assert flat < 1e-15
