@@ -1,6 +1,7 @@
-// lsm_adjoint.h -- reverse-mode derivative of the fast WENO5 face
-// reconstruction (weno5_face in lsm_device.h), for the adjoint's transposed
-// right-hand side.  Host-callable so it can be checked on the CPU.
+// lsm_adjoint.h -- reverse-mode derivatives of the WENO5 reconstructions of
+// lsm_device.h (the fast face form weno5_face and the reference-order
+// weno5), for the adjoint's transposed right-hand side.  Host-callable so
+// they can be checked on the CPU.
 #pragma once
 #include <math.h>
 #include "lsm_device.h"
@@ -93,6 +94,76 @@ __host__ __device__ inline void weno5_face_adj(const T* q, const FaceScale<T>& s
   dq[3] += dr[2] - dr[3];
   dq[4] += dr[3] - dr[4];
   dq[5] += dr[4];
+}
+
+// dv[0..4] += s * d weno5_combine(v)/dv (the reference-order HJ-WENO5
+// combination, lsm_device.h), with the largest v_k^2 in the epsilon taken as
+// the forward takes it.
+template <typename T>
+__host__ __device__ inline void weno5_combine_adj(const T* v, T s, T* dv) {
+  const T c13 = T(13.0 / 12.0), tiny = T(1.0e-35);
+  // eps = 1e-6 max v_k^2 + 1e-35; tmax(a, b) = a > b ? a : b keeps the later
+  // of equal operands
+  T mx = v[0] * v[0];
+  int jm = 0;
+  for (int k = 1; k < 5; ++k) {
+    const T sq = v[k] * v[k];
+    if (!(mx > sq)) { mx = sq; jm = k; }
+  }
+  const T eps = T(1e-6) * mx + tiny;
+  const T p1 = T(1.0 / 3.0) * v[0] - T(7.0 / 6.0) * v[1] + T(11.0 / 6.0) * v[2];
+  const T p2 = -T(1.0 / 6.0) * v[1] + T(5.0 / 6.0) * v[2] + T(1.0 / 3.0) * v[3];
+  const T p3 = T(1.0 / 3.0) * v[2] + T(5.0 / 6.0) * v[3] - T(1.0 / 6.0) * v[4];
+  const T t1 = v[0] - T(2) * v[1] + v[2], u1 = v[0] - T(4) * v[1] + T(3) * v[2];
+  const T t2 = v[1] - T(2) * v[2] + v[3], u2 = v[1] - v[3];
+  const T t3 = v[2] - T(2) * v[3] + v[4], u3 = T(3) * v[2] - T(4) * v[3] + v[4];
+  const T S1 = c13 * (t1 * t1) + T(0.25) * (u1 * u1);
+  const T S2 = c13 * (t2 * t2) + T(0.25) * (u2 * u2);
+  const T S3 = c13 * (t3 * t3) + T(0.25) * (u3 * u3);
+  const T e1 = S1 + eps, e2 = S2 + eps, e3 = S3 + eps;
+  const T a1 = T(0.1) / (e1 * e1), a2 = T(0.6) / (e2 * e2), a3 = T(0.3) / (e3 * e3);
+  const T A = a1 + a2 + a3;
+  const T out = (a1 * p1 + a2 * p2 + a3 * p3) / A;
+  // out = sum a_k p_k / A: d/dp_k = a_k / A, d/da_k = (p_k - out) / A
+  const T dp1 = s * a1 / A, dp2 = s * a2 / A, dp3 = s * a3 / A;
+  const T da1 = s * (p1 - out) / A, da2 = s * (p2 - out) / A, da3 = s * (p3 - out) / A;
+  // a_k = c_k / e_k^2: d/de_k = -2 a_k / e_k
+  const T de1 = -T(2) * da1 * a1 / e1, de2 = -T(2) * da2 * a2 / e2, de3 = -T(2) * da3 * a3 / e3;
+  const T deps = de1 + de2 + de3;
+  const T dt1 = T(2) * c13 * t1 * de1, du1 = T(0.5) * u1 * de1;
+  const T dt2 = T(2) * c13 * t2 * de2, du2 = T(0.5) * u2 * de2;
+  const T dt3 = T(2) * c13 * t3 * de3, du3 = T(0.5) * u3 * de3;
+  dv[0] += T(1.0 / 3.0) * dp1 + dt1 + du1;
+  dv[1] += -T(7.0 / 6.0) * dp1 - T(1.0 / 6.0) * dp2 - T(2) * dt1 - T(4) * du1 + dt2 + du2;
+  dv[2] += T(11.0 / 6.0) * dp1 + T(5.0 / 6.0) * dp2 + T(1.0 / 3.0) * dp3 + dt1 + T(3) * du1 -
+           T(2) * dt2 + dt3 + T(3) * du3;
+  dv[3] += T(1.0 / 3.0) * dp2 + T(5.0 / 6.0) * dp3 + dt2 - du2 - T(2) * dt3 - T(4) * du3;
+  dv[4] += -T(1.0 / 6.0) * dp3 + dt3 + du3;
+  dv[jm] += deps * T(1e-6) * T(2) * v[jm];
+}
+
+// dp[0..6] += sp * d plus/dp + sm * d minus/dp for weno5(p, inv_d) of
+// lsm_device.h (p = phi(-3..+3))
+template <typename T>
+__host__ __device__ inline void weno5_line_adj(const T* p, T inv_d, T sp, T sm, T* dp) {
+  const T D[6] = {p[1] - p[0], p[2] - p[1], p[3] - p[2], p[4] - p[3], p[5] - p[4], p[6] - p[5]};
+  // D1mm, D1m, D1, D1p, D1pp, D1ppp = D[0..5]
+  T dD[6] = {0, 0, 0, 0, 0, 0};
+  // plus = combine(D1ppp, D1pp, D1p, D1, D1m) * inv_d each
+  const T vp[5] = {D[5] * inv_d, D[4] * inv_d, D[3] * inv_d, D[2] * inv_d, D[1] * inv_d};
+  T dvp[5] = {0, 0, 0, 0, 0};
+  weno5_combine_adj(vp, sp, dvp);
+  dD[5] += dvp[0] * inv_d; dD[4] += dvp[1] * inv_d; dD[3] += dvp[2] * inv_d;
+  dD[2] += dvp[3] * inv_d; dD[1] += dvp[4] * inv_d;
+  // minus = combine(D1mm, D1m, D1, D1p, D1pp)
+  const T vm[5] = {D[0] * inv_d, D[1] * inv_d, D[2] * inv_d, D[3] * inv_d, D[4] * inv_d};
+  T dvm[5] = {0, 0, 0, 0, 0};
+  weno5_combine_adj(vm, sm, dvm);
+  for (int k = 0; k < 5; ++k) dD[k] += dvm[k] * inv_d;
+  for (int k = 0; k < 6; ++k) {
+    dp[k] -= dD[k];
+    dp[k + 1] += dD[k];
+  }
 }
 
 }  // namespace lsm

@@ -211,6 +211,17 @@ __device__ __forceinline__ void scatter_axis(const real* phi, real* adj, int64_t
       if (dl[o] != real(0)) atomicAdd(&adj[idx + (o - 3) * st], lam * dl[o]);
     return;
   }
+  if constexpr (!FW && S == WENO5) {
+    // reverse mode through the reference-order reconstruction
+    real l[7], dl[7] = {0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int o = 0; o < 7; ++o) l[o] = phi[idx + (o - 3) * st];
+    weno5_line_adj(l, inv_axis, fp, fm, dl);
+#pragma unroll
+    for (int o = 0; o < 7; ++o)
+      if (dl[o] != real(0)) atomicAdd(&adj[idx + (o - 3) * st], lam * dl[o]);
+    return;
+  }
   Dual<7> l[7];
 #pragma unroll
   for (int o = 0; o < 7; ++o) {

@@ -76,7 +76,7 @@ __device__ __forceinline__ void eno3(const T* p, T inv_plus, T inv_minus, T& plu
 
 // HJ-WENO5 combination with linear weights 0.1/0.6/0.3 (:1960-2035)
 template <typename T>
-__device__ __forceinline__ T weno5_combine(T v1, T v2, T v3, T v4, T v5) {
+__host__ __device__ __forceinline__ T weno5_combine(T v1, T v2, T v3, T v4, T v5) {
   const T one_third = T(1.0 / 3.0), seven_sixths = T(7.0 / 6.0);
   const T eleven_sixths = T(11.0 / 6.0), one_sixth = T(1.0 / 6.0);
   const T five_sixths = T(5.0 / 6.0), thirteen_twelfths = T(13.0 / 12.0);
@@ -102,7 +102,7 @@ __device__ __forceinline__ T weno5_combine(T v1, T v2, T v3, T v4, T v5) {
 }
 
 template <typename T>
-__device__ __forceinline__ void weno5(const T* p, T inv_d, T& plus, T& minus) {
+__host__ __device__ __forceinline__ void weno5(const T* p, T inv_d, T& plus, T& minus) {
   T D1 = p[3] - p[2], D1p = p[4] - p[3], D1m = p[2] - p[1];
   T D1pp = p[5] - p[4], D1ppp = p[6] - p[5], D1mm = p[1] - p[0];
   plus = weno5_combine(D1ppp * inv_d, D1pp * inv_d, D1p * inv_d, D1 * inv_d, D1m * inv_d);

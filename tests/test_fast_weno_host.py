@@ -29,14 +29,17 @@ def test_weno_face_adjoint_matches_differences():
     """weno5_face_adj, the reverse-mode derivative the FP32 adjoint uses for the
     fast WENO5 faces, against central differences of weno5_face (double), and
     its float instantiation against its double one; plus the flat-window
-    branch, whose derivative is the middle candidate's stencil."""
+    branch, whose derivative is the middle candidate's stencil; and
+    weno5_line_adj (the FP64 adjoint's reference-order WENO5) against central
+    differences of weno5."""
     exe = os.path.join(tempfile.mkdtemp(prefix="lsm_wa_"), "weno_adj_check")
     src = os.path.join(ROOT, "tests", "native", "weno_adj_check.cpp")
     subprocess.run([HIPCC, "-O1", "-std=c++17", "-x", "hip", "--offload-arch=gfx950",
                     "-I", os.path.join(ROOT, "lsmpqs-1.0_amd", "csrc"), "-I", os.path.join(ROOT, "include"),
                     "-DQSSLIB_REAL=float", src, "-o", exe], check=True, capture_output=True)
     out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split()
-    fd, f32, flat = (float(x) for x in out)
+    fd, f32, flat, line = (float(x) for x in out)
     assert fd < 1e-6
     assert f32 < 1e-4
     assert flat < 1e-15
+    assert line < 1e-6   # weno5_line_adj, the FP64 adjoint's reference-order WENO5
