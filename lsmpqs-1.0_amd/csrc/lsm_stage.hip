@@ -187,9 +187,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile) {
     for (int o = 0; o < 6; ++o) w[o] = col[(int64_t)(k0 - 3 + o) * plane];
     wnext = col[(int64_t)(k0 + 3) * plane];
   }
-  // halo voxel of plane k+1 in flight
-  const real* hp = hs.off >= 0 ? in + hs.off : nullptr;
-  auto halo = [&](int k) { return hp ? hp[(int64_t)k * plane] : real(0); };
+  // halo voxel of plane k+1 in flight; a halo voxel outside the ghost box
+  // (partial tiles) is read by no active voxel's stencil, so its thread
+  // reads voxel 0's column instead
+  const real* hp = in + (hs.off >= 0 ? hs.off : 0);
+  auto halo = [&](int k) { return hs.off >= 0 ? hp[(int64_t)k * plane] : real(0); };
   if (hs.lds >= 0) {
     pl[0][hs.lds] = halo(k0 - 1);
     pl[1][hs.lds] = halo(k0);
@@ -202,20 +204,44 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile) {
   if constexpr (FW) zcarry = weno5_face(&w[0], sz).minus_right;
   __syncthreads();
 
+  // Plane k+1 into the LDS ring (read after this plane's barrier), the z
+  // line's plane k+3, and the next loads in flight: plane k+4 of the z line
+  // and the halo voxel of plane k+2.  Stage 1 does this at the start of the
+  // plane with its loads guarded.  Stages 2 and 3, which also load u^n (and
+  // the floor), do it after the x and y faces with every load unguarded
+  // (inactive lanes of partial tiles read clamped, in-box addresses and
+  // discard the values; plane k+4 is clamped to the box on the last planes,
+  // where it is not used): the compiler then neither folds the u^n product
+  // into the load block, which made each plane wait for its loads at once,
+  // nor drains the whole load queue, the previous plane's store included, at
+  // the top of each plane (512^3 FP32: stages 2 and 3 1.45 / 1.54 ms ->
+  // 1.13 / 1.28 ms; stage 1 is slower in that form).
+  constexpr bool EARLY_RING = STAGE == 1;
+  auto ring_next = [&](int k, int sp) {
+    w[6] = wnext;
+    wnext = col[(int64_t)min(k + 4, bx.nz - 1) * plane];
+    pl[sp][me] = w[4];
+    if (hs.lds >= 0) pl[sp][hs.lds] = hnext;
+    hnext = hp[(int64_t)(k + 2) * plane];  // with hs.off < 0 only inactive voxels read it
+  };
   int64_t idx = bx.at(i, j, k0);
-  for (int k = k0; k <= k1; ++k, idx += plane) {
+  int64_t pidx = bx.at(min(i, bx.ihi), min(j, bx.jhi), k0);
+  for (int k = k0; k <= k1; ++k, idx += plane, pidx += plane) {
     const int sm = (k - k0) & 3, sc = (k - k0 + 1) & 3, sp = (k - k0 + 2) & 3;
     const int buf = (k - k0) & 1;
-    // ---- phase A: stage plane k+1, start this voxel's loads, x/y/z faces of plane k
-    w[6] = wnext;
-    if (inbox && k + 4 < bx.nz) wnext = col[(int64_t)(k + 4) * plane];
-    pl[sp][me] = w[4];
-    if (hs.lds >= 0) {
-      pl[sp][hs.lds] = hnext;
-      hnext = halo(k + 2);
+    // ---- phase A: start this voxel's loads, x/y faces of plane k, stage
+    // plane k+1, z face of plane k
+    if constexpr (EARLY_RING) {
+      w[6] = wnext;
+      if (inbox && k + 4 < bx.nz) wnext = col[(int64_t)(k + 4) * plane];
+      pl[sp][me] = w[4];
+      if (hs.lds >= 0) {
+        pl[sp][hs.lds] = hnext;
+        hnext = halo(k + 2);
+      }
     }
     PointIn<real> cur;  // in flight during the face computations
-    if (active) cur.template load<VAR, STAGE, FINAL>(a, idx);
+    if (!EARLY_RING || active) cur.template load<VAR, STAGE, FINAL>(a, EARLY_RING ? idx : pidx);
     real d[6];
     real qx[6], qy[6];
     if constexpr (FW) {
@@ -242,7 +268,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile) {
         real* dst = yrow ? &yp[buf][TY][tid] : &xp[buf][tid - TX][TX];
         *dst = f.plus_left;
       }
-      f = weno5_face(&w[1], sz);
+    }
+    if constexpr (!EARLY_RING) ring_next(k, sp);
+    if constexpr (FW) {
+      Weno5Face<real> f = weno5_face(&w[1], sz);
       d[4] = f.plus_left;
       d[5] = zcarry;
       zcarry = f.minus_right;
