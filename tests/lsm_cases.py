@@ -18,6 +18,21 @@ _libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
 DEVNULL = C.c_void_p(_libc.fopen(b"/dev/null", b"w"))
 
 
+def void_api(lib):
+    """Declare the record entry points the application tests call with
+    untyped pointers.  The library object is shared by every test in the
+    process, and other tests declare typed signatures on it, so each caller
+    sets the form it uses."""
+    for name in ("readGridFromBinaryFile", "createOptionsDefault"):
+        getattr(lib, name).restype = C.c_void_p
+    lib.createOptionsFromInputFile.restype = C.c_void_p
+    lib.createOptionsFromInputFile.argtypes = [C.c_char_p]
+    lib.readGridFromBinaryFile.argtypes = [C.c_char_p]
+    lib.destroyGrid.argtypes = [C.c_void_p]
+    lib.destroyOptions.argtypes = [C.c_void_p]
+    return lib
+
+
 def case(n, dtype=np.float64, radius=4.0, porosity=0.4, seed=1001):
     """(grid, mask, phi0) of a seeded n^3 pack (cached)."""
     dims = (n, n, n) if np.isscalar(n) else tuple(n)

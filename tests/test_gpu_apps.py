@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 import oracle_py as op
+from lsm_cases import void_api
 from lsmpqs_amd import hiplib
 from lsmpqs_amd.abi import types_for
 from test_oracle_apps import segmented, step_pair
@@ -18,10 +19,9 @@ pytestmark = pytest.mark.gpu
 
 
 def lib_for(dtype):
-    lib = hiplib.load(dtype)
+    lib = void_api(hiplib.load(dtype))
     lib.lsmpqs_mask_from_image.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p,
                                            C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]
-    lib.createOptionsDefault.restype = C.c_void_p
     return lib
 
 

@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 import oracle_py as op
+from lsm_cases import void_api
 from lsmpqs_amd import hiplib
 
 APP = os.path.join(os.path.dirname(op.__file__), "_ref", "app_{}", "{}")
@@ -39,8 +40,7 @@ def read_array(path, dtype):
 
 
 def grid_from(lib, path):
-    lib.readGridFromBinaryFile.restype = C.c_void_p
-    return C.c_void_p(lib.readGridFromBinaryFile(path.encode()))
+    return C.c_void_p(void_api(lib).readGridFromBinaryFile(path.encode()))
 
 
 def run(cmd, cwd):
@@ -66,8 +66,7 @@ def test_mask_from_image_matches_createMaskUBC(dtype, tmp_path):
     lib = hiplib.load(dtype)
     g = grid_from(lib, str(tmp_path / "grid.gz"))
     dims, want = read_array(str(tmp_path / "mask.gz"), dtype)
-    lib.createOptionsDefault.restype = C.c_void_p
-    o = C.c_void_p(lib.createOptionsDefault())
+    o = C.c_void_p(void_api(lib).createOptionsDefault())
     got = np.zeros_like(want)
     op.oracle_lib(dtype).orc_mask_from_image(img.ctypes.data_as(C.c_void_p), nx, ny, nz, g, o,
                                               got.ctypes.data_as(C.c_void_p))
@@ -92,8 +91,7 @@ def test_init_from_mask_matches_createInitFromMask(dtype, center, tmp_path):
     g = grid_from(lib, str(tmp_path / "grid.gz"))
     _, mask = read_array(str(tmp_path / "mask.gz"), dtype)
     _, want = read_array(str(tmp_path / "data_init.gz"), dtype)
-    lib.createOptionsFromInputFile.restype = C.c_void_p
-    o = C.c_void_p(lib.createOptionsFromInputFile(str(tmp_path / "opts").encode()))
+    o = C.c_void_p(void_api(lib).createOptionsFromInputFile(str(tmp_path / "opts").encode()))
     got = np.zeros_like(want)
     op.oracle_lib(dtype).orc_init_from_mask(mask.ctypes.data_as(C.c_void_p), g, o,
                                              got.ctypes.data_as(C.c_void_p))
