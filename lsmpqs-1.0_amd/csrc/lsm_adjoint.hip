@@ -184,6 +184,7 @@ __device__ __forceinline__ void line_derivs(const T* l, real h, real inv_axis, r
     sc.inv = T(f.inv);
     sc.thr = T(f.thr);
     sc.c1 = T(f.c1);
+    sc.c2 = T(f.c2);
     minus = weno5_face(l, sc).minus_right;
     plus = weno5_face(l + 1, sc).plus_left;
   } else {

@@ -135,6 +135,7 @@ struct FaceScale {
   T inv;  // 1/dx
   T thr;  // 1e-18 dx: flat-window bound on the largest raw difference
   T c1;   // sqrt(1e-35) dx: eps / (m/dx)^2 = 1e-6 + (c1/m)^2
+  T c2;   // 2 c1
 };
 template <typename T>
 __host__ __device__ inline FaceScale<T> face_scale(T dx) {
@@ -142,6 +143,7 @@ __host__ __device__ inline FaceScale<T> face_scale(T dx) {
   s.inv = T(1) / dx;
   s.thr = T(1e-18) * dx;
   s.c1 = T(3.16227766016837933e-18) * dx;
+  s.c2 = T(6.32455532033675866e-18) * dx;
   return s;
 }
 
@@ -164,48 +166,60 @@ __host__ __device__ __forceinline__ double fast_rcp(double x) { return 1.0 / x; 
 // with the third differences g1 = r0 - 3r1 + 3r2 - r3, g2 = r1 - 3r2 + 3r3 - r4
 // (Jiang and Peng, SIAM J. Sci. Comput. 21 (2000), eq. 2.10); the reversed
 // window has g1' = -g2 and g2' = -g1.
+//
+// weno5_face_raw returns both values times dx (the caller applies 1/dx,
+// folded into another product).  All weight factors are carried at a
+// common scale, which the normalised weights do not see: the smoothness
+// indicators as 4 S_k = 13/3 t_k^2 + u_k^2 with eps times 4, so q_k is 16
+// times the q_k above (then in [1.6e-11, 3.2e4]), and the weight sums and
+// numerators times 10.  With A = q23 g1 and B = q12 g2 the numerators are
+// A/3 + B/2 = (A + B)/3 + B/6 and -(A/2 + B/3) = -((A + B)/3 + A/6), the sums
+// q23 + 3 q12 + 6 q13 and q12 + 3 q23 + 6 q13.  The middle candidates are
+// r2 + (D2 + 2 D3)/6 and r2 - (2 D2 + D3)/6 with the second differences
+// D_k = r_k - r_{k-1}.
 template <typename T>
-__host__ __device__ __forceinline__ Weno5Face<T> weno5_face(const T* q, const FaceScale<T>& sc) {
+__host__ __device__ __forceinline__ Weno5Face<T> weno5_face_raw(const T* q, const FaceScale<T>& sc) {
 #pragma clang fp contract(fast)
   T r0 = q[1] - q[0], r1 = q[2] - q[1], r2 = q[3] - q[2], r3 = q[4] - q[3], r4 = q[5] - q[4];
-  Weno5Face<T> f;
-  // middle candidates of the two windows
-  T pm2 = -(T(1) / T(6)) * r1 + (T(5) / T(6)) * r2 + (T(1) / T(3)) * r3;
-  T pp2 = -(T(1) / T(6)) * r3 + (T(5) / T(6)) * r2 + (T(1) / T(3)) * r1;
-  T m = fmax(fmax(fmax(fabs(r0), fabs(r1)), fmax(fabs(r2), fabs(r3))), fabs(r4));
-  if (!(m > sc.thr)) {
-    f.minus_right = pm2 * sc.inv;
-    f.plus_left = pp2 * sc.inv;
-    return f;
-  }
-  // smoothness indicators of the raw differences (S_k = 13/12 t_k^2 + 1/4 u_k^2
-  // with t, u the reference's combinations, here from second differences);
-  // scaled by 1/m^2 inside q_k = (S_k/m^2 + eps')^2
   T D1 = r1 - r0, D2 = r2 - r1, D3 = r3 - r2, D4 = r4 - r3;
-  T t1 = D2 - D1, t2 = D3 - D2, t3 = D4 - D3;       // e1-2e2+e3, e2-2e3+e4, e3-2e4+e5
-  T u1 = T(3) * D2 - D1, u2 = D2 + D3, u3 = T(3) * D3 - D4;  // |e1-4e2+3e3|, |e2-e4|, |3e3-4e4+e5|
-  T S1 = (T(13) / T(12)) * (t1 * t1) + T(0.25) * (u1 * u1);
-  T S2 = (T(13) / T(12)) * (t2 * t2) + T(0.25) * (u2 * u2);
-  T S3 = (T(13) / T(12)) * (t3 * t3) + T(0.25) * (u3 * u3);
+  T u2 = D2 + D3;
+  Weno5Face<T> f;
+  f.minus_right = (T(1) / T(6)) * (u2 + D3) + r2;
+  f.plus_left = (T(-1) / T(6)) * (u2 + D2) + r2;
+  T m = fmax(fmax(fmax(fabs(r0), fabs(r1)), fabs(r2)), fmax(fabs(r3), fabs(r4)));
+  if (!(m > sc.thr)) return f;
+  T t1 = D2 - D1, t2 = D3 - D2, t3 = D4 - D3;         // e1-2e2+e3, e2-2e3+e4, e3-2e4+e5
+  T u1 = T(3) * D2 - D1, u3 = T(3) * D3 - D4;         // e1-4e2+3e3, 3e3-4e4+e5
+  T S1 = (T(13) / T(3) * t1) * t1 + u1 * u1;
+  T S2 = (T(13) / T(3) * t2) * t2 + u2 * u2;
+  T S3 = (T(13) / T(3) * t3) * t3 + u3 * u3;
   T im = fast_rcp(m);
   T im2 = im * im;
-  T ce = sc.c1 * im;
-  T eps = T(1e-6) + ce * ce;
+  T ce = sc.c2 * im;
+  T eps = ce * ce + T(4e-6);
   T q1 = S1 * im2 + eps; q1 = q1 * q1;
   T q2 = S2 * im2 + eps; q2 = q2 * q2;
   T q3 = S3 * im2 + eps; q3 = q3 * q3;
   T q12 = q1 * q2, q13 = q1 * q3, q23 = q2 * q3;
-  T g1 = t1 - t2;  // r0 - 3r1 + 3r2 - r3
-  T g2 = t2 - t3;  // r1 - 3r2 + 3r3 - r4
-  T u = q23 * g1, v = q12 * g2;
-  T mid = T(0.6) * q13;
-  // phi^-: weights 0.1 q23, 0.6 q13, 0.3 q12; phi^+: 0.1 q12, 0.6 q13, 0.3 q23
-  T sum_m = T(0.1) * q23 + T(0.3) * q12 + mid;
-  T sum_p = T(0.1) * q12 + T(0.3) * q23 + mid;
-  T num_m = (T(1) / T(30)) * u + (T(1) / T(20)) * v;
-  T num_p = -((T(1) / T(20)) * u + (T(1) / T(30)) * v);
-  f.minus_right = (pm2 + num_m * fast_rcp(sum_m)) * sc.inv;
-  f.plus_left = (pp2 + num_p * fast_rcp(sum_p)) * sc.inv;
+  T A = q23 * (t1 - t2);  // g1 = r0 - 3r1 + 3r2 - r3
+  T B = q12 * (t2 - t3);  // g2 = r1 - 3r2 + 3r3 - r4
+  T base = T(6) * q13 + (q12 + q23);
+  T sum_m = T(2) * q12 + base;
+  T sum_p = T(2) * q23 + base;
+  T s3 = (A + B) * (T(1) / T(3));
+  T num_m = (T(1) / T(6)) * B + s3;
+  T num_p = (T(1) / T(6)) * A + s3;
+  f.minus_right = num_m * fast_rcp(sum_m) + f.minus_right;
+  f.plus_left = -(num_p * fast_rcp(sum_p)) + f.plus_left;
+  return f;
+}
+
+// weno5_face_raw scaled by 1/dx
+template <typename T>
+__host__ __device__ __forceinline__ Weno5Face<T> weno5_face(const T* q, const FaceScale<T>& sc) {
+  Weno5Face<T> f = weno5_face_raw(q, sc);
+  f.minus_right = f.minus_right * sc.inv;
+  f.plus_left = f.plus_left * sc.inv;
   return f;
 }
 
@@ -229,6 +243,30 @@ __device__ __forceinline__ T curvature_fast(T c, T xm2, T xm1, T xp1, T xp2, T y
   return curv * fast_rcp(g2);
 }
 
+// curvature_fast on a grid with dx = dy = dz (f = fx = fy = fz): the
+// expression is homogeneous, so the differences are taken unscaled, the
+// common factor f^2 is applied once, and the |grad phi|^2 < 1e-5 test reads
+// g2 < thr with thr = 1e-5 / f^2 on the unscaled sum
+template <typename T>
+__device__ __forceinline__ T curvature_iso(T c, T xm2, T xm1, T xp1, T xp2, T ym2, T ym1, T yp1,
+                                           T yp2, T zm2, T zm1, T zp1, T zp2, const T* xy,
+                                           const T* xz, const T* yz, T f2, T thr) {
+#pragma clang fp contract(fast)
+  T px = xp1 - xm1, py = yp1 - ym1, pz = zp1 - zm1;
+  T pxx = (xp2 + xm2) - T(2.0) * c;
+  T pzz = (zp2 + zm2) - T(2.0) * c;
+  T pyy = (yp2 + ym2) - T(2.0) * c;
+  T pxy = (xy[0] + xy[1]) - (xy[2] + xy[3]);
+  T pxz = (xz[0] + xz[1]) - (xz[2] + xz[3]);
+  T pyz = (yz[0] + yz[1]) - (yz[2] + yz[3]);
+  T sx = px * px, sy = py * py, sz = pz * pz;
+  T g2 = sx + sy + sz;
+  if (g2 < thr) return T(0.0);
+  T curv = pxx * (sy + sz) + pyy * (sx + sz) + pzz * (sx + sy)
+             - T(2.0) * (pxy * px * py + pxz * px * pz + pyz * py * pz);
+  return (f2 * curv) * fast_rcp(g2);
+}
+
 // Upwind derivative pair along one axis with the scheme's scalings.
 template <typename T, int S>
 __device__ __forceinline__ void axis_derivs(const T* p, T inv_axis, T inv_x, T& plus, T& minus) {
@@ -249,6 +287,39 @@ __device__ __forceinline__ T godunov_sq(T vn, const T* d) {
   T y = fmax(fmax(s * d[3], -(s * d[2])), T(0));
   T z = fmax(fmax(s * d[5], -(s * d[4])), T(0));
   return x * x + y * y + z * z;
+}
+
+// godunov_sq of one-sided derivatives already multiplied by s = sign(vel_n):
+// per axis max(d-, -d+, 0)^2
+template <typename T>
+__device__ __forceinline__ T godunov_sq_signed(const T* d) {
+#pragma clang fp contract(fast)
+  T x = fmax(fmax(d[1], -d[0]), T(0));
+  T y = fmax(fmax(d[3], -d[2]), T(0));
+  T z = fmax(fmax(d[5], -d[4]), T(0));
+  return x * x + y * y + z * z;
+}
+
+// Right-hand side of the variable-coefficient path (fast T path):
+// temp_norm + b kappa|grad phi| + adv_sign (V . upwind grad phi), with
+// adv_sign = -1, or -sign(vel_n) when the derivatives carry that sign
+template <typename T>
+__device__ __forceinline__ T var_rhs_fast(T temp_norm, T b, T curv, T vx, T vy, T vz, T ax, T ay,
+                                          T az, T adv_sign) {
+#pragma clang fp contract(fast)
+  return (b * curv + temp_norm) + adv_sign * (vx * ax + vy * ay + vz * az);
+}
+
+// TVD-RK stage value from the stage input c, u^n and the right-hand side
+// (fast T path): RK1 c + dt L; "RK2 stage 2" and RK3 stage 2
+// 3/4 u^n + 1/4 (c + dt L); RK3 stage 3 1/3 u^n + 2/3 (c + dt L)
+template <int STAGE, typename T>
+__device__ __forceinline__ T rk_value_fast(T c, T un, T dt, T rhs) {
+#pragma clang fp contract(fast)
+  T v = dt * rhs + c;
+  if (STAGE == 1) return v;
+  if (STAGE == 2) return T(0.75) * un + T(0.25) * v;
+  return T(1.0 / 3.0) * un + T(2.0 / 3.0) * v;
 }
 
 // H/dx of the CFL bound (:488-513); d = {x+, x-, y+, y-, z+, z-}

@@ -173,6 +173,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile) {
 
   const real inv_dx = real(1.0) / a.dx, inv_dy = real(1.0) / a.dy, inv_dz = real(1.0) / a.dz;
   const real fx = real(0.5) / a.dx, fy = real(0.5) / a.dy, fz = real(0.5) / a.dz;
+  // cubic voxels: the fast curvature factors out f^2 = (0.5/dx)^2
+  const bool iso = a.dx == a.dy && a.dy == a.dz;
+  const real f2 = fx * fx, curv_thr = real(1.e-5) / f2;
   const FaceScale<real> sx = face_scale(a.dx), sy = face_scale(a.dy), sz = face_scale(a.dz);
   const real dt = *a.dt;
 
@@ -201,7 +204,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile) {
   real hnext = hs.lds >= 0 ? halo(k0 + 1) : real(0);
 
   real zcarry = real(0);  // phi_z^-(k) from the face k-1/2 (FW only)
-  if constexpr (FW) zcarry = weno5_face(&w[0], sz).minus_right;
+  if constexpr (FW) zcarry = weno5_face_raw(&w[0], sz).minus_right;
   __syncthreads();
 
   // Plane k+1 into the LDS ring (read after this plane's barrier), the z
@@ -251,10 +254,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile) {
         qx[o] = P[(ty + 3) * PX + tx + o];   // phi(i-3+o, j)
         qy[o] = P[(ty + o) * PX + tx + 3];   // phi(i, j-3+o)
       }
-      Weno5Face<real> f = weno5_face(qx, sx);
+      // faces times dx; phase B applies sign(vel_n)/dx
+      Weno5Face<real> f = weno5_face_raw(qx, sx);
       d[1] = f.minus_right;
       xp[buf][ty][tx] = f.plus_left;
-      f = weno5_face(qy, sy);
+      f = weno5_face_raw(qy, sy);
       d[3] = f.minus_right;
       yp[buf][ty][tx] = f.plus_left;
       if (tid < TX + TY) {  // far y row and far x column of the tile
@@ -264,14 +268,14 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile) {
         real q[6];
 #pragma unroll
         for (int o = 0; o < 6; ++o) q[o] = q0[o * st];
-        f = weno5_face(q, yrow ? sy : sx);
+        f = weno5_face_raw(q, yrow ? sy : sx);
         real* dst = yrow ? &yp[buf][TY][tid] : &xp[buf][tid - TX][TX];
         *dst = f.plus_left;
       }
     }
     if constexpr (!EARLY_RING) ring_next(k, sp);
     if constexpr (FW) {
-      Weno5Face<real> f = weno5_face(&w[1], sz);
+      Weno5Face<real> f = weno5_face_raw(&w[1], sz);
       d[4] = f.plus_left;
       d[5] = zcarry;
       zcarry = f.minus_right;
@@ -301,9 +305,20 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile) {
         axis_derivs<real, S>(w, inv_dz, inv_dx, d[4], d[5]);
       }
 
+      real adv_sign = real(-1);
+      if constexpr (FW) {
+        // one-sided derivatives times sign(vel_n), so that the Godunov
+        // selection needs no per-axis sign products
+        const real sg = vn > real(0) ? real(1) : real(-1);
+        const real gx = sg * inv_dx, gy = sg * inv_dy, gz = sg * inv_dz;
+        d[0] *= gx; d[1] *= gx;
+        d[2] *= gy; d[3] *= gy;
+        d[4] *= gz; d[5] *= gz;
+        adv_sign = -sg;
+      }
       real temp_norm;
       if (fabs(vn) >= real(1.e-5)) {
-        real g2 = godunov_sq(vn, d);
+        real g2 = FW ? godunov_sq_signed(d) : godunov_sq(vn, d);
         real gn;
         if constexpr (FAST) gn = __builtin_amdgcn_sqrtf(g2);
         else gn = sqrt(g2);
@@ -317,10 +332,14 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile) {
       real xz[4] = {Pp[me + 1], Pm[me - 1], Pp[me - 1], Pm[me + 1]};
       real yz[4] = {Pp[me + PX], Pm[me - PX], Pp[me - PX], Pm[me + PX]};
       real curv;
-      if constexpr (FAST)
-        curv = curvature_fast(lx[3], lx[1], lx[2], lx[4], lx[5], ly[1], ly[2], ly[4], ly[5], w[1],
-                              w[2], w[4], w[5], xy, xz, yz, fx, fy, fz);
-      else
+      if constexpr (FAST) {
+        if (iso)
+          curv = curvature_iso(lx[3], lx[1], lx[2], lx[4], lx[5], ly[1], ly[2], ly[4], ly[5], w[1],
+                               w[2], w[4], w[5], xy, xz, yz, f2, curv_thr);
+        else
+          curv = curvature_fast(lx[3], lx[1], lx[2], lx[4], lx[5], ly[1], ly[2], ly[4], ly[5],
+                                w[1], w[2], w[4], w[5], xy, xz, yz, fx, fy, fz);
+      } else
         curv = curvature(lx[3], lx[1], lx[2], lx[4], lx[5], ly[1], ly[2], ly[4], ly[5], w[1],
                          w[2], w[4], w[5], xy, xz, yz, fx, fy, fz);
       real rhs;
@@ -328,13 +347,19 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile) {
         real ax = vxv >= real(0) ? d[1] : d[0];
         real ay = vyv >= real(0) ? d[3] : d[2];
         real az = vzv >= real(0) ? d[5] : d[4];
-        real temp_vel = -(vxv * ax + vyv * ay + vzv * az);
-        rhs = temp_norm + bv * curv + temp_vel;
+        if constexpr (FAST) {
+          rhs = var_rhs_fast(temp_norm, bv, curv, vxv, vyv, vzv, ax, ay, az, adv_sign);
+        } else {
+          real temp_vel = -(vxv * ax + vyv * ay + vzv * az);
+          rhs = temp_norm + bv * curv + temp_vel;
+        }
       } else {
         rhs = temp_norm + bv * curv;
       }
       real val;
-      if (STAGE == 1) {
+      if constexpr (FAST) {
+        val = rk_value_fast<STAGE>(lx[3], unv, dt, rhs);
+      } else if (STAGE == 1) {
         val = lx[3] + dt * rhs;
       } else if (STAGE == 2) {
         val = real(0.75) * unv + real(0.25) * (lx[3] + dt * rhs);
