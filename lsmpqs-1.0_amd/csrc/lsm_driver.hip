@@ -58,11 +58,21 @@ static int rhs_scheme(const Options* o, bool var_path) {
 // result; A, B and S serve as next-iterate, phi0 copy and distance buffers.
 static void reinit_dev(Session* s, real*& f, const Grid* g, const Options* o) {
   const Box& bx = s->bx;
+  real t_r = 0, dt_r = real(0.5) * g->dx[0];
+  if (!(t_r < o->tmax_r)) return;
+  if (!(dt_r < o->tmax_r)) {
+    // one iteration (tmax_r = 0.4 at dx = 1): data = phi0, fused first step
+    DCHECK(launch_reinit_first_step(bx, s->A, f, dt_r, g->dx[0], g->dx[1], g->dx[2], s->st));
+    BcArgs ba{};
+    ba.bx = bx; ba.src = s->A; ba.dst = s->A;
+    DCHECK(launch_bc(ba, false, s->st));
+    std::swap(f, s->A);
+    return;
+  }
   real* phi0 = s->B;
   real* dist0 = s->S;
   DCHECK(hipMemcpyAsync(phi0, f, bytes(s), hipMemcpyDeviceToDevice, s->st));
   DCHECK(launch_reinit_distance(bx, dist0, phi0, g->dx[0], g->dx[1], g->dx[2], s->st));
-  real t_r = 0, dt_r = real(0.5) * g->dx[0];
   while (t_r < o->tmax_r) {
     DCHECK(launch_reinit_step(bx, s->A, f, phi0, dist0, dt_r, g->dx[0], g->dx[1], g->dx[2], s->st));
     BcArgs ba{};

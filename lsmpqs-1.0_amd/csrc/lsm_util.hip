@@ -228,6 +228,39 @@ __device__ __forceinline__ bool sign_change(real a, real c) {
   return (a > real(0) && c <= real(0)) || (a <= real(0) && c > real(0));
 }
 
+// D0 of one voxel from phi0 and its six neighbours (:518-570): phi0 / delta
+// next to the interface (a sign change to a face neighbour), else the flag
+__device__ __forceinline__ real reinit_d0(real c, real xp, real xm, real yp, real ym, real zp,
+                                          real zm, real flag) {
+  bool near = sign_change(xp, c) || sign_change(xm, c) || sign_change(yp, c) ||
+              sign_change(ym, c) || sign_change(zp, c) || sign_change(zm, c);
+  if (!near) return flag;
+  real delta = tmax(tmax(tmax(fabs(xp - xm) / real(2), fabs(xp - c)), fabs(c - xm)), real(1.e-5));
+  delta = tmax(tmax(tmax(fabs(yp - ym) / real(2), fabs(yp - c)), fabs(c - ym)), delta);
+  delta = tmax(tmax(tmax(fabs(zp - zm) / real(2), fabs(zp - c)), fabs(c - zm)), delta);
+  return c / delta;
+}
+
+// First-order subcell-fix right-hand side (:665-737): sgn(phi0)(1 - |grad phi|_G)
+// with one-sided differences gp (forward), gm (backward), or, next to the
+// interface (d0 != flag), d0 - sgn(phi0) |phi| / max dx
+__device__ __forceinline__ real reinit_rhs(real c, const real* gp, const real* gm, real p0,
+                                           real d0, real flag, real max_dx) {
+  real sgn0 = fabs(p0) > real(1.e-5) ? copysign(real(1), p0) : real(0);
+  if (d0 != flag) return d0 - sgn0 * fabs(c) / max_dx;
+  real st[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    real a, b;
+    if (c > real(0)) { a = tmax(-gp[d], real(0)); b = tmax(gm[d], real(0)); }
+    else { a = tmax(gp[d], real(0)); b = tmax(-gm[d], real(0)); }
+    st[d] = tmax(a, b);
+  }
+  if (!(fabs(c) > real(1.e-5))) return real(0);
+  real n2 = st[0] * st[0] + st[1] * st[1] + st[2] * st[2];
+  return sgn0 - sgn0 * sqrt(n2);
+}
+
 __global__ void reinit_distance_kernel(Box bx, real* dist0, const real* phi0, real dx, real dy,
                                        real dz) {
   FillIter it(bx);
@@ -238,20 +271,8 @@ __global__ void reinit_distance_kernel(Box bx, real* dist0, const real* phi0, re
     it.ijk(f, i, j, k);
     int64_t idx = bx.at(i, j, k);
     int64_t sy = bx.nx, sz = (int64_t)bx.nx * bx.ny;
-    real c = phi0[idx];
-    real xp = phi0[idx + 1], xm = phi0[idx - 1];
-    real yp = phi0[idx + sy], ym = phi0[idx - sy];
-    real zp = phi0[idx + sz], zm = phi0[idx - sz];
-    bool near = sign_change(xp, c) || sign_change(xm, c) || sign_change(yp, c) ||
-                sign_change(ym, c) || sign_change(zp, c) || sign_change(zm, c);
-    if (!near) {
-      dist0[idx] = flag;
-    } else {
-      real delta = tmax(tmax(tmax(fabs(xp - xm) / real(2), fabs(xp - c)), fabs(c - xm)), real(1.e-5));
-      delta = tmax(tmax(tmax(fabs(yp - ym) / real(2), fabs(yp - c)), fabs(c - ym)), delta);
-      delta = tmax(tmax(tmax(fabs(zp - zm) / real(2), fabs(zp - c)), fabs(c - zm)), delta);
-      dist0[idx] = c / delta;
-    }
+    dist0[idx] = reinit_d0(phi0[idx], phi0[idx + 1], phi0[idx - 1], phi0[idx + sy],
+                           phi0[idx - sy], phi0[idx + sz], phi0[idx - sz], flag);
   }
 }
 hipError_t launch_reinit_distance(const Box& bx, real* dist0, const real* phi0, real dx, real dy,
@@ -284,29 +305,7 @@ __global__ void reinit_step_kernel(Box bx, real* next, const real* data, const r
     gm[1] = (c - data[idx - sy]) * inv_dy;
     gp[2] = (data[idx + sz] - c) * inv_dz;
     gm[2] = (c - data[idx - sz]) * inv_dz;
-    real p0 = phi0[idx];
-    real sgn0 = fabs(p0) > real(1.e-5) ? copysign(real(1), p0) : real(0);
-    real d0 = dist0[idx];
-    real rhs;
-    if (d0 == flag) {
-      real st[3];
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        real a, b;
-        if (c > real(0)) { a = tmax(-gp[d], real(0)); b = tmax(gm[d], real(0)); }
-        else { a = tmax(gp[d], real(0)); b = tmax(-gm[d], real(0)); }
-        st[d] = tmax(a, b);
-      }
-      if (fabs(c) > real(1.e-5)) {
-        real n2 = st[0] * st[0] + st[1] * st[1] + st[2] * st[2];
-        rhs = sgn0 - sgn0 * sqrt(n2);
-      } else {
-        rhs = real(0);
-      }
-    } else {
-      rhs = d0 - sgn0 * fabs(c) / max_dx;
-    }
-    next[idx] = c + dt_r * rhs;
+    next[idx] = c + dt_r * reinit_rhs(c, gp, gm, phi0[idx], dist0[idx], flag, max_dx);
   }
 }
 hipError_t launch_reinit_step(const Box& bx, real* next, const real* data, const real* phi0,
@@ -315,6 +314,39 @@ hipError_t launch_reinit_step(const Box& bx, real* next, const real* data, const
   FillIter it(bx);
   hipLaunchKernelGGL(reinit_step_kernel, dim3(stream_blocks(it.n)), dim3(256), 0, st, bx, next,
                      data, phi0, dist0, dt_r, dx, dy, dz);
+  return hipGetLastError();
+}
+
+// The first step from phi0 itself (data = phi0): D0 and the step from the
+// same seven values, with no copy of phi0 and no D0 field.  Bit for bit the
+// distance kernel followed by the step kernel on that first iteration.
+__global__ void reinit_first_step_kernel(Box bx, real* next, const real* phi0, real dt_r, real dx,
+                                         real dy, real dz) {
+  FillIter it(bx);
+  real max_dx = tmax(tmax(dx, dy), dz);
+  real flag = real(-1000.0) * max_dx;
+  real inv_dx = real(1.0) / dx, inv_dy = real(1.0) / dy, inv_dz = real(1.0) / dz;
+  int64_t sy = bx.nx, sz = (int64_t)bx.nx * bx.ny;
+  for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < it.n;
+       f += (int64_t)gridDim.x * blockDim.x) {
+    int i, j, k;
+    it.ijk(f, i, j, k);
+    int64_t idx = bx.at(i, j, k);
+    real c = phi0[idx];
+    real xp = phi0[idx + 1], xm = phi0[idx - 1];
+    real yp = phi0[idx + sy], ym = phi0[idx - sy];
+    real zp = phi0[idx + sz], zm = phi0[idx - sz];
+    real d0 = reinit_d0(c, xp, xm, yp, ym, zp, zm, flag);
+    real gp[3] = {(xp - c) * inv_dx, (yp - c) * inv_dy, (zp - c) * inv_dz};
+    real gm[3] = {(c - xm) * inv_dx, (c - ym) * inv_dy, (c - zm) * inv_dz};
+    next[idx] = c + dt_r * reinit_rhs(c, gp, gm, c, d0, flag, max_dx);
+  }
+}
+hipError_t launch_reinit_first_step(const Box& bx, real* next, const real* phi0, real dt_r,
+                                    real dx, real dy, real dz, hipStream_t st) {
+  FillIter it(bx);
+  hipLaunchKernelGGL(reinit_first_step_kernel, dim3(stream_blocks(it.n)), dim3(256), 0, st, bx,
+                     next, phi0, dt_r, dx, dy, dz);
   return hipGetLastError();
 }
 

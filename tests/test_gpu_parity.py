@@ -90,12 +90,18 @@ def test_labels_bit_exact(p):
     assert np.array_equal(res[0], res[1])
 
 
-def test_reinit_and_bc_bit_exact():
-    print("case reinit+bc", flush=True)
+@pytest.mark.parametrize("tmax_r", [None, 1.6])
+def test_reinit_and_bc_bit_exact(tmax_r):
+    """tmax_r None: the reference default (0.4, one iteration at dx = 1, which
+    the library runs as one fused pass); 1.6: four iterations from a copy of
+    phi0 and a stored D0 field."""
+    print(f"case reinit+bc tmax_r={tmax_r}", flush=True)
     g, mask, phi0 = case(32)
     lib = hiplib.load(np.float64)
     orc = op.oracle_lib(np.float64)
     o = options(np.float64)
+    if tmax_r is not None:
+        o.tmax_r = tmax_r
     a = mask.copy()
     b = mask.copy()
     orc.orc_reinitializeSubcellFix3d(a.ctypes.data_as(C.c_void_p), C.byref(g), C.byref(o))
