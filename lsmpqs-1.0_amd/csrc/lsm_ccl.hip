@@ -14,12 +14,28 @@
 
 namespace lsm {
 
+// Entries that other threads change concurrently are read and written with
+// relaxed atomics: every access goes to memory, and the accesses keep their
+// LDS or global address space (volatile ones would be issued as flat
+// instructions).
+__device__ __forceinline__ int ld_dev(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_dev(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int ld_wg(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void st_wg(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 __device__ __forceinline__ int uf_find(int* L, int x) {
-  volatile int* V = L;
-  int p = V[x];
+  int p = ld_dev(&L[x]);
   while (p != x) {
-    int gp = V[p];
-    if (gp != p) V[x] = gp;  // path halving; gp is an ancestor of x
+    int gp = ld_dev(&L[p]);
+    if (gp != p) st_dev(&L[x], gp);  // path halving; gp is an ancestor of x
     x = p;
     p = gp;
   }
@@ -50,11 +66,11 @@ static_assert(CX == 64, "one wave per tile row");
 
 // root of x with path halving (each visited entry is re-pointed at its
 // grandparent, an ancestor, so concurrent finds and links stay consistent)
-__device__ __forceinline__ int lds_find(volatile int* p, int x) {
-  int q = p[x];
+__device__ __forceinline__ int lds_find(int* p, int x) {
+  int q = ld_wg(&p[x]);
   while (q != x) {
-    int g = p[q];
-    if (g != q) p[x] = g;
+    int g = ld_wg(&p[q]);
+    if (g != q) st_wg(&p[x], g);
     x = q;
     q = g;
   }
@@ -99,7 +115,11 @@ __global__ __launch_bounds__(CX * CY) void ccl_local_kernel(Box bx, const uint8_
   // +x neighbour is already joined by the run.  A union (l, l + (di, dj, dk))
   // with di != 0 is implied when l + di is foreground (that voxel is joined
   // to l by the run and to the target by the pair with di = 0); one with
-  // di = 0, dj != 0, dk = 1 is implied when l + dj (y) is foreground.
+  // di = 0, dj != 0, dk = 1 is implied when l + dj (y) is foreground.  Any
+  // union (l, m) is implied when l - 1 and m - 1 are both foreground: voxel
+  // l - 1 takes the same-direction pair (l - 1, m - 1) (or skips it by these
+  // rules), and the runs join l - 1 to l and m - 1 to m.  Inside a run only
+  // the first voxel facing a run of the neighbouring row then unites with it.
   for (int z = 0; z < CZ; ++z) {
     if (!(fgm >> z & 1u)) continue;
     const int l = tx + CX * (ty + CY * z);
@@ -118,6 +138,7 @@ __global__ __launch_bounds__(CX * CY) void ccl_local_kernel(Box bx, const uint8_
           if (x2 < 0 || x2 >= CX || y2 < 0 || y2 >= CY || z2 >= CZ) continue;
           int m = x2 + CX * (y2 + CY * z2);
           if (p[m] < 0) continue;
+          if (xm && x2 >= 1 && p[m - 1] >= 0) continue;
           lds_unite(p, l, m);
         }
   }
@@ -219,12 +240,11 @@ __global__ __launch_bounds__(1024) void ccl_border_kernel(Box bx, int* L, const 
 // a concurrently finalised entry yields its parent or its root, both on the
 // way to the same root.
 __global__ void ccl_compress_kernel(int* L, int n) {
-  volatile int* V = L;
   for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < n; l += gridDim.x * blockDim.x) {
-    int r = V[l];
+    int r = ld_dev(&L[l]);
     if (r < 0) continue;
-    for (int q = V[r]; q != r; q = V[r]) r = q;
-    V[l] = r;
+    for (int q = ld_dev(&L[r]); q != r; q = ld_dev(&L[r])) r = q;
+    st_dev(&L[l], r);
   }
 }
 

@@ -4,7 +4,9 @@ binary volumes, 26- and 6-connectivity:
 
 * local pass: every foreground voxel starts at the end of its x run, and a union
   with a diagonal neighbour is skipped when it is implied through a foreground
-  x (or y) neighbour;
+  x (or y) neighbour; any union (l, m) is skipped when l - 1 and m - 1 (same
+  tile row) are both foreground: l - 1 takes the same-direction pair
+  (l - 1, m - 1), and the x runs join l - 1 to l and m - 1 to m;
 * cross-tile pass: a pair of neighbouring voxels in different tiles is united
   only by the voxel whose neighbour's tile is lexicographically larger in
   (z, y, x) -- each such voxel lies on a +x/+y/+z tile face.
@@ -87,8 +89,11 @@ def _label(fg, tiles, conn, skip_bug=None, seed=0):
                         continue
                     if (i2 // CX, j2 // CY, k2 // CZ) != (i // CX, j // CY, k // CZ):
                         continue
-                    if fg[k2, j2, i2]:
-                        unite(l, at(i2, j2, k2))
+                    if not fg[k2, j2, i2]:
+                        continue
+                    if xm and i2 % CX >= 1 and (fg[k2, j2, i2 - 1] or skip_bug == "prevx"):
+                        continue
+                    unite(l, at(i2, j2, k2))
     T = np.array([find(l) if flat[l] else -1 for l in range(n)])
     # cross-tile pass
     offs = [o for o in itertools.product((-1, 0, 1), repeat=3) if o != (0, 0, 0)]
@@ -120,7 +125,8 @@ def _expected(fg, conn):
 
 CASES = [((9, 13, 21), (8, 4, 4), 0.3, 26), ((10, 11, 19), (8, 4, 4), 0.55, 26),
          ((9, 9, 17), (8, 4, 4), 0.4, 6), ((12, 10, 20), (4, 4, 2), 0.7, 26),
-         ((20, 18, 70), (64, 16, 16), 0.45, 26), ((18, 17, 66), (64, 16, 16), 0.35, 6)]
+         ((20, 18, 70), (64, 16, 16), 0.45, 26), ((18, 17, 66), (64, 16, 16), 0.35, 6),
+         ((12, 14, 40), (16, 8, 4), 0.85, 26), ((12, 14, 40), (16, 8, 4), 0.8, 6)]
 
 
 @pytest.mark.parametrize("shape,tiles,p,conn", CASES)
@@ -129,10 +135,12 @@ def test_labelling_rules_match_scipy(shape, tiles, p, conn):
     assert (_label(fg, tiles, conn) == _expected(fg, conn)).all()
 
 
-@pytest.mark.parametrize("bug", ["link", "runs", "border"])
-def test_planted_rule_errors_are_caught(bug):
+@pytest.mark.parametrize("bug,conn", [("link", 26), ("runs", 26), ("border", 26),
+                                      ("prevx", 6)])
+def test_planted_rule_errors_are_caught(bug, conn):
     """Larger root linked under the smaller (labels become minima); runs not
     initialised although +x unions are still skipped; pairs crossing an x tile
-    face never united."""
+    face never united; the x-previous rule applied without checking that
+    m - 1 is foreground (26-connectivity mostly finds another path)."""
     fg = np.random.default_rng(11).random((10, 12, 20)) < 0.3
-    assert not (_label(fg, (8, 4, 4), 26, skip_bug=bug) == _expected(fg, 26)).all()
+    assert not (_label(fg, (8, 4, 4), conn, skip_bug=bug) == _expected(fg, conn)).all()
