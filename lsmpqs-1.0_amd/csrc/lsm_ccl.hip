@@ -1,8 +1,9 @@
 // lsm_ccl.hip -- connected-component labelling on the GPU and the
 // trapped-phase mask rebuild.
 //
-// Labelling is a concurrent union-find in two levels: inside 32x16x8 tiles in
-// LDS, then across tile boundaries in global memory.  Each voxel hooks itself
+// Labelling is a concurrent union-find in two levels: inside 64x16x16 tiles
+// in LDS, then across tile boundaries in global memory (three launches, one
+// per axis of the tile grid).  Each voxel hooks itself
 // to its foreground neighbours (13 of the 26, the other 13 hook to it) by
 // linking the smaller root under the larger one with atomicCAS; a final pass
 // compresses every path.  Linking only ever points a root at a larger index,
@@ -162,17 +163,19 @@ __global__ __launch_bounds__(CX * CY) void ccl_local_kernel(Box bx, const uint8_
 // voxels (l, m) in different tiles, with tile offset o = tile(m) - tile(l),
 // is united by l when o is positive in (z, y, x) lexicographic order and by
 // m otherwise, so every cross-tile pair is taken exactly once, and the voxel
-// taking it lies on the +x, +y or +z face of its tile.  One block per tile,
-// one thread per face voxel (+z face, then the rest of the +y face, then the
-// rest of the +x face).  Each thread reads the tile roots of its candidate
-// neighbours in one batch (T = -1 marks background), and the tile-root pairs
-// go through a block-wide hash set in LDS, so each distinct pair of tile
-// roots is united once per tile.  Unions start from the tile roots in T, so
-// only root entries of L move.
-constexpr int NFACE_Z = CX * CY;                // lz = CZ-1
-constexpr int NFACE_Y = CX * (CZ - 1);          // ly = CY-1, lz < CZ-1
-constexpr int NFACE_X = (CY - 1) * (CZ - 1);    // lx = CX-1, ly < CY-1, lz < CZ-1
-constexpr int NFACE = NFACE_Z + NFACE_Y + NFACE_X;  // 2209
+// taking it lies on the +x, +y or +z face of its tile.  The pairs are taken
+// in three launches by offset class -- o = (0, 0, 1) from the +x faces, then
+// o = (0, 1, *) from the +y faces, then o = (1, *, *) from the +z faces --
+// with a pass between them that points every tile root straight at its
+// current root.  Each launch then only builds chains along one axis of the
+// tile grid (at most its tile count long), where one launch over all pairs
+// built chains across the whole grid, and every hop of a find is a
+// device-coherent load.  One block per tile, one thread per face voxel.
+// Each thread reads the tile roots of its candidate neighbours in one batch
+// (T = -1 marks background), and the tile-root pairs go through a block-wide
+// hash set in LDS, so each distinct pair of tile roots is united once per
+// tile.  Unions start from the tile roots in T, so only root entries of L
+// move.
 constexpr int PAIR_SET = 2048;                  // slots of the per-tile pair set
 constexpr unsigned long long NO_PAIR = ~0ull;
 
@@ -189,13 +192,20 @@ __device__ __forceinline__ bool pair_insert(unsigned long long* set, int a, int 
   return true;
 }
 
-// face job e of tile blockIdx
+// face voxels of one tile for offset class AX (0: +x face, 1: +y, 2: +z)
+template <int AX>
+constexpr int face_count() {
+  return AX == 0 ? CY * CZ : (AX == 1 ? CX * CZ : CX * CY);
+}
+
+// face job e of tile blockIdx, offset class AX
+template <int AX>
 __device__ __forceinline__ void border_job(const Box& bx, int* L, const int* T, int conn,
                                            unsigned long long* set, int plane, int e) {
   int lx, ly, lz;
-  if (e < NFACE_Z) { lx = e % CX; ly = e / CX; lz = CZ - 1; }
-  else if (e < NFACE_Z + NFACE_Y) { int f = e - NFACE_Z; lx = f % CX; ly = CY - 1; lz = f / CX; }
-  else { int f = e - NFACE_Z - NFACE_Y; lx = CX - 1; ly = f % (CY - 1); lz = f / (CY - 1); }
+  if (AX == 0) { lx = CX - 1; ly = e % CY; lz = e / CY; }
+  else if (AX == 1) { lx = e % CX; ly = CY - 1; lz = e / CX; }
+  else { lx = e % CX; ly = e / CX; lz = CZ - 1; }
   const int i = blockIdx.x * CX + lx, j = blockIdx.y * CY + ly, k = blockIdx.z * CZ + lz;
   if (i >= bx.nx || j >= bx.ny || k >= bx.nz) return;
   const int l = i + bx.nx * j + plane * k;
@@ -211,7 +221,9 @@ __device__ __forceinline__ void border_job(const Box& bx, int* L, const int* T, 
     const int oz = (lz + dk) >= CZ ? 1 : ((lz + dk) < 0 ? -1 : 0);
     const int oy = (ly + dj) >= CY ? 1 : ((ly + dj) < 0 ? -1 : 0);
     const int ox = (lx + di) >= CX ? 1 : ((lx + di) < 0 ? -1 : 0);
-    if (!(oz > 0 || (oz == 0 && (oy > 0 || (oy == 0 && ox > 0))))) continue;
+    const bool cls = AX == 2 ? oz > 0 : (AX == 1 ? (oz == 0 && oy > 0)
+                                                 : (oz == 0 && oy == 0 && ox > 0));
+    if (!cls) continue;
     const int i2 = i + di, j2 = j + dj, k2 = k + dk;
     if (i2 < 0 || i2 >= bx.nx || j2 < 0 || j2 >= bx.ny || k2 < 0 || k2 >= bx.nz) continue;
     nb[q] = T[i2 + bx.nx * j2 + plane * k2];
@@ -226,12 +238,27 @@ __device__ __forceinline__ void border_job(const Box& bx, int* L, const int* T, 
   }
 }
 
+template <int AX>
 __global__ __launch_bounds__(1024) void ccl_border_kernel(Box bx, int* L, const int* T, int conn) {
   __shared__ unsigned long long set[PAIR_SET];
   for (int e = threadIdx.x; e < PAIR_SET; e += blockDim.x) set[e] = NO_PAIR;
   __syncthreads();
   const int plane = bx.nx * bx.ny;
-  for (int e = threadIdx.x; e < NFACE; e += blockDim.x) border_job(bx, L, T, conn, set, plane, e);
+  for (int e = threadIdx.x; e < face_count<AX>(); e += blockDim.x)
+    border_job<AX>(bx, L, T, conn, set, plane, e);
+}
+
+// Between the border launches: every tile root (T[v] == v) is pointed
+// straight at its current root.  Only tile-root entries are written, each by
+// its own thread; the walk reads entries that no thread writes meanwhile
+// except to another ancestor of the same root.
+__global__ void ccl_jump_kernel(int* L, const int* T, int n) {
+  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    if (T[v] != v) continue;
+    int r = ld_dev(&L[v]);
+    for (int q = ld_dev(&L[r]); q != r; q = ld_dev(&L[r])) r = q;
+    st_dev(&L[v], r);
+  }
 }
 
 // Final labels.  Each thread writes only its own entry: a find with path
@@ -260,7 +287,11 @@ hipError_t launch_label_components(const Box& bx, const uint8_t* bin, int* L, in
   if (n >= ((int64_t)1 << 31)) return hipErrorInvalidValue;
   dim3 tiles((bx.nx + CX - 1) / CX, (bx.ny + CY - 1) / CY, (bx.nz + CZ - 1) / CZ);
   hipLaunchKernelGGL(ccl_local_kernel, tiles, dim3(CX, CY, 1), 0, st, bx, bin, L, T, conn);
-  hipLaunchKernelGGL(ccl_border_kernel, tiles, dim3(1024), 0, st, bx, L, (const int*)T, conn);
+  hipLaunchKernelGGL(ccl_border_kernel<0>, tiles, dim3(256), 0, st, bx, L, (const int*)T, conn);
+  hipLaunchKernelGGL(ccl_jump_kernel, dim3(grid_for(n)), dim3(256), 0, st, L, (const int*)T, (int)n);
+  hipLaunchKernelGGL(ccl_border_kernel<1>, tiles, dim3(1024), 0, st, bx, L, (const int*)T, conn);
+  hipLaunchKernelGGL(ccl_jump_kernel, dim3(grid_for(n)), dim3(256), 0, st, L, (const int*)T, (int)n);
+  hipLaunchKernelGGL(ccl_border_kernel<2>, tiles, dim3(1024), 0, st, bx, L, (const int*)T, conn);
   hipLaunchKernelGGL(ccl_compress_kernel, dim3(grid_for(n)), dim3(256), 0, st, L, (int)n);
   return hipGetLastError();
 }
