@@ -90,9 +90,24 @@ __device__ __forceinline__ void lds_unite(int* p, int a, int b) {
   }
 }
 
+// unions of voxel column (tx, ty) with its neighbours one step along
+// (di, dj, dk), at the planes z of the bit mask `need`
+__device__ __forceinline__ void unite_planes(int* p, unsigned need, int tx, int ty, int di, int dj,
+                                             int dk) {
+  while (need) {
+    const int z = __builtin_ctz(need);
+    need &= need - 1;
+    const int l = tx + CX * (ty + CY * z);
+    lds_unite(p, l, l + di + CX * (dj + CY * dk));
+  }
+}
+
 __global__ __launch_bounds__(CX * CY) void ccl_local_kernel(Box bx, const uint8_t* bin, int* L,
                                                           int* T, int conn) {
   __shared__ int p[CN];
+  // bit z of cm[ty + 1][tx + 1] is voxel (tx, ty, z) of the tile; the frame
+  // of one column around the tile reads as background
+  __shared__ unsigned short cm[CY + 2][CX + 2];
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int i = blockIdx.x * CX + tx, j = blockIdx.y * CY + ty, kb = blockIdx.z * CZ;
   const bool col = i < bx.nx && j < bx.ny;
@@ -111,38 +126,47 @@ __global__ __launch_bounds__(CX * CY) void ccl_local_kernel(Box bx, const uint8_
     const int run = ones ? __ffsll((long long)ones) - 1 : CX - tx;
     p[l] = fg ? l + run - 1 : -1;
   }
-  __syncthreads();
-  // Unions with the forward neighbours of the 26- (6-) neighbourhood.  The
-  // +x neighbour is already joined by the run.  A union (l, l + (di, dj, dk))
-  // with di != 0 is implied when l + di is foreground (that voxel is joined
-  // to l by the run and to the target by the pair with di = 0); one with
-  // di = 0, dj != 0, dk = 1 is implied when l + dj (y) is foreground.  Any
-  // union (l, m) is implied when l - 1 and m - 1 are both foreground: voxel
-  // l - 1 takes the same-direction pair (l - 1, m - 1) (or skips it by these
-  // rules), and the runs join l - 1 to l and m - 1 to m.  Inside a run only
-  // the first voxel facing a run of the neighbouring row then unites with it.
-  for (int z = 0; z < CZ; ++z) {
-    if (!(fgm >> z & 1u)) continue;
-    const int l = tx + CX * (ty + CY * z);
-    const bool xm = tx > 0 && p[l - 1] >= 0, xp = tx < CX - 1 && p[l + 1] >= 0;
-    const bool ym = ty > 0 && p[l - CX] >= 0, yp = ty < CY - 1 && p[l + CX] >= 0;
-    for (int dk = 0; dk <= 1; ++dk)
-      for (int dj = (dk ? -1 : 0); dj <= 1; ++dj)
-        for (int di = ((dk || dj) ? -1 : 1); di <= 1; ++di) {
-          if (dk == 0 && dj == 0) continue;  // +x: the run
-          if (conn == 6 && (abs(di) + abs(dj) + abs(dk)) != 1) continue;
-          if (conn != 6) {
-            if ((di == 1 && xp) || (di == -1 && xm)) continue;
-            if (di == 0 && dk == 1 && ((dj == 1 && yp) || (dj == -1 && ym))) continue;
-          }
-          int x2 = tx + di, y2 = ty + dj, z2 = z + dk;
-          if (x2 < 0 || x2 >= CX || y2 < 0 || y2 >= CY || z2 >= CZ) continue;
-          int m = x2 + CX * (y2 + CY * z2);
-          if (p[m] < 0) continue;
-          if (xm && x2 >= 1 && p[m - 1] >= 0) continue;
-          lds_unite(p, l, m);
-        }
+  cm[ty + 1][tx + 1] = (unsigned short)fgm;
+  for (int e = ty * CX + tx; e < (CY + 2) * (CX + 2); e += CX * CY) {
+    const int r = e / (CX + 2), c = e - r * (CX + 2);
+    if (r == 0 || r == CY + 1 || c == 0 || c == CX + 1) cm[r][c] = 0;
   }
+  __syncthreads();
+  // Unions with the forward neighbours of the 26- (6-) neighbourhood, all 16
+  // planes of the column at once as bit masks.  The +x neighbour is already
+  // joined by the run.  A union (l, l + (di, dj, dk)) with di != 0 is implied
+  // when l + di is foreground (that voxel is joined to l by the run and to
+  // the target by the pair with di = 0); one with di = 0, dj != 0, dk = 1 is
+  // implied when l + dj (y) is foreground.  A union (l, m) with di >= 0 is
+  // implied when l - 1 and m - 1 are both foreground: voxel l - 1 takes the
+  // same-direction pair (l - 1, m - 1) (or skips it by these rules), and the
+  // runs join l - 1 to l and m - 1 to m.  (With di = -1 the first rule
+  // already covers it.)  Inside a run only the first voxel facing a run of
+  // the neighbouring row then unites with it.
+  unsigned C[3][3];  // C[a + 1][b + 1]: foreground planes of column (tx + a, ty + b)
+#pragma unroll
+  for (int b = 0; b < 3; ++b)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) C[a][b] = cm[ty + b][tx + a];
+  const unsigned xm = C[0][1], xp = C[2][1];
+#pragma unroll
+  for (int dk = 0; dk <= 1; ++dk)
+#pragma unroll
+    for (int dj = (dk ? -1 : 0); dj <= 1; ++dj)
+#pragma unroll
+      for (int di = ((dk || dj) ? -1 : 1); di <= 1; ++di) {
+        if (dk == 0 && dj == 0) continue;  // +x: the run
+        const bool face = (abs(di) + abs(dj) + abs(dk)) == 1;
+        if (conn == 6 && !face) continue;
+        unsigned need = fgm & (C[di + 1][dj + 1] >> dk);
+        if (conn != 6) {
+          if (di == 1) need &= ~xp;
+          if (di == -1) need &= ~xm;
+          if (di == 0 && dk == 1 && dj != 0) need &= ~C[1][dj + 1];
+        }
+        if (di >= 0) need &= ~(xm & (C[di][dj + 1] >> dk));
+        unite_planes(p, need, tx, ty, di, dj, dk);
+      }
   __syncthreads();
   if (!col) return;
   for (int z = 0; z < CZ && kb + z < bx.nz; ++z) {
