@@ -11,6 +11,9 @@
 // the label the reference's sequential union-find produces
 // (reference: src/connectivity/connectivity.c:48-83,148-217).  Labels are
 // therefore bit-identical to the reference's, not just the same partition.
+#include <algorithm>
+#include <cstdint>
+
 #include "lsm_kernels.h"
 
 namespace lsm {
@@ -102,8 +105,11 @@ __device__ __forceinline__ void unite_planes(int* p, unsigned need, int tx, int 
   }
 }
 
+// roots: [count, tile root, tile root, ...] with room for `cap` roots; every
+// tile root is appended (when the list is full, the count still grows and
+// the root-jump passes scan T instead)
 __global__ __launch_bounds__(CX * CY) void ccl_local_kernel(Box bx, const uint8_t* bin, int* L,
-                                                          int* T, int conn) {
+                                                          int* T, int conn, int* roots, int cap) {
   __shared__ int p[CN];
   // bit z of cm[ty + 1][tx + 1] is voxel (tx, ty, z) of the tile; the frame
   // of one column around the tile reads as background
@@ -177,6 +183,10 @@ __global__ __launch_bounds__(CX * CY) void ccl_local_kernel(Box bx, const uint8_
       int r = lds_find(p, l);
       int rx = r % CX, ry = (r / CX) % CY, rz = r / (CX * CY);
       root = (blockIdx.x * CX + rx) + bx.nx * ((blockIdx.y * CY + ry) + bx.ny * (kb + rz));
+      if (r == l && roots) {
+        const int q = atomicAdd(&roots[0], 1);
+        if (q < cap) roots[1 + q] = g;
+      }
     }
     L[g] = root;
     T[g] = root;
@@ -272,51 +282,68 @@ __global__ __launch_bounds__(1024) void ccl_border_kernel(Box bx, int* L, const 
     border_job<AX>(bx, L, T, conn, set, plane, e);
 }
 
-// Between the border launches: every tile root (T[v] == v) is pointed
-// straight at its current root.  Only tile-root entries are written, each by
-// its own thread; the walk reads entries that no thread writes meanwhile
-// except to another ancestor of the same root.
-__global__ void ccl_jump_kernel(int* L, const int* T, int n) {
-  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
-    if (T[v] != v) continue;
-    int r = ld_dev(&L[v]);
-    for (int q = ld_dev(&L[r]); q != r; q = ld_dev(&L[r])) r = q;
-    st_dev(&L[v], r);
+// After each border launch: every tile root (T[v] == v) is pointed straight
+// at its current root, from the root list of the local pass, or by a scan of
+// T when there is no list or it overflowed.  Only tile-root entries are
+// written, each by its own thread; a walk reads entries that no thread
+// writes meanwhile except to another ancestor of the same root.
+__device__ __forceinline__ void jump_root(int* L, int v) {
+  int r = ld_dev(&L[v]);
+  for (int q = ld_dev(&L[r]); q != r; q = ld_dev(&L[r])) r = q;
+  st_dev(&L[v], r);
+}
+__global__ void ccl_jump_kernel(int* L, const int* T, int n, const int* roots, int cap) {
+  const int stride = gridDim.x * blockDim.x, first = blockIdx.x * blockDim.x + threadIdx.x;
+  const int count = roots ? roots[0] : 0;
+  if (roots && count <= cap) {
+    for (int q = first; q < count; q += stride) jump_root(L, roots[1 + q]);
+  } else {
+    for (int v = first; v < n; v += stride)
+      if (T[v] == v) jump_root(L, v);
   }
 }
 
-// Final labels.  Each thread writes only its own entry: a find with path
-// halving here could re-point an entry another thread has already set to
-// its root at a non-root ancestor, after that thread's last write.  Reading
-// a concurrently finalised entry yields its parent or its root, both on the
-// way to the same root.
-__global__ void ccl_compress_kernel(int* L, int n) {
-  for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < n; l += gridDim.x * blockDim.x) {
-    int r = ld_dev(&L[l]);
-    if (r < 0) continue;
-    for (int q = ld_dev(&L[r]); q != r; q = ld_dev(&L[r])) r = q;
-    st_dev(&L[l], r);
+// Full labels once every tile root points at its component's root: each
+// voxel takes its tile root's entry (tile-root entries stay as they are).
+__global__ void ccl_resolve_kernel(int* L, const int* T, int n) {
+  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    const int t = T[v];
+    if (t >= 0 && t != v) L[v] = L[t];
   }
 }
+
+__global__ void ccl_reset_kernel(int* roots) { roots[0] = 0; }
 
 static int grid_for(int64_t n) {
   int64_t b = (n + 255) / 256;
   return (int)(b > 16384 ? 16384 : (b < 1 ? 1 : b));
 }
 
-// labels: root (largest linear index of the component) on foreground, -1 on background
+// labels: root (largest linear index of the component) on foreground, -1 on
+// background; with full = false only the tile roots' entries hold it (the
+// label of voxel v is then L[T[v]]).
 hipError_t launch_label_components(const Box& bx, const uint8_t* bin, int* L, int* T, int conn,
-                                   hipStream_t st) {
+                                   hipStream_t st, int* roots, int64_t roots_cap, bool full) {
   int64_t n = bx.size();
   if (n >= ((int64_t)1 << 31)) return hipErrorInvalidValue;
+  const int cap = roots ? (int)std::min<int64_t>(roots_cap, INT32_MAX - 1) : 0;
   dim3 tiles((bx.nx + CX - 1) / CX, (bx.ny + CY - 1) / CY, (bx.nz + CZ - 1) / CZ);
-  hipLaunchKernelGGL(ccl_local_kernel, tiles, dim3(CX, CY, 1), 0, st, bx, bin, L, T, conn);
+  const int gj = roots ? (int)std::min<int64_t>(std::max<int64_t>((cap + 255) / 256, 1), 16384)
+                       : grid_for(n);
+  if (roots) hipLaunchKernelGGL(ccl_reset_kernel, dim3(1), dim3(1), 0, st, roots);
+  hipLaunchKernelGGL(ccl_local_kernel, tiles, dim3(CX, CY, 1), 0, st, bx, bin, L, T, conn, roots,
+                     cap);
   hipLaunchKernelGGL(ccl_border_kernel<0>, tiles, dim3(256), 0, st, bx, L, (const int*)T, conn);
-  hipLaunchKernelGGL(ccl_jump_kernel, dim3(grid_for(n)), dim3(256), 0, st, L, (const int*)T, (int)n);
+  hipLaunchKernelGGL(ccl_jump_kernel, dim3(gj), dim3(256), 0, st, L, (const int*)T, (int)n,
+                     (const int*)roots, cap);
   hipLaunchKernelGGL(ccl_border_kernel<1>, tiles, dim3(1024), 0, st, bx, L, (const int*)T, conn);
-  hipLaunchKernelGGL(ccl_jump_kernel, dim3(grid_for(n)), dim3(256), 0, st, L, (const int*)T, (int)n);
+  hipLaunchKernelGGL(ccl_jump_kernel, dim3(gj), dim3(256), 0, st, L, (const int*)T, (int)n,
+                     (const int*)roots, cap);
   hipLaunchKernelGGL(ccl_border_kernel<2>, tiles, dim3(1024), 0, st, bx, L, (const int*)T, conn);
-  hipLaunchKernelGGL(ccl_compress_kernel, dim3(grid_for(n)), dim3(256), 0, st, L, (int)n);
+  hipLaunchKernelGGL(ccl_jump_kernel, dim3(gj), dim3(256), 0, st, L, (const int*)T, (int)n,
+                     (const int*)roots, cap);
+  if (full) hipLaunchKernelGGL(ccl_resolve_kernel, dim3(grid_for(n)), dim3(256), 0, st, L,
+                               (const int*)T, (int)n);
   return hipGetLastError();
 }
 
@@ -326,9 +353,11 @@ hipError_t launch_label_components(const Box& bx, const uint8_t* bin, int* L, in
 // and 0 elsewhere; a voxel is "disconnected" when that value is neither the
 // value at the inlet/outlet index nor 0.
 
-__device__ __forceinline__ int conn_value(const int* L, int64_t idx) {
-  int v = L[idx];
-  return v >= 0 ? v : 0;
+// through the tile root: L holds the label at tile roots (and, after a full
+// labelling, everywhere)
+__device__ __forceinline__ int conn_value(const int* L, const int* T, int64_t idx) {
+  int t = T[idx];
+  return t >= 0 ? L[t] : 0;
 }
 
 // phi = merge(phi, mask_nw); bin = phi < 0, plus the inlet plane forced to 1
@@ -358,10 +387,10 @@ __global__ void trap_nw_bin_kernel(TrapArgs a) {
 __global__ void trap_w_bin_kernel(TrapArgs a) {
   const Box bx = a.bx;
   int64_t n = bx.size();
-  int main_val = conn_value(a.labels, a.nw_ind);
+  int main_val = conn_value(a.labels, a.troot, a.nw_ind);
   for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n;
        l += (int64_t)gridDim.x * blockDim.x) {
-    int cv = conn_value(a.labels, l);
+    int cv = conn_value(a.labels, a.troot, l);
     bool disc = !(cv == main_val || cv == 0);
     real p = a.phi[l];
     real mn = (disc && p < real(0)) ? -p : a.fill;
@@ -385,10 +414,10 @@ __global__ void trap_w_bin_kernel(TrapArgs a) {
 
 __global__ void trap_w_mask_kernel(TrapArgs a) {
   int64_t n = a.bx.size();
-  int main_val = conn_value(a.labels, a.w_ind);
+  int main_val = conn_value(a.labels, a.troot, a.w_ind);
   for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n;
        l += (int64_t)gridDim.x * blockDim.x) {
-    int cv = conn_value(a.labels, l);
+    int cv = conn_value(a.labels, a.troot, l);
     bool disc = !(cv == main_val || cv == 0);
     real s = a.s[l];
     a.mask_w[l] = (disc && s < real(0)) ? -s : a.fill;
@@ -398,10 +427,12 @@ __global__ void trap_w_mask_kernel(TrapArgs a) {
 hipError_t launch_trap_components(const TrapArgs& a, int conn, hipStream_t st) {
   int64_t n = a.bx.size();
   hipLaunchKernelGGL(trap_nw_bin_kernel, dim3(grid_for(n)), dim3(256), 0, st, a);
-  hipError_t e = launch_label_components(a.bx, a.bin, a.labels, a.troot, conn, st);
+  hipError_t e = launch_label_components(a.bx, a.bin, a.labels, a.troot, conn, st, a.roots,
+                                         a.roots_cap, false);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(trap_w_bin_kernel, dim3(grid_for(n)), dim3(256), 0, st, a);
-  e = launch_label_components(a.bx, a.bin, a.labels, a.troot, conn, st);
+  e = launch_label_components(a.bx, a.bin, a.labels, a.troot, conn, st, a.roots, a.roots_cap,
+                              a.full_labels);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(trap_w_mask_kernel, dim3(grid_for(n)), dim3(256), 0, st, a);
   return hipGetLastError();

@@ -109,12 +109,15 @@ static void reinit_medium_dev(Session* s, real*& f, const Grid* g, const Options
 }
 
 // Trapped-phase masks (reference: src/connectivity/trapComponents.c:156-375)
-static void trap_dev(Session* s, const Grid* g, const Options* o) {
+// full_labels: leave the wetting-phase labels resolved at every voxel (for
+// callers that read them back); otherwise only tile roots carry them
+static void trap_dev(Session* s, const Grid* g, const Options* o, bool full_labels = false) {
   ensure_labels(s);
   TrapArgs ta{};
   ta.bx = s->bx;
   ta.phi = s->phi; ta.mask = s->mask; ta.mask_nw = s->mask_nw; ta.mask_w = s->mask_w;
   ta.s = s->S; ta.bin = s->bin; ta.labels = s->labels; ta.troot = s->troot;
+  ta.roots = s->roots; ta.roots_cap = s->roots_cap; ta.full_labels = full_labels;
   ta.nw_ind = o->phi_nw_ind; ta.w_ind = o->phi_w_ind;
   ta.center_inlet = o->center_inlet;
   ta.fill = -20 * g->dx[0];
@@ -653,7 +656,7 @@ void trapComponents_mask(QSS_DataArrays* p, Grid* g, Options* o) {
   up(s, s->phi, p->phi);
   up(s, s->mask, p->mask);
   up(s, s->mask_nw, p->mask_nw);
-  trap_dev(s, g, o);
+  trap_dev(s, g, o, p->connectivity != nullptr);
   down(s, p->phi, s->phi);
   down(s, p->mask_nw, s->mask_nw);
   down(s, p->mask_w, s->mask_w);
@@ -682,7 +685,8 @@ void findConnectivity3d(QSS_DataArrays* p, Grid* g) {
   ensure_labels(s);
   DCHECK(hipMemcpyAsync(s->bin, p->phi_bin, (size_t)s->n, hipMemcpyHostToDevice, s->st));
   int conn = getenv("LSMPQS_CONNECTIVITY") ? atoi(getenv("LSMPQS_CONNECTIVITY")) : 26;
-  DCHECK(launch_label_components(s->bx, s->bin, s->labels, s->troot, conn == 6 ? 6 : 26, s->st));
+  DCHECK(launch_label_components(s->bx, s->bin, s->labels, s->troot, conn == 6 ? 6 : 26, s->st,
+                                 s->roots, s->roots_cap, true));
   std::vector<int> lab((size_t)s->n);
   DCHECK(hipMemcpyAsync(lab.data(), s->labels, (size_t)s->n * sizeof(int), hipMemcpyDeviceToHost,
                         s->st));

@@ -205,14 +205,21 @@ struct TrapArgs {
   uint8_t* bin;
   int* labels;
   int* troot;            // tile roots (labelling scratch)
+  int* roots;            // tile-root list (labelling scratch) or NULL
+  int64_t roots_cap;     // room for roots in that list
+  bool full_labels;      // leave the wetting-phase labels resolved at every voxel
   int nw_ind, w_ind;
   int center_inlet;
   real fill;             // -20 dx
 };
 hipError_t launch_trap_components(const TrapArgs& a, int connectivity, hipStream_t st);
 // labels: largest linear index of the component on foreground, -1 on
-// background; troot: scratch of the same size
+// background; troot: scratch of the same size (each voxel's tile root);
+// roots: optional list [count, roots...] with room for roots_cap tile roots
+// (NULL: the passes between the border launches scan troot).  full = false
+// leaves the label only at the tile roots: the label of v is labels[troot[v]].
 hipError_t launch_label_components(const Box& bx, const uint8_t* bin, int* labels, int* troot,
-                                   int connectivity, hipStream_t st);
+                                   int connectivity, hipStream_t st, int* roots = nullptr,
+                                   int64_t roots_cap = 0, bool full = true);
 
 }  // namespace lsm

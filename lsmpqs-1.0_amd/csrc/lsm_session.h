@@ -40,6 +40,8 @@ struct Session {
   real *gx = nullptr, *gy = nullptr, *gz = nullptr;
   int* labels = nullptr;
   int* troot = nullptr;
+  int* roots = nullptr;   // tile-root list of the labelling: [count, roots...]
+  int64_t roots_cap = 0;
   uint8_t* bin = nullptr;
   double *part = nullptr, *red = nullptr;
   real* dt = nullptr;

@@ -68,8 +68,8 @@ static void dalloc(P*& p, int64_t count) {
 static void free_all(Session* s) {
   void* bufs[] = {s->phi, s->A, s->B, s->S, s->D, s->phi_prev, s->mask, s->mask_w, s->mask_nw,
                   s->floor, s->phi_extra, s->vn, s->b, s->vx, s->vy, s->vz, s->gx, s->gy, s->gz,
-                  s->labels, s->troot, s->bin, s->part, s->red, s->dt, s->count, s->stale_idx,
-                  s->stale_src, s->stale_list};
+                  s->labels, s->troot, s->roots, s->bin, s->part, s->red, s->dt, s->count,
+                  s->stale_idx, s->stale_src, s->stale_list};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (s->h_dt) (void)hipHostFree(s->h_dt);
@@ -155,6 +155,10 @@ void ensure_labels(Session* s) {
   dalloc(s->labels, s->n);
   dalloc(s->troot, s->n);
   dalloc(s->bin, s->n);
+  // room for n/16 tile roots (a 64x16x16 tile holds 16384 voxels); a labelling
+  // with more falls back to scanning troot
+  s->roots_cap = s->n / 16 + 1;
+  dalloc(s->roots, s->roots_cap + 1);
 }
 
 void session_release(const QSS_DataArrays* p) {
