@@ -94,31 +94,30 @@ int64_t ghost_count(const Box& bx);
 // of the fill rows.
 __host__ __device__ inline void ghost_ijk(const Box& bx, int64_t g, int64_t nA, int64_t nB, int& i,
                                           int& j, int& k) {
-  int nxf = bx.ihi - bx.ilo + 1, nyf = bx.jhi - bx.jlo + 1;
-  int gxn = bx.nx - nxf, gyn = bx.ny - nyf;
+  // 32-bit divisions: a ghost box holds fewer than 2^31 voxels
+  const unsigned nx = (unsigned)bx.nx, ny = (unsigned)bx.ny;
+  const unsigned nxf = (unsigned)(bx.ihi - bx.ilo + 1), nyf = (unsigned)(bx.jhi - bx.jlo + 1);
+  const unsigned gxn = nx - nxf, gyn = ny - nyf;
   if (g < nA) {
-    int64_t plane = (int64_t)bx.nx * bx.ny;
-    int kk = (int)(g / plane);
-    int64_t r = g - kk * plane;
-    j = (int)(r / bx.nx);
-    i = (int)(r - (int64_t)j * bx.nx);
-    k = kk < bx.klo ? kk : bx.khi + 1 + (kk - bx.klo);
+    const unsigned u = (unsigned)g, t = u / nx, kk = t / ny;
+    i = (int)(u - t * nx);
+    j = (int)(t - kk * ny);
+    k = (int)kk < bx.klo ? (int)kk : bx.khi + 1 + ((int)kk - bx.klo);
   } else if (g < nA + nB) {
-    int64_t h = g - nA;
-    i = (int)(h % bx.nx);
-    int64_t t = h / bx.nx;
-    int jj = (int)(t % gyn);
-    k = bx.klo + (int)(t / gyn);
+    const unsigned h = (unsigned)(g - nA), t = h / nx, kk = t / gyn;
+    i = (int)(h - t * nx);
+    const int jj = (int)(t - kk * gyn);
+    k = bx.klo + (int)kk;
     j = jj < bx.jlo ? jj : bx.jhi + 1 + (jj - bx.jlo);
   } else {
-    int64_t h = g - nA - nB;
-    int ii = (int)(h % gxn);
-    int64_t t = h / gxn;
-    j = bx.jlo + (int)(t % nyf);
-    k = bx.klo + (int)(t / nyf);
+    const unsigned h = (unsigned)(g - nA - nB), t = h / gxn, kk = t / nyf;
+    const int ii = (int)(h - t * gxn);
+    j = bx.jlo + (int)(t - kk * nyf);
+    k = bx.klo + (int)kk;
     i = ii < bx.ilo ? ii : bx.ihi + 1 + (ii - bx.ilo);
   }
 }
+
 __host__ __device__ inline void ghost_regions(const Box& bx, int64_t& nA, int64_t& nB,
                                               int64_t& nC) {
   int nxf = bx.ihi - bx.ilo + 1, nyf = bx.jhi - bx.jlo + 1, nzf = bx.khi - bx.klo + 1;
