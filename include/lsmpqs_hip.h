@@ -47,8 +47,9 @@ QSSLIB_REAL constCurvModel3dNoVar(Options *options, QSS_DataArrays *p, Grid *g, 
  * Read phi (data), mask and grid files (.gz allowed), run drain_model3d /
  * imbibe_model3d with the log in options->outfile, write
  * options->path/data_final.gz when save_data.  Returns 0, or 1 on an input
- * error, a mismatch of the array and grid dimensions, a 2D grid (only 3D is
- * supported) or a device error (message in lsmpqs_hip_last_error()). */
+ * error, a mismatch of the array and grid dimensions or a device error
+ * (message in lsmpqs_hip_last_error()).  2D grids run drain_model2d /
+ * imbibe_model2d, as the reference does (drain_top.c:116-119). */
 int drainTop(Options *options, char *fname_data_in, char *fname_grid_in, char *fname_mask_in);
 int imbibeTop(Options *options, char *fname_data_in, char *fname_grid_in, char *fname_mask_in);
 
@@ -59,6 +60,23 @@ int imbibeTop(Options *options, char *fname_data_in, char *fname_grid_in, char *
  * as the reference does, unless LSMPQS_NO_STEP_FILES=1. */
 void drain_model3d(Options *options, QSS_DataArrays *p, Grid *g, FILE *fp_out);
 void imbibe_model3d(Options *options, QSS_DataArrays *p, Grid *g, FILE *fp_out);
+
+/* ---- 2D models (reference: src/level_set_evolution/drain_model2d.h,
+ *      imbibe_model2d.h, constCurvModel2d.h) ----
+ * The same schedule on a 2D grid (grid_dims_ghostbox = {nx, ny, 1}), with
+ * the reference's 2D discretization: HJ-ENO2 / HJ-ENO3 (no WENO5 in 2D),
+ * the 2D subcell-fix reinitialization and 8-connected trapping.  Host arrays
+ * are copied to the device on entry and back on return.  use_var_theta is
+ * rejected (error message set).  constCurvModel2d / constCurvModel2dNoVar
+ * return -2 on an error, else what the reference returns (the elapsed time,
+ * or -1 when the saturation limit or the vanishing volume stops the run). */
+void drain_model2d(Options *options, QSS_DataArrays *p, Grid *g, FILE *fp_out);
+void imbibe_model2d(Options *options, QSS_DataArrays *p, Grid *g, FILE *fp_out);
+QSSLIB_REAL constCurvModel2d(Options *options, QSS_DataArrays *p, Grid *g, FILE *fp_out);
+QSSLIB_REAL constCurvModel2dNoVar(Options *options, QSS_DataArrays *p, Grid *g, FILE *fp_out,
+                                  QSSLIB_REAL a);
+/* reference: src/reinitialization/reinitialize_top.h (reinitialize_top.c:123-203) */
+void reinitializeSubcellFix2d(QSSLIB_REAL *data, Grid *g, Options *o);
 
 /* ---- trapping (reference: src/connectivity/connectivity.h:23-27) ---- */
 void trapComponents_mask(QSS_DataArrays *p, Grid *g, Options *o);

@@ -348,7 +348,8 @@ hipError_t launch_label_components(const Box& bx, const uint8_t* bin, int* L, in
 }
 
 // ---------------------------------------------------------------------------
-// Trapped-phase masks (reference: src/connectivity/trapComponents.c:156-375).
+// Trapped-phase masks (reference: src/connectivity/trapComponents.c:156-375;
+// the 2D inlet/outlet rules at :196-211 and :297-337).
 // The reference's connectivity value is the root label on foreground voxels
 // and 0 elsewhere; a voxel is "disconnected" when that value is neither the
 // value at the inlet/outlet index nor 0.
@@ -365,6 +366,7 @@ __global__ void trap_nw_bin_kernel(TrapArgs a) {
   const Box bx = a.bx;
   int64_t n = bx.size();
   int ci = (int)(0.5 * (bx.ilo + bx.ihi)), ck = (int)(0.5 * (bx.klo + bx.khi));
+  int cj = (int)(0.5 * (bx.jlo + bx.jhi));
   for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n;
        l += (int64_t)gridDim.x * blockDim.x) {
     real mn = a.mask_nw[l];
@@ -373,7 +375,14 @@ __global__ void trap_nw_bin_kernel(TrapArgs a) {
     uint8_t b = p < real(0) ? 1 : 0;
     int64_t t = l / bx.nx;
     int i = (int)(l - t * bx.nx), j = (int)(t % bx.ny), k = (int)(t / bx.ny);
-    if (a.center_inlet) {
+    if (a.dims2) {
+      // 2D: the centre point, or the whole ghost column i = ilo - 1
+      if (a.center_inlet) {
+        if (i == ci && j == cj) b = 1;
+      } else if (i == bx.ilo - 1) {
+        b = 1;
+      }
+    } else if (a.center_inlet) {
       if (i == ci && k == ck) b = 1;
     } else if (i == bx.ilo - 1 && j >= bx.jlo && j <= bx.jhi && k >= bx.klo && k <= bx.khi) {
       b = 1;
@@ -403,7 +412,15 @@ __global__ void trap_w_bin_kernel(TrapArgs a) {
     uint8_t b = s < real(0) ? 1 : 0;
     int64_t t = l / bx.nx;
     int i = (int)(l - t * bx.nx), j = (int)(t % bx.ny), k = (int)(t / bx.ny);
-    if (a.center_inlet) {
+    if (a.dims2) {
+      // 2D: the four ghost lines next to the fill box, or (as the reference
+      // does) the fill column i = ilo + 1
+      if (a.center_inlet) {
+        if (i == bx.ilo - 1 || i == bx.ihi + 1 || j == bx.jlo - 1 || j == bx.jhi + 1) b = 1;
+      } else if (i == bx.ilo + 1 && j >= bx.jlo && j <= bx.jhi) {
+        b = 1;
+      }
+    } else if (a.center_inlet) {
       if (k == bx.klo - 1 || k == bx.khi + 1 || i == bx.ilo - 1 || i == bx.ihi + 1) b = 1;
     } else if (i == bx.ihi + 1 && j >= bx.jlo && j <= bx.jhi && k >= bx.klo && k <= bx.khi) {
       b = 1;

@@ -341,8 +341,8 @@ int run_top(Options* o, char* fdata, char* fgrid, char* fmask, bool drain) {
       return 1;
     }
   }
-  if (g->num_dims != 3) {
-    fprintf(stderr, "lsmpqs_hip: only 3D grids are supported (num_dims = %d)\n", g->num_dims);
+  if (g->num_dims != 3 && g->num_dims != 2) {
+    fprintf(stderr, "lsmpqs_hip: grids must be 2D or 3D (num_dims = %d)\n", g->num_dims);
     destroyQSSDataArrays(p);
     destroyGrid(g);
     return 1;
@@ -362,8 +362,13 @@ int run_top(Options* o, char* fdata, char* fgrid, char* fmask, bool drain) {
     printGrid(g, fp);
   }
   fflush(fp);
-  if (drain) drain_model3d(o, p, g, fp);
-  else imbibe_model3d(o, p, g, fp);
+  if (g->num_dims == 3) {
+    if (drain) drain_model3d(o, p, g, fp);
+    else imbibe_model3d(o, p, g, fp);
+  } else {
+    if (drain) drain_model2d(o, p, g, fp);
+    else imbibe_model2d(o, p, g, fp);
+  }
   int rc = *lsmpqs_hip_last_error() ? 1 : 0;
   if (o->save_data) {
     std::string name = std::string(o->path) + "/data_final";

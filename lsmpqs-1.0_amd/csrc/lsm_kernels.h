@@ -210,6 +210,7 @@ struct TrapArgs {
   int nw_ind, w_ind;
   int center_inlet;
   real fill;             // -20 dx
+  int dims2;             // 2D grid on a one-plane box: the 2D inlet/outlet rules
 };
 hipError_t launch_trap_components(const TrapArgs& a, int connectivity, hipStream_t st);
 // labels: largest linear index of the component on foreground, -1 on

@@ -70,6 +70,7 @@ unsigned quirks();
 void set_quirks(unsigned q);
 void set_slabs(int s);
 int dt_slabs(const Grid* g);
+int dt_slab_count(int planes);  // the same for a fill box of that many slab planes
 
 void ensure_var(Session* s);     // vn, b, vx, vy, vz
 void ensure_grad(Session* s);    // mask gradient and reservoir box

@@ -24,10 +24,12 @@ unsigned quirks() { return g_quirks; }
 void set_quirks(unsigned q) { g_quirks = q; }
 void set_slabs(int s) { g_slabs = s; }
 
-// Number of z-slabs the reference's OpenMP team would use: LSMPQS_DT_SLABS,
+// Number of slabs the reference's OpenMP team would use: LSMPQS_DT_SLABS,
 // else OMP_NUM_THREADS, else the online CPU count, clamped to the number of
-// fill-box planes.
-int dt_slabs(const Grid* g) {
+// fill-box planes (z-planes in 3D, j-rows in 2D).
+int dt_slabs(const Grid* g) { return dt_slab_count(g->khi_fb - g->klo_fb + 1); }
+
+int dt_slab_count(int nsl) {
   int s = g_slabs;
   if (s <= 0) {
     const char* e = getenv("LSMPQS_DT_SLABS");
@@ -35,7 +37,6 @@ int dt_slabs(const Grid* g) {
     if (e && *e) s = atoi(e);
     if (s <= 0) s = (int)sysconf(_SC_NPROCESSORS_ONLN);
   }
-  int nsl = g->khi_fb - g->klo_fb + 1;
   if (s > nsl) s = nsl;
   if (s > 1024) s = 1024;
   if (s < 1) s = 1;

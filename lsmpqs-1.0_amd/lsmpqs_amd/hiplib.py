@@ -35,6 +35,8 @@ class BatchStats(C.Structure):
 EXPORTS = (
     "constCurvModel3d", "constCurvModel3dNoVar", "drain_model3d", "imbibe_model3d",
     "trapComponents_mask", "findConnectivity3d", "getMainInd_new", "reinitializeSubcellFix3d",
+    "drain_model2d", "imbibe_model2d", "constCurvModel2d", "constCurvModel2dNoVar",
+    "reinitializeSubcellFix2d",
     "signedLinearExtrapolationBCqss", "initializeDisconnectedMasks", "createReservoirInlet3d",
     "setSpaceDerivFunc", "setThetaOverlap", "createOptionsDefault", "destroyOptions",
     "writeDataArrayQSS", "readDataArrayQSS", "allocateQSSDataArrays",
@@ -67,6 +69,9 @@ def load(dtype):
     lib.constCurvModel3d.restype = R
     lib.constCurvModel3dNoVar.restype = R
     lib.constCurvModel3dNoVar.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, R]
+    lib.constCurvModel2d.restype = R
+    lib.constCurvModel2dNoVar.restype = R
+    lib.constCurvModel2dNoVar.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, R]
     lib.lsmpqs_hip_last_error.restype = C.c_char_p
     lib.lsmpqs_bench_inner_steps.restype = C.c_int
     lib.lsmpqs_bench_inner_steps.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
