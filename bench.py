@@ -361,6 +361,102 @@ def precision_leg(n=512, curv_steps=40, tmax=20.0, seed=1003):
     return out
 
 
+def disk_pack_2d(g, seed, porosity=0.55, rmin=6.0, rmax=14.0):
+    """(mask, phi0) on a 2D grid: signed distance to a Boolean disk pack
+    (> 0 in solid, clipped at -6), non-wetting phase in a slab of 12 dx at the
+    x-low face with the solid imposed.  Arrays (ny, nx), i fastest."""
+    rng = np.random.default_rng(seed)
+    nx, ny = g.grid_dims_ghostbox[0], g.grid_dims_ghostbox[1]
+    x = g.x_lo_ghostbox[0] + g.dx[0] * np.arange(nx)
+    y = g.x_lo_ghostbox[1] + g.dx[1] * np.arange(ny)
+    mask = np.full((ny, nx), -6.0)
+    area = (g.x_hi[0] - g.x_lo[0]) * (g.x_hi[1] - g.x_lo[1])
+    ndisk = int(np.ceil(-np.log(porosity) * area / (np.pi * (rmin * rmin + rmin * rmax + rmax * rmax) / 3)))
+    for _ in range(ndisk):
+        cx = rng.uniform(g.x_lo[0] + 16, g.x_hi[0])
+        cy = rng.uniform(g.x_lo[1], g.x_hi[1])
+        r = rng.uniform(rmin, rmax)
+        i0, i1 = np.searchsorted(x, cx - r - 7), np.searchsorted(x, cx + r + 7)
+        j0, j1 = np.searchsorted(y, cy - r - 7), np.searchsorted(y, cy + r + 7)
+        d = r - np.hypot(x[None, i0:i1] - cx, y[j0:j1, None] - cy)
+        np.maximum(mask[j0:j1, i0:i1], d, out=mask[j0:j1, i0:i1])
+    phi0 = np.broadcast_to(x[None, :] - (g.x_lo[0] + 12.0), mask.shape)
+    return mask, np.maximum(phi0, mask)
+
+
+def model2d_leg(n=1024, curv_steps=2, tmax=4.0, seed=1006, cpu=True):
+    """drain_model2d end to end on an n x n disk pack (HJ-ENO3 + TVD-RK3,
+    theta 45 deg, trapping on, curv_steps curvature steps from a = 0.05 by
+    0.02, pseudo-time tmax each, step files off): host wall time per
+    precision on the GPU, and the reference build's drain_model2d (FP64,
+    OpenMP) on the same input, with the largest difference of the final phi."""
+    import tempfile
+    from lsmpqs_amd.abi import Arrays
+    out = {"grid": [n, n], "curvature_steps": curv_steps, "tmax": tmax}
+    finals = {}
+    libc = C.CDLL("libc.so.6")
+    libc.fopen.restype = C.c_void_p
+    libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
+    devnull = C.c_void_p(libc.fopen(b"/dev/null", b"w"))  # both models write their log here
+
+    def grid_of(lib, dt):
+        Grid, _, _ = types_for(dt)
+        R = C.c_double if dt == np.float64 else C.c_float
+        lib.createGridSetDx.restype = C.POINTER(Grid)
+        lib.createGridSetDx.argtypes = [C.c_int, R, C.POINTER(R), C.POINTER(R), C.c_int]
+        return lib.createGridSetDx(2, 1.0, (R * 3)(0, 0, 0), (R * 3)(n, n, 0), 1).contents
+
+    def opts(dt):
+        o = default_options(dt)
+        o.order_space_accur, o.order_time_accur = 3, 3
+        o.theta = 45 * 3.14159 / 180
+        o.a, o.b, o.dc = 0.05, 0.04, 0.5
+        o.amax = o.a + o.b * o.dc * (curv_steps - 1) + 1e-3
+        o.tmax = tmax
+        return o
+
+    def run(lib, dt, mask, phi0, threads_env):
+        g = grid_of(lib, dt)
+        A = Arrays(g, dt)
+        A["mask"] = mask.reshape(A.shape)
+        A["phi"] = phi0.reshape(A.shape)
+        o = opts(dt)
+        cwd = os.getcwd()
+        os.chdir(tempfile.mkdtemp(prefix="lsm2d_bench_"))
+        try:
+            t0 = time.perf_counter()
+            lib.drain_model2d(C.byref(o), A.ptr(), C.byref(g), devnull)
+            el = time.perf_counter() - t0
+        finally:
+            os.chdir(cwd)
+        return el, A["phi"].reshape(mask.shape).astype(np.float64)
+
+    g64 = None
+    for key in ("f32", "f64"):
+        dt = np.float32 if key == "f32" else np.float64
+        lib = hiplib.load(dt)
+        if g64 is None:
+            g64 = grid_of(hiplib.load(np.float64), np.float64)
+            mask, phi0 = disk_pack_2d(g64, seed)
+        lib.lsmpqs_set_dt_slabs(16)
+        os.environ["LSMPQS_NO_STEP_FILES"] = "1"
+        el, fin = run(lib, dt, mask.astype(dt), phi0.astype(dt), None)
+        hiplib.check(lib)
+        out[key + "_s"] = round(el, 3)
+        finals[key] = fin
+    if cpu:
+        import oracle_py as op
+        if op.have_ref(np.float64):
+            threads = int(os.environ.get("OMP_NUM_THREADS", "16") or 16)
+            op.set_omp_threads(threads)
+            el, fin = run(op.ref_lib(np.float64), np.float64, mask, phi0, threads)
+            out["reference_f64_s"] = round(el, 3)
+            out["reference_threads"] = threads
+            out["max_abs_diff_f64_vs_reference"] = float(np.max(np.abs(finals["f64"] - fin)))
+    out["max_abs_diff_f32_vs_f64"] = float(np.max(np.abs(finals["f32"] - finals["f64"])))
+    return out
+
+
 def cpu_leg(n, steps=8):
     """The reference build (oracle/_ref, OpenMP, FP64) on a bounded sample:
     one outer step of constCurvModel3d at n^3 whose pseudo-time window holds
@@ -469,6 +565,9 @@ def main():
     ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint leg")
     ap.add_argument("--batch-only", action="store_true", help="only the batched-subvolume leg")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--model2d-only", action="store_true",
+                    help="only the 2D drainage leg (--model2d-n, --curve-steps, --curve-tmax)")
+    ap.add_argument("--model2d-n", type=int, default=2048)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-subvolume leg")
     ap.add_argument("--precision-only", action="store_true",
                     help="only the FP32-vs-FP64 drainage-curve comparison (--n, --curve-steps, "
@@ -488,6 +587,10 @@ def main():
     if args.precision_only:
         print(json.dumps({"precision": precision_leg(args.n, args.curve_steps, args.curve_tmax)}),
               flush=True)
+        return
+    if args.model2d_only:
+        print(json.dumps({"model2d": model2d_leg(args.model2d_n, args.curve_steps, args.curve_tmax,
+                                                 cpu=not args.no_cpu)}), flush=True)
         return
     if args.batch_only:
         print(json.dumps({"batch": batch_leg()}), flush=True)

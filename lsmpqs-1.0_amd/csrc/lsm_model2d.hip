@@ -400,7 +400,8 @@ struct Run2 {
   std::vector<void*> owned;
   real *phi = nullptr, *next = nullptr, *tmp = nullptr, *prev = nullptr, *mask = nullptr;
   real *mask_w = nullptr, *mask_nw = nullptr, *vn = nullptr, *bb = nullptr, *vx = nullptr,
-       *vy = nullptr, *mx = nullptr, *my = nullptr, *s = nullptr, *d0 = nullptr, *p0 = nullptr;
+       *vy = nullptr, *mx = nullptr, *my = nullptr, *s = nullptr, *d0 = nullptr, *p0 = nullptr,
+       *zero = nullptr;
   real* dt = nullptr;
   double *part = nullptr, *red = nullptr;
   uint8_t* bin = nullptr;
@@ -425,7 +426,7 @@ struct Run2 {
     const int nsl = b.jhi - b.jlo + 1;
     nslabs = dt_slab_count(nsl);
     real** f[] = {&phi, &next, &tmp, &prev, &mask, &mask_w, &mask_nw, &vn, &bb, &vx, &vy,
-                  &mx, &my, &s, &d0, &p0};
+                  &mx, &my, &s, &d0, &p0, &zero};
     for (real** q : f) {
       *q = alloc<real>(n);
       X2CHECK(launch_fill(*q, real(0), n, st));
@@ -629,20 +630,15 @@ void update_coeffs2(Run2& r, Options* o, const Grid* g, real a0, bool var) {
     hipLaunchKernelGGL(varcurvadv2d_kernel, dim3(blocks2(nf)), dim3(256), 0, r.st, r.b, r.bb, r.vx,
                        r.vy, r.mask, r.mx, r.my, o->b, g->dx[0]);
     X2CHECK(hipGetLastError());
-    // b_max and U_max over the fill box, in the reference's scan order (max is exact)
-    std::vector<real> hb(r.n), hx(r.n), hy(r.n);
-    r.down(hb.data(), r.bb);
-    r.down(hx.data(), r.vx);
-    r.down(hy.data(), r.vy);
+    // b_max and U_max = max |vx|/dx + |vy|/dx over the fill box (a max is
+    // exact in any order); the 3D reduction with a zero third component
+    // adds +0 to each U
     const real inv_dx = real(1.0) / g->dx[0], inv_dy = real(1.0) / g->dx[0];
-    real umax = real(-1), bmax = real(-1);
-    for (int j = r.b.jlo; j <= r.b.jhi; ++j)
-      for (int i = r.b.ilo; i <= r.b.ihi; ++i) {
-        const int l = r.b.at(i, j);
-        if (hb[l] > bmax) bmax = hb[l];
-        const real u = fabs(hx[l]) * inv_dx + fabs(hy[l]) * inv_dy;
-        if (u > umax) umax = u;
-      }
+    double m[2];
+    X2CHECK(launch_coeff_reduce(r.b3, r.bb, r.vx, r.vy, r.zero, inv_dx, r.part, r.red, r.st));
+    X2CHECK(hipMemcpyAsync(m, r.red, 2 * sizeof(double), hipMemcpyDeviceToHost, r.st));
+    X2CHECK(hipStreamSynchronize(r.st));
+    const real bmax = (real)m[0], umax = (real)m[1];
     o->max_U_over_dx = umax;
     o->b_max_over_dx = 2 * bmax * (inv_dx * inv_dx + inv_dy * inv_dy);
     hipLaunchKernelGGL(varnorm2d_kernel, dim3(blocks2(nf)), dim3(256), 0, r.st, r.b, r.vn, r.mask,
@@ -694,7 +690,11 @@ void model2d(Options* o, QSS_DataArrays* p, Grid* g, FILE* fp, bool drain) {
   if (o->use_var_theta)
     throw std::runtime_error("use_var_theta (theta.gz) is not supported by the 2D models");
   Run2 r(g);
-  up_all(r, p);
+  // drain_model2d / imbibe_model2d read only phi and mask: the trapped masks,
+  // the mask gradient and the coefficient fields are set before first use
+  r.up(r.phi, p->phi);
+  r.up(r.mask, p->mask);
+  const bool var = o->theta > 0;
   real a0 = o->a;
   const real da = o->b * o->dc;
   int step = o->init_step + 1;
@@ -712,7 +712,6 @@ void model2d(Options* o, QSS_DataArrays* p, Grid* g, FILE* fp, bool drain) {
                      r.mask, g->dx[0], g->dx[1]);
   X2CHECK(hipGetLastError());
   // setSpaceDerivFunc and setThetaOverlap in 2D (qss_general_util.c:84-112,196-201)
-  const bool var = o->theta > 0;
   o->overlap = (o->theta <= 20 * (real)(3.14159 / 180)) ? real(0) : 3 * g->dx[0];
   main_ind2(o, g);
   const bool files = !(getenv("LSMPQS_NO_STEP_FILES") && atoi(getenv("LSMPQS_NO_STEP_FILES")));
@@ -737,7 +736,20 @@ void model2d(Options* o, QSS_DataArrays* p, Grid* g, FILE* fp, bool drain) {
     }
     step += 1;
   }
-  down_all(r, p);
+  // the fields the models write (theta = 0 leaves the coefficient fields alone)
+  X2CHECK(hipStreamSynchronize(r.st));
+  r.down(p->phi, r.phi);
+  r.down(p->mask, r.mask);
+  r.down(p->mask_w, r.mask_w);
+  r.down(p->mask_nw, r.mask_nw);
+  r.down(p->mask_x, r.mx);
+  r.down(p->mask_y, r.my);
+  if (var) {
+    r.down(p->normal_velocity, r.vn);
+    r.down(p->curvature_coeff, r.bb);
+    r.down(p->external_velocity_x, r.vx);
+    r.down(p->external_velocity_y, r.vy);
+  }
 }
 
 }  // namespace

@@ -181,3 +181,79 @@ def test_novar_equals_var_with_uniform_fields():
         res.append((t, A["phi"].copy()))
     assert res[0][0] == res[1][0]
     np.testing.assert_array_equal(res[0][1], res[1][1])
+
+
+OPTIONS_2D = """\
+outfile run.log
+accuracy medium
+space_accur 3
+time_accur 3
+theta 30
+a 0.05
+b 0.04
+dc 0.5
+amax 0.0901
+amin 0.0
+tmax 4
+tplot 0.5
+check_connectivity 1
+save_data 1
+print_details 1
+"""
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(REF_SO), reason="reference build not present")
+def test_drain_top_dispatches_2d_grids(tmp_path):
+    """drainTop on 2D files runs drain_model2d (drain_top.c:116-119): the step
+    files and data_final.gz match the reference build's drainTop on the same files."""
+    import gzip
+    import oracle_py as op
+    print("\n[model2d] drainTop on a 40x32 grid", flush=True)
+    d = fixture()
+    hiplib, ours = _hip()
+    ref = op.ref_lib(np.float64)
+    g = mk.grid2d(ours)
+    name = "eno3_rk3_t45"
+    files = {k: str(tmp_path / k) for k in ("data", "mask", "grid", "opts")}
+    for key, f in (("in_phi0", "data"), ("in_mask", "mask")):
+        arr = np.ascontiguousarray(d[f"{name}/{key}"])
+        ours.writeDataArrayQSS(arr.ctypes.data_as(C.c_void_p), C.byref(g), files[f].encode(), 0)
+    ours.writeGridToBinaryFile(C.byref(g), files["grid"].encode(), 0)
+    open(files["opts"], "w").write(OPTIONS_2D)
+    op.set_omp_threads(mk.SLABS)
+    ours.lsmpqs_set_dt_slabs(mk.SLABS)
+    saved = os.environ.pop("LSMPQS_NO_STEP_FILES", None)
+    rcs = {}
+    try:
+        for tag, lib in (("ours", ours), ("ref", ref)):
+            wd = tmp_path / tag
+            os.makedirs(wd)
+            cwd = os.getcwd()
+            os.chdir(wd)
+            try:
+                lib.createOptionsFromInputFile.restype = C.c_void_p
+                lib.createOptionsFromInputFile.argtypes = [C.c_char_p]
+                o = lib.createOptionsFromInputFile(files["opts"].encode())
+                rcs[tag] = lib.drainTop(C.c_void_p(o), files["data"].encode(),
+                                        files["grid"].encode(), files["mask"].encode())
+                lib.destroyOptions.argtypes = [C.c_void_p]
+                lib.destroyOptions(C.c_void_p(o))
+            finally:
+                os.chdir(cwd)
+    finally:
+        if saved is not None:
+            os.environ["LSMPQS_NO_STEP_FILES"] = saved
+    assert rcs["ours"] == 0, hiplib.last_error(ours)
+    assert rcs["ref"] == 0
+    outs = sorted(f for f in os.listdir(tmp_path / "ref") if f.startswith("data_"))
+    assert "data_final.gz" in outs and any(f.startswith("data_step") for f in outs)
+    assert outs == sorted(f for f in os.listdir(tmp_path / "ours") if f.startswith("data_"))
+    for f in outs:
+        a, b = (gzip.open(str(tmp_path / t / f)).read() for t in ("ours", "ref"))
+        assert a[:12] == b[:12], f
+        x, y = np.frombuffer(a[12:], np.float64), np.frombuffer(b[12:], np.float64)
+        assert np.max(np.abs(x - y)) < PHI_TOL, f
+    log_o = open(tmp_path / "ours" / "run.log").read()
+    log_r = open(tmp_path / "ref" / "run.log").read()
+    assert log_o.count("nw phase satn") == log_r.count("nw phase satn") > 0
