@@ -77,6 +77,13 @@ QSSLIB_REAL constCurvModel2dNoVar(Options *options, QSS_DataArrays *p, Grid *g, 
                                   QSSLIB_REAL a);
 /* reference: src/reinitialization/reinitialize_top.h (reinitialize_top.c:123-203) */
 void reinitializeSubcellFix2d(QSSLIB_REAL *data, Grid *g, Options *o);
+/* labels of phi_bin (8-neighbourhood, largest linear index, 0 on background)
+   into p->connectivity (reference: src/connectivity/connectivity.h:22) */
+void findConnectivity2d(QSS_DataArrays *p, Grid *g);
+/* trapComponents_mask, getMainInd_new, setThetaOverlap,
+   initializeDisconnectedMasks and signedLinearExtrapolationBCqss (index 9
+   only; other indices set an error) take 2D grids too, with the reference's
+   2D rules. */
 
 /* ---- trapping (reference: src/connectivity/connectivity.h:23-27) ---- */
 void trapComponents_mask(QSS_DataArrays *p, Grid *g, Options *o);

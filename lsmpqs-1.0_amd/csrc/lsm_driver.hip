@@ -652,6 +652,10 @@ void imbibe_model3d(Options* o, QSS_DataArrays* p, Grid* g, FILE* fp) {
 void trapComponents_mask(QSS_DataArrays* p, Grid* g, Options* o) {
   API_TRY
   lsm::set_error("");
+  if (g->num_dims == 2) {
+    api2d_trap(p, g, o);
+    return;
+  }
   Session* s = session_for(p, g);
   up(s, s->phi, p->phi);
   up(s, s->mask, p->mask);
@@ -721,6 +725,10 @@ void reinitializeMedium3d(QSSLIB_REAL* data, Grid* g, Options* o) {
 void signedLinearExtrapolationBCqss(QSSLIB_REAL* phi, Grid* g, int bdry_location_idx) {
   API_TRY
   lsm::set_error("");
+  if (g->num_dims == 2) {
+    api2d_bc(phi, g, bdry_location_idx);
+    return;
+  }
   if (bdry_location_idx != 9) return;  // the reference's 3D dispatcher handles only 9
   Session* s = session_for((const QSS_DataArrays*)phi, g);
   up(s, s->D, phi);
@@ -734,12 +742,14 @@ void signedLinearExtrapolationBCqss(QSSLIB_REAL* phi, Grid* g, int bdry_location
 
 void getMainInd_new(Options* o, QSS_DataArrays* p, Grid* g) {
   (void)p;
-  main_ind(o, g);
+  if (g->num_dims == 2) api2d_main_ind(o, g);
+  else main_ind(o, g);
 }
 
 void setThetaOverlap(QSS_DataArrays* p, Grid* g, Options* o) {
   (void)p;
   if (g->num_dims == 3) set_theta_overlap(o, g);
+  else if (g->num_dims == 2) api2d_overlap(o, g);
 }
 
 void setSpaceDerivFunc(Grid* g, Options* o) {
@@ -761,6 +771,10 @@ void createReservoirInlet3d(QSS_DataArrays* p, Grid* g) {
 
 void initializeDisconnectedMasks(QSSLIB_REAL* data, Grid* g) {
   API_TRY
+  if (g->num_dims == 2) {
+    api2d_disconn(data, g);
+    return;
+  }
   Session* s = session_for((const QSS_DataArrays*)data, g);
   disconnected_init(s, s->D, g);
   down(s, data, s->D);

@@ -491,12 +491,12 @@ void main_ind2(Options* o, const Grid* g) {
   }
 }
 
-void trap2(Run2& r, const Options* o, const Grid* g) {
+void trap2(Run2& r, const Options* o, const Grid* g, bool full_labels = false) {
   TrapArgs ta{};
   ta.bx = r.b3;
   ta.phi = r.phi; ta.mask = r.mask; ta.mask_nw = r.mask_nw; ta.mask_w = r.mask_w;
   ta.s = r.s; ta.bin = r.bin; ta.labels = r.labels; ta.troot = r.troot;
-  ta.roots = r.roots; ta.roots_cap = r.roots_cap; ta.full_labels = false;
+  ta.roots = r.roots; ta.roots_cap = r.roots_cap; ta.full_labels = full_labels;
   ta.nw_ind = o->phi_nw_ind; ta.w_ind = o->phi_w_ind;
   ta.center_inlet = o->center_inlet;
   ta.fill = -20 * g->dx[0];
@@ -752,7 +752,83 @@ void model2d(Options* o, QSS_DataArrays* p, Grid* g, FILE* fp, bool drain) {
   }
 }
 
+void labels_down(Run2& r, int* conn) {
+  std::vector<int> lab((size_t)r.n);
+  X2CHECK(hipMemcpyAsync(lab.data(), r.labels, (size_t)r.n * sizeof(int), hipMemcpyDeviceToHost,
+                         r.st));
+  X2CHECK(hipStreamSynchronize(r.st));
+  for (int64_t l = 0; l < r.n; ++l) conn[l] = lab[l] >= 0 ? lab[l] : 0;
+}
+
 }  // namespace
+
+// 2D forms of the shared helper entry points (called from their C entry
+// points when num_dims == 2)
+
+// signedLinearExtrapolationBCqss, all four edges (index 9) composed in the
+// reference's order (qss_general_util.c:428-520)
+void api2d_bc(real* phi, const Grid* g, int bdry_location_idx) {
+  check2d(g);
+  if (bdry_location_idx != 9)
+    throw std::runtime_error("2D boundary conditions: only bdry_location_idx 9 (all edges) is built");
+  Run2 r(g);
+  r.up(r.phi, phi);
+  r.bc(r.phi);
+  X2CHECK(hipStreamSynchronize(r.st));
+  r.down(phi, r.phi);
+}
+
+// trapComponents_mask on a 2D grid (trapComponents.c:156-375, 2D branches)
+void api2d_trap(QSS_DataArrays* p, Grid* g, Options* o) {
+  check2d(g);
+  Run2 r(g);
+  r.up(r.phi, p->phi);
+  r.up(r.mask, p->mask);
+  r.up(r.mask_nw, p->mask_nw);
+  trap2(r, o, g, p->connectivity != nullptr);
+  X2CHECK(hipStreamSynchronize(r.st));
+  r.down(p->phi, r.phi);
+  r.down(p->mask_nw, r.mask_nw);
+  r.down(p->mask_w, r.mask_w);
+  if (p->scratch1)
+    for (int64_t l = 0; l < r.n; ++l) p->scratch1[l] = 0;
+  if (p->phi_bin)
+    X2CHECK(hipMemcpy(p->phi_bin, r.bin, (size_t)r.n, hipMemcpyDeviceToHost));
+  if (p->connectivity) labels_down(r, p->connectivity);
+}
+
+// findConnectivity2d: 8-connected labels of phi_bin over the ghost box, each
+// component labelled with its largest linear index, 0 on background
+// (connectivity.c:86-146)
+void api2d_conn(QSS_DataArrays* p, Grid* g) {
+  check2d(g);
+  Run2 r(g);
+  X2CHECK(hipMemcpy(r.bin, p->phi_bin, (size_t)r.n, hipMemcpyHostToDevice));
+  X2CHECK(launch_label_components(r.b3, r.bin, r.labels, r.troot, 26, r.st, r.roots, r.roots_cap,
+                                  true));
+  labels_down(r, p->connectivity);
+}
+
+// initializeDisconnectedMasks on a 2D grid (qss_general_util.c:711-719)
+void api2d_disconn(real* data, const Grid* g) {
+  check2d(g);
+  Run2 r(g);
+  const real cx = (real)(0.5 * (g->x_lo[0] + g->x_hi[0])), cy = (real)(0.5 * (g->x_lo[1] + g->x_hi[1]));
+  const real rad = 2 * (g->x_hi[0] - g->x_lo[0]);
+  hipLaunchKernelGGL(circle2d_kernel, dim3(blocks2(r.n)), dim3(256), 0, r.st, r.b, r.phi,
+                     g->x_lo_ghostbox[0], g->x_lo_ghostbox[1], g->dx[0], g->dx[1], cx, cy, rad);
+  X2CHECK(hipGetLastError());
+  X2CHECK(hipStreamSynchronize(r.st));
+  r.down(data, r.phi);
+}
+
+void api2d_main_ind(Options* o, const Grid* g) { main_ind2(o, g); }
+
+// setThetaOverlap's 2D rule (qss_general_util.c:200-205)
+void api2d_overlap(Options* o, const Grid* g) {
+  o->overlap = (o->theta <= 20 * (real)(3.14159 / 180)) ? real(0) : 3 * g->dx[0];
+}
+
 }  // namespace lsm
 
 using namespace lsm;
@@ -801,6 +877,15 @@ QSSLIB_REAL constCurvModel2d(Options* options, QSS_DataArrays* p, Grid* g, FILE*
 QSSLIB_REAL constCurvModel2dNoVar(Options* options, QSS_DataArrays* p, Grid* g, FILE* fp_out,
                                   QSSLIB_REAL a) {
   return ccm2_entry(options, p, g, fp_out, false, a);
+}
+
+void findConnectivity2d(QSS_DataArrays* p, Grid* g) {
+  set_error("");
+  try {
+    api2d_conn(p, g);
+  } catch (const std::exception& e) {
+    set_error(e.what());
+  }
 }
 
 void reinitializeSubcellFix2d(QSSLIB_REAL* data, Grid* g, Options* o) {

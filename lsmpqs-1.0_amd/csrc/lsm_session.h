@@ -72,6 +72,15 @@ void set_slabs(int s);
 int dt_slabs(const Grid* g);
 int dt_slab_count(int planes);  // the same for a fill box of that many slab planes
 
+// 2D forms of the shared helper entry points (lsm_model2d.hip); they throw
+// std::runtime_error on a device or input error
+void api2d_bc(real* phi, const Grid* g, int bdry_location_idx);
+void api2d_trap(QSS_DataArrays* p, Grid* g, Options* o);
+void api2d_conn(QSS_DataArrays* p, Grid* g);
+void api2d_disconn(real* data, const Grid* g);
+void api2d_main_ind(Options* o, const Grid* g);
+void api2d_overlap(Options* o, const Grid* g);
+
 void ensure_var(Session* s);     // vn, b, vx, vy, vz
 void ensure_grad(Session* s);    // mask gradient and reservoir box
 void ensure_labels(Session* s);  // component labels and binary field

@@ -36,7 +36,7 @@ EXPORTS = (
     "constCurvModel3d", "constCurvModel3dNoVar", "drain_model3d", "imbibe_model3d",
     "trapComponents_mask", "findConnectivity3d", "getMainInd_new", "reinitializeSubcellFix3d",
     "drain_model2d", "imbibe_model2d", "constCurvModel2d", "constCurvModel2dNoVar",
-    "reinitializeSubcellFix2d",
+    "reinitializeSubcellFix2d", "findConnectivity2d",
     "signedLinearExtrapolationBCqss", "initializeDisconnectedMasks", "createReservoirInlet3d",
     "setSpaceDerivFunc", "setThetaOverlap", "createOptionsDefault", "destroyOptions",
     "writeDataArrayQSS", "readDataArrayQSS", "allocateQSSDataArrays",

@@ -9,8 +9,9 @@ on the drained result) with the reference's OpenMP team fixed at SLABS
 threads, in a scratch directory (the models write data_step_N files there).
 The .npz holds, per case, the inputs (mask, phi0), the option values used,
 every output field the models write, the number of data_step files and the
-grid record createGridSetDx(2, ...) returns, plus one reinitializeSubcellFix2d
-input/output pair.  Data only.
+grid record createGridSetDx(2, ...) returns, one reinitializeSubcellFix2d
+input/output pair and the outputs of the shared helper entry points on the
+2D grid (run_ref_helpers).  Data only.
 """
 import ctypes as C
 import os
@@ -164,8 +165,61 @@ def run_ref_reinit(seed=77, tmax_r=2.0):
     return {"reinit/in": f, "reinit/out": data.copy(), "reinit/tmax_r": np.array(tmax_r)}
 
 
+def blob_field(g, shift=0.1):
+    """A field whose sign pattern has many components of both phases."""
+    nx, ny = g.grid_dims_ghostbox[0], g.grid_dims_ghostbox[1]
+    X, Y = np.meshgrid(np.arange(nx, dtype=DT), np.arange(ny, dtype=DT))
+    return 0.8 * np.sin(0.45 * X) * np.cos(0.38 * Y) - shift
+
+
+def run_ref_helpers(seed=2024):
+    """The shared helper entry points on a 2D grid: getMainInd_new +
+    trapComponents_mask (side and centre inlet), findConnectivity2d,
+    signedLinearExtrapolationBCqss (index 9), initializeDisconnectedMasks and
+    setThetaOverlap."""
+    lib = op.ref_lib(DT)
+    op.set_omp_threads(SLABS)
+    g = grid2d(lib)
+    mask, _ = disk_pack(g, seed, 0)
+    blobs = blob_field(g)
+    out = {"helpers/mask": mask, "helpers/phi": blobs}
+    for center in (0, 1):
+        A = Arrays(g, DT)
+        A["mask"] = mask.reshape(A.shape)
+        # non-wetting islands (trapped unless they touch the inlet) in a wetting sea
+        A["phi"] = np.maximum(0.35 - (blobs + 0.1), mask).reshape(A.shape)
+        A["mask_nw"] = -20.0
+        o = default_options(DT)
+        o.center_inlet = center
+        lib.getMainInd_new(C.byref(o), A.ptr(), C.byref(g))
+        lib.trapComponents_mask(A.ptr(), C.byref(g), C.byref(o))
+        key = f"helpers/trap{center}"
+        for f in ("phi", "mask_nw", "mask_w", "connectivity", "phi_bin"):
+            out[f"{key}/{f}"] = A[f].reshape(mask.shape).copy()
+        out[f"{key}/phi_nw_ind"] = np.array(o.phi_nw_ind)
+        out[f"{key}/phi_w_ind"] = np.array(o.phi_w_ind)
+    A = Arrays(g, DT)
+    A["phi_bin"] = (blobs + 0.1 > 0.35).astype(np.uint8).reshape(A.shape)
+    lib.findConnectivity2d(A.ptr(), C.byref(g))
+    out["helpers/conn"] = A["connectivity"].reshape(mask.shape).copy()
+    f = np.ascontiguousarray(blobs + 0.01 * np.random.default_rng(seed).normal(size=blobs.shape))
+    out["helpers/bc_in"] = f.copy()
+    lib.signedLinearExtrapolationBCqss(f.ctypes.data_as(C.c_void_p), C.byref(g), 9)
+    out["helpers/bc_out"] = f
+    f = np.zeros_like(blobs)
+    lib.initializeDisconnectedMasks(f.ctypes.data_as(C.c_void_p), C.byref(g))
+    out["helpers/disconn"] = f
+    for deg in (10.0, 30.0):
+        o = default_options(DT)
+        o.theta = deg * 3.14159 / 180
+        lib.setThetaOverlap(None, C.byref(g), C.byref(o))
+        out[f"helpers/overlap_{int(deg)}"] = np.array(o.overlap)
+    return out
+
+
 def main():
     data = run_ref_reinit()
+    data.update(run_ref_helpers())
     for name in CASES:
         out, g = run_ref(name)
         data.update(out)

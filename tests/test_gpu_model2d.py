@@ -257,3 +257,50 @@ def test_drain_top_dispatches_2d_grids(tmp_path):
     log_o = open(tmp_path / "ours" / "run.log").read()
     log_r = open(tmp_path / "ref" / "run.log").read()
     assert log_o.count("nw phase satn") == log_r.count("nw phase satn") > 0
+
+
+@pytest.mark.gpu
+def test_helpers_on_2d_grids_match_reference():
+    """getMainInd_new + trapComponents_mask (side and centre inlet),
+    findConnectivity2d, signedLinearExtrapolationBCqss, initializeDisconnectedMasks
+    and setThetaOverlap on a 2D grid, bit for bit against the reference build's
+    outputs (integer labels and IEEE add/multiply/sqrt arithmetic only)."""
+    print("\n[model2d] helper entry points on a 2D grid", flush=True)
+    d = fixture()
+    hiplib, lib = _hip()
+    g = mk.grid2d(lib)
+    mask, blobs = d["helpers/mask"], d["helpers/phi"]
+    for center in (0, 1):
+        print(f"[model2d] trapComponents_mask center_inlet={center}", flush=True)
+        A = Arrays(g, np.float64)
+        A["mask"] = mask.reshape(A.shape)
+        A["phi"] = np.maximum(0.35 - (blobs + 0.1), mask).reshape(A.shape)
+        A["mask_nw"] = -20.0
+        o = default_options(np.float64)
+        o.center_inlet = center
+        lib.getMainInd_new(C.byref(o), A.ptr(), C.byref(g))
+        key = f"helpers/trap{center}"
+        assert o.phi_nw_ind == int(d[f"{key}/phi_nw_ind"])
+        assert o.phi_w_ind == int(d[f"{key}/phi_w_ind"])
+        lib.trapComponents_mask(A.ptr(), C.byref(g), C.byref(o))
+        hiplib.check(lib)
+        for f in ("phi", "mask_nw", "mask_w", "connectivity", "phi_bin"):
+            np.testing.assert_array_equal(A[f].reshape(mask.shape), d[f"{key}/{f}"], err_msg=f)
+    A = Arrays(g, np.float64)
+    A["phi_bin"] = (blobs + 0.1 > 0.35).astype(np.uint8).reshape(A.shape)
+    lib.findConnectivity2d(A.ptr(), C.byref(g))
+    hiplib.check(lib)
+    np.testing.assert_array_equal(A["connectivity"].reshape(mask.shape), d["helpers/conn"])
+    f = np.ascontiguousarray(d["helpers/bc_in"].copy())
+    lib.signedLinearExtrapolationBCqss(f.ctypes.data_as(C.c_void_p), C.byref(g), 9)
+    hiplib.check(lib)
+    np.testing.assert_array_equal(f, d["helpers/bc_out"])
+    f = np.zeros_like(blobs)
+    lib.initializeDisconnectedMasks(f.ctypes.data_as(C.c_void_p), C.byref(g))
+    hiplib.check(lib)
+    np.testing.assert_array_equal(f, d["helpers/disconn"])
+    for deg in (10, 30):
+        o = default_options(np.float64)
+        o.theta = deg * 3.14159 / 180
+        lib.setThetaOverlap(None, C.byref(g), C.byref(o))
+        assert o.overlap == float(d[f"helpers/overlap_{deg}"])
