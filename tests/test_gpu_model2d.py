@@ -304,3 +304,48 @@ def test_helpers_on_2d_grids_match_reference():
         o.theta = deg * 3.14159 / 180
         lib.setThetaOverlap(None, C.byref(g), C.byref(o))
         assert o.overlap == float(d[f"helpers/overlap_{deg}"])
+
+
+@pytest.mark.gpu
+def test_model2d_fp32_tracks_fp64_reference():
+    """The FP32 library on the ENO3/RK3 45-degree case against the FP64
+    reference vectors: same curvature steps, saturation within 2e-3, the
+    interface (sign of phi) equal on at least 99.5 % of the fill box."""
+    print("\n[model2d] FP32 eno3_rk3_t45", flush=True)
+    from lsmpqs_amd import hiplib
+    d = fixture()
+    name = "eno3_rk3_t45"
+    space, time, theta_deg, center, _ = mk.CASES[name]
+    lib = hiplib.load(np.float32)
+    lib.lsmpqs_set_dt_slabs(mk.SLABS)
+    g = mk.grid2d(lib, np.float32)
+    A = Arrays(g, np.float32)
+    A["mask"] = d[f"{name}/in_mask"].reshape(A.shape)
+    A["phi"] = d[f"{name}/in_phi0"].reshape(A.shape)
+    import make_model2d
+    o = default_options(np.float32)
+    for f in ("order_space_accur", "order_time_accur", "theta", "a", "b", "dc", "amax", "amin",
+              "tmax", "center_inlet"):
+        setattr(o, f, getattr(make_model2d.case_options(space, time, theta_deg, center), f))
+    saved = os.environ.get("LSMPQS_NO_STEP_FILES")
+    os.environ["LSMPQS_NO_STEP_FILES"] = "1"
+    try:
+        lib.drain_model2d(C.byref(o), A.ptr(), C.byref(g), DEVNULL)
+    finally:
+        if saved is None:
+            del os.environ["LSMPQS_NO_STEP_FILES"]
+        else:
+            os.environ["LSMPQS_NO_STEP_FILES"] = saved
+    hiplib.check(lib)
+    shape = d[f"{name}/in_mask"].shape
+    fb = fill2d(g)
+    phi, ref, mask = A["phi"].reshape(shape).astype(np.float64), d[f"{name}/phi"], d[f"{name}/in_mask"]
+
+    def satn(f):
+        q = f[fb] / 1.5
+        h = np.where(f[fb] < -1.5, 1.0, np.where(f[fb] < 1.5, 0.5 * (1 - q - np.sin(np.pi * q) / np.pi), 0.0))
+        return h.sum()
+    s32, s64 = satn(phi) / satn(mask), satn(ref) / satn(mask)
+    assert abs(s32 - s64) < 2e-3, (s32, s64)
+    agree = np.mean((phi[fb] < 0) == (ref[fb] < 0))
+    assert agree >= 0.995, agree
