@@ -85,6 +85,79 @@ void findConnectivity2d(QSS_DataArrays *p, Grid *g);
    only; other indices set an error) take 2D grids too, with the reference's
    2D rules. */
 
+/* ---- kernel level, Fortran calling convention (every argument by reference;
+ *      arrays over ghost-box bounds gb = ilo_gb..khi_gb with a fill box fb).
+ *      Reference symbols of the same names: src/spatial_derivatives/
+ *      qss_spatial_derivatives3d.f (getRHS3d2ndConst :302, getRHS3d3rdConst :572,
+ *      getRHS3d2nd :1020, getRHS3d3rd :1335, getRHS3d5th :1827,
+ *      setVarNorm3d :13, setVarCurvAdv3d :178, getCentralGrad3dOrder2 :2987,
+ *      signedLinearExtrapolation3d :3055), src/time_integration/
+ *      qss_tvd_runge_kutta3d.f (:2, :61, :131), src/utilities/qss_util3d.f
+ *      (qss3dMaxNormDiffLocal :65, imposeMaskParallel :130,
+ *      qss3dVolumeRegionPhiLessThanZero :288), src/reinitialization/
+ *      qss_reinitialization3d.f (:476, :585).  Each call stages its host
+ *      arrays on the GPU and back; arrays of one call must share the ghost-box
+ *      bounds.  Errors set lsmpqs_hip_last_error(). ---- */
+#define LSMPQS_GB6 const int *, const int *, const int *, const int *, const int *, const int *
+void getrhs3d2ndconst_(const QSSLIB_REAL *phi, const QSSLIB_REAL *vel_n, const QSSLIB_REAL *b,
+                       QSSLIB_REAL *max_H_over_dX, QSSLIB_REAL *lse_rhs, LSMPQS_GB6, LSMPQS_GB6,
+                       const QSSLIB_REAL *dx, const QSSLIB_REAL *dy, const QSSLIB_REAL *dz);
+void getrhs3d3rdconst_(const QSSLIB_REAL *phi, const QSSLIB_REAL *vel_n, const QSSLIB_REAL *b,
+                       QSSLIB_REAL *max_H_over_dX, QSSLIB_REAL *lse_rhs, LSMPQS_GB6, LSMPQS_GB6,
+                       const QSSLIB_REAL *dx, const QSSLIB_REAL *dy, const QSSLIB_REAL *dz);
+void getrhs3d2nd_(const QSSLIB_REAL *phi, const QSSLIB_REAL *vel_n, const QSSLIB_REAL *b,
+                  const QSSLIB_REAL *vel_x, const QSSLIB_REAL *vel_y, const QSSLIB_REAL *vel_z,
+                  QSSLIB_REAL *max_H_over_dX, QSSLIB_REAL *lse_rhs, LSMPQS_GB6, LSMPQS_GB6,
+                  const QSSLIB_REAL *dx, const QSSLIB_REAL *dy, const QSSLIB_REAL *dz);
+void getrhs3d3rd_(const QSSLIB_REAL *phi, const QSSLIB_REAL *vel_n, const QSSLIB_REAL *b,
+                  const QSSLIB_REAL *vel_x, const QSSLIB_REAL *vel_y, const QSSLIB_REAL *vel_z,
+                  QSSLIB_REAL *max_H_over_dX, QSSLIB_REAL *lse_rhs, LSMPQS_GB6, LSMPQS_GB6,
+                  const QSSLIB_REAL *dx, const QSSLIB_REAL *dy, const QSSLIB_REAL *dz);
+void getrhs3d5th_(const QSSLIB_REAL *phi, const QSSLIB_REAL *vel_n, const QSSLIB_REAL *b,
+                  const QSSLIB_REAL *vel_x, const QSSLIB_REAL *vel_y, const QSSLIB_REAL *vel_z,
+                  QSSLIB_REAL *max_H_over_dX, QSSLIB_REAL *lse_rhs, LSMPQS_GB6, LSMPQS_GB6,
+                  const QSSLIB_REAL *dx, const QSSLIB_REAL *dy, const QSSLIB_REAL *dz);
+void qss3drk1step_(QSSLIB_REAL *u_next, LSMPQS_GB6, const QSSLIB_REAL *u_cur, LSMPQS_GB6,
+                   const QSSLIB_REAL *rhs, LSMPQS_GB6, LSMPQS_GB6, const QSSLIB_REAL *dt);
+void qss3dtvdrk2stage2_(QSSLIB_REAL *u_stage2, LSMPQS_GB6, const QSSLIB_REAL *u_stage1,
+                        LSMPQS_GB6, const QSSLIB_REAL *u_cur, LSMPQS_GB6,
+                        const QSSLIB_REAL *rhs, LSMPQS_GB6, LSMPQS_GB6, const QSSLIB_REAL *dt);
+void qss3dtvdrk3stage3_(QSSLIB_REAL *u_next, LSMPQS_GB6, const QSSLIB_REAL *u_stage2,
+                        LSMPQS_GB6, const QSSLIB_REAL *u_cur, LSMPQS_GB6,
+                        const QSSLIB_REAL *rhs, LSMPQS_GB6, LSMPQS_GB6, const QSSLIB_REAL *dt);
+/* one face (0..5 = x-, x+, y-, y+, z-, z+) over the whole ghost range of the other axes */
+void signedlinearextrapolation3d_(QSSLIB_REAL *phi, LSMPQS_GB6, LSMPQS_GB6,
+                                  const int *bdry_location_idx);
+void imposemaskparallel_(QSSLIB_REAL *phi_masked, const QSSLIB_REAL *mask,
+                         const QSSLIB_REAL *phi, const QSSLIB_REAL *overlap, LSMPQS_GB6,
+                         const int *cur_klo, const int *cur_khi);
+/* FP64 accumulation in a fixed tree order (the reference accumulates in
+   QSSLIB_REAL in loop order) */
+void qss3dvolumeregionphilessthanzero_(QSSLIB_REAL *volume, const QSSLIB_REAL *phi, LSMPQS_GB6,
+                                       LSMPQS_GB6, const QSSLIB_REAL *dx, const QSSLIB_REAL *dy,
+                                       const QSSLIB_REAL *dz, const QSSLIB_REAL *epsilon);
+void qss3dmaxnormdifflocal_(QSSLIB_REAL *max_norm, const QSSLIB_REAL *field1, LSMPQS_GB6,
+                            const QSSLIB_REAL *field2, LSMPQS_GB6, LSMPQS_GB6,
+                            const QSSLIB_REAL *zone);
+void getcentralgrad3dorder2_(QSSLIB_REAL *gx, QSSLIB_REAL *gy, QSSLIB_REAL *gz, LSMPQS_GB6,
+                             const QSSLIB_REAL *phi, LSMPQS_GB6, LSMPQS_GB6,
+                             const QSSLIB_REAL *dx, const QSSLIB_REAL *dy, const QSSLIB_REAL *dz);
+void setvarnorm3d_(const QSSLIB_REAL *mask, const QSSLIB_REAL *mask_x, const QSSLIB_REAL *mask_y,
+                   const QSSLIB_REAL *mask_z, QSSLIB_REAL *var_a, const QSSLIB_REAL *a0,
+                   const QSSLIB_REAL *ca, LSMPQS_GB6, LSMPQS_GB6, const QSSLIB_REAL *dx);
+void setvarcurvadv3d_(const QSSLIB_REAL *mask, const QSSLIB_REAL *mask_x,
+                      const QSSLIB_REAL *mask_y, const QSSLIB_REAL *mask_z, QSSLIB_REAL *var_b,
+                      QSSLIB_REAL *vel_x, QSSLIB_REAL *vel_y, QSSLIB_REAL *vel_z,
+                      QSSLIB_REAL *b_max_over_dx, QSSLIB_REAL *max_U_over_dx,
+                      const QSSLIB_REAL *const_b, LSMPQS_GB6, LSMPQS_GB6, const QSSLIB_REAL *dx);
+void qss3dcomputedistanceforsubcellfix_(QSSLIB_REAL *distance0, const QSSLIB_REAL *phi0,
+                                        LSMPQS_GB6, LSMPQS_GB6, const QSSLIB_REAL *dx,
+                                        const QSSLIB_REAL *dy, const QSSLIB_REAL *dz);
+void qss3dcomputereinitializationeqnrhssubcellfixorder1_(
+    QSSLIB_REAL *reinit_rhs, const QSSLIB_REAL *phi, LSMPQS_GB6, const QSSLIB_REAL *phi0,
+    const QSSLIB_REAL *distance0, LSMPQS_GB6, const QSSLIB_REAL *dx, const QSSLIB_REAL *dy,
+    const QSSLIB_REAL *dz);
+
 /* ---- trapping (reference: src/connectivity/connectivity.h:23-27) ---- */
 void trapComponents_mask(QSS_DataArrays *p, Grid *g, Options *o);
 /* labels of phi_bin (26-neighbourhood, each component labelled with its

@@ -168,6 +168,9 @@ hipError_t launch_reinit_distance(const Box& bx, real* dist0, const real* phi0, 
 hipError_t launch_reinit_step(const Box& bx, real* next, const real* data, const real* phi0,
                               const real* dist0, real dt_r, real dx, real dy, real dz,
                               hipStream_t st);
+// the subcell-fix right-hand side alone, fill box
+hipError_t launch_reinit_rhs(const Box& bx, real* rhs, const real* data, const real* phi0,
+                             const real* dist0, real dx, real dy, real dz, hipStream_t st);
 // the first step with data = phi0, D0 formed on the fly (no phi0 copy, no D0 field)
 hipError_t launch_reinit_first_step(const Box& bx, real* next, const real* phi0, real dt_r,
                                     real dx, real dy, real dz, hipStream_t st);

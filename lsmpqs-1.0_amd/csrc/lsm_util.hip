@@ -317,6 +317,38 @@ hipError_t launch_reinit_step(const Box& bx, real* next, const real* data, const
   return hipGetLastError();
 }
 
+// the right-hand side alone (qss3dComputeReinitializationEqnRHSSubcellFixOrder1)
+__global__ void reinit_rhs_kernel(Box bx, real* rhs, const real* data, const real* phi0,
+                                  const real* dist0, real dx, real dy, real dz) {
+  FillIter it(bx);
+  real max_dx = tmax(tmax(dx, dy), dz);
+  real flag = real(-1000.0) * max_dx;
+  real inv_dx = real(1.0) / dx, inv_dy = real(1.0) / dy, inv_dz = real(1.0) / dz;
+  int64_t sy = bx.nx, sz = (int64_t)bx.nx * bx.ny;
+  for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < it.n;
+       f += (int64_t)gridDim.x * blockDim.x) {
+    int i, j, k;
+    it.ijk(f, i, j, k);
+    int64_t idx = bx.at(i, j, k);
+    real c = data[idx];
+    real gp[3], gm[3];
+    gp[0] = (data[idx + 1] - c) * inv_dx;
+    gm[0] = (c - data[idx - 1]) * inv_dx;
+    gp[1] = (data[idx + sy] - c) * inv_dy;
+    gm[1] = (c - data[idx - sy]) * inv_dy;
+    gp[2] = (data[idx + sz] - c) * inv_dz;
+    gm[2] = (c - data[idx - sz]) * inv_dz;
+    rhs[idx] = reinit_rhs(c, gp, gm, phi0[idx], dist0[idx], flag, max_dx);
+  }
+}
+hipError_t launch_reinit_rhs(const Box& bx, real* rhs, const real* data, const real* phi0,
+                             const real* dist0, real dx, real dy, real dz, hipStream_t st) {
+  FillIter it(bx);
+  hipLaunchKernelGGL(reinit_rhs_kernel, dim3(stream_blocks(it.n)), dim3(256), 0, st, bx, rhs,
+                     data, phi0, dist0, dx, dy, dz);
+  return hipGetLastError();
+}
+
 // The first step from phi0 itself (data = phi0): D0 and the step from the
 // same seven values, with no copy of phi0 and no D0 field.  Bit for bit the
 // distance kernel followed by the step kernel on that first iteration.

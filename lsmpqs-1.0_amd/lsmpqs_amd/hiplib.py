@@ -37,6 +37,11 @@ EXPORTS = (
     "trapComponents_mask", "findConnectivity3d", "getMainInd_new", "reinitializeSubcellFix3d",
     "drain_model2d", "imbibe_model2d", "constCurvModel2d", "constCurvModel2dNoVar",
     "reinitializeSubcellFix2d", "findConnectivity2d",
+    "getrhs3d2ndconst_", "getrhs3d3rdconst_", "getrhs3d2nd_", "getrhs3d3rd_", "getrhs3d5th_",
+    "qss3drk1step_", "qss3dtvdrk2stage2_", "qss3dtvdrk3stage3_", "signedlinearextrapolation3d_",
+    "imposemaskparallel_", "qss3dvolumeregionphilessthanzero_", "qss3dmaxnormdifflocal_",
+    "getcentralgrad3dorder2_", "setvarnorm3d_", "setvarcurvadv3d_",
+    "qss3dcomputedistanceforsubcellfix_", "qss3dcomputereinitializationeqnrhssubcellfixorder1_",
     "signedLinearExtrapolationBCqss", "initializeDisconnectedMasks", "createReservoirInlet3d",
     "setSpaceDerivFunc", "setThetaOverlap", "createOptionsDefault", "destroyOptions",
     "writeDataArrayQSS", "readDataArrayQSS", "allocateQSSDataArrays",
