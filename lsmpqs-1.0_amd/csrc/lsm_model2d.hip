@@ -860,7 +860,7 @@ static QSSLIB_REAL ccm2_entry(Options* o, QSS_DataArrays* p, Grid* g, FILE* fp, 
     check2d(g);
     Run2 r(g);
     up_all(r, p);
-    if (o->check_connectivity) main_ind2(o, g);
+    // phi_nw_ind / phi_w_ind as the caller set them (getMainInd_new), as in the reference
     QSSLIB_REAL t = ccm2(r, o, g, fp, var, a);
     down_all(r, p);
     return t;

@@ -349,3 +349,47 @@ def test_model2d_fp32_tracks_fp64_reference():
     assert abs(s32 - s64) < 2e-3, (s32, s64)
     agree = np.mean((phi[fb] < 0) == (ref[fb] < 0))
     assert agree >= 0.995, agree
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(REF_SO), reason="reference build not present")
+def test_const_curv_model2d_direct_call_matches_reference_build():
+    """constCurvModel2d called directly (trapping on, indices from
+    getMainInd_new, coefficient fields supplied by the caller) against the
+    reference build's constCurvModel2d on the same records."""
+    import oracle_py as op
+    print("\n[model2d] constCurvModel2d direct call", flush=True)
+    d = fixture()
+    name = "eno3_rk3_t45"
+    hiplib, ours = _hip()
+    ref = op.ref_lib(np.float64)
+    op.set_omp_threads(mk.SLABS)
+    ours.lsmpqs_set_dt_slabs(mk.SLABS)
+    ref.constCurvModel2d.restype = C.c_double
+    out = {}
+    for tag, lib in (("ours", ours), ("ref", ref)):
+        g = mk.grid2d(lib)
+        A = Arrays(g, np.float64)
+        for f, key in (("mask", "mask"), ("phi", "in_phi0"), ("normal_velocity", "normal_velocity"),
+                       ("curvature_coeff", "curvature_coeff"),
+                       ("external_velocity_x", "external_velocity_x"),
+                       ("external_velocity_y", "external_velocity_y")):
+            A[f] = d[f"{name}/{key}"].reshape(A.shape)
+        A["mask_w"] = -20.0
+        A["mask_nw"] = -20.0
+        o = mk.case_options(3, 3, 45.0, 0)
+        o.tmax = 2.0
+        o.overlap = float(d[f"{name}/opt_overlap"])
+        o.b_max_over_dx = float(d[f"{name}/opt_b_max_over_dx"])
+        o.max_U_over_dx = float(d[f"{name}/opt_max_U_over_dx"])
+        lib.getMainInd_new(C.byref(o), A.ptr(), C.byref(g))
+        # the reference calls options->space_deriv_func, which this sets
+        # (a NULL slot crashes it); this library chooses its kernels itself
+        lib.setSpaceDerivFunc(C.byref(g), C.byref(o))
+        t = lib.constCurvModel2d(C.byref(o), A.ptr(), C.byref(g), DEVNULL)
+        out[tag] = (float(t), A["phi"].copy(), A["mask_w"].copy(), A["mask_nw"].copy())
+    hiplib.check(ours)
+    assert out["ours"][0] == pytest.approx(out["ref"][0], rel=1e-12, abs=1e-12)
+    for i, f in ((1, "phi"), (2, "mask_w"), (3, "mask_nw")):
+        e = float(np.max(np.abs(out["ours"][i] - out["ref"][i])))
+        assert e < PHI_TOL, f"{f} max diff {e:.3e}"
