@@ -47,7 +47,11 @@ CASES = {
     "eno3_rk3_t45": (3, 3, 45.0, 0, False),
     "eno2_rk3_t30_center": (2, 3, 30.0, 1, False),
     "eno3_rk2_t30_imbibe": (3, 2, 30.0, 0, True),
+    "eno3_rk3_t45_dxdy": (3, 3, 45.0, 0, False),
 }
+# cases on createGridSetDxDyDz grids (dx, dy): with dx != dy the reference's
+# 2D quirks show (the ENO3 y-minus side and the U bound scaled by 1/dx)
+DXDY = {"eno3_rk3_t45_dxdy": (0.5, 0.75)}
 
 
 def grid2d(lib, dtype=DT, nx=NX, ny=NY):
@@ -59,6 +63,21 @@ def grid2d(lib, dtype=DT, nx=NX, ny=NY):
     lo = (R * 3)(0.0, 0.0, 0.0)
     hi = (R * 3)(float(nx), float(ny), 0.0)
     return lib.createGridSetDx(2, 1.0, lo, hi, 1).contents
+
+
+def grid_for(name, lib, dtype=DT):
+    """The case's grid from the given library: createGridSetDx(2, 1, ...) on
+    40 x 32 cells, or createGridSetDxDyDz(2, dx, dy, 1, ...) on 40 x 32 cells."""
+    if name not in DXDY:
+        return grid2d(lib, dtype)
+    Grid, _, _ = types_for(dtype)
+    R = C.c_double if np.dtype(dtype) == np.float64 else C.c_float
+    dx, dy = DXDY[name]
+    lib.createGridSetDxDyDz.restype = C.POINTER(Grid)
+    lib.createGridSetDxDyDz.argtypes = [C.c_int, R, R, R, C.POINTER(R), C.POINTER(R), C.c_int]
+    lo = (R * 3)(0.0, 0.0, 0.0)
+    hi = (R * 3)(NX * dx, NY * dy, 0.0)
+    return lib.createGridSetDxDyDz(2, dx, dy, 1.0, lo, hi, 1).contents
 
 
 def grid_record(g):
@@ -120,7 +139,7 @@ def run_ref(name, seed=2024):
     space, time, theta_deg, center, imbibe = CASES[name]
     lib = op.ref_lib(DT)
     op.set_omp_threads(SLABS)
-    g = grid2d(lib)
+    g = grid_for(name, lib)
     mask, phi0 = disk_pack(g, seed, center)
     A = Arrays(g, DT)
     A["mask"] = mask.reshape(A.shape)

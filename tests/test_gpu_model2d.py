@@ -75,7 +75,7 @@ def _run_hip(name, d):
     space, time, theta_deg, center, imbibe = mk.CASES[name]
     hiplib, lib = _hip()
     lib.lsmpqs_set_dt_slabs(mk.SLABS)
-    g = mk.grid2d(lib)
+    g = mk.grid_for(name, lib)
     A = Arrays(g, np.float64)
     A["mask"] = d[f"{name}/in_mask"].reshape(A.shape)
     A["phi"] = d[f"{name}/in_phi0"].reshape(A.shape)
