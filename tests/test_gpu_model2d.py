@@ -393,3 +393,28 @@ def test_const_curv_model2d_direct_call_matches_reference_build():
     for i, f in ((1, "phi"), (2, "mask_w"), (3, "mask_nw")):
         e = float(np.max(np.abs(out["ours"][i] - out["ref"][i])))
         assert e < PHI_TOL, f"{f} max diff {e:.3e}"
+
+
+def test_model2d_input_errors_need_no_gpu():
+    """Input errors of the 2D entry points are reported before any device
+    call: a 3D grid, use_var_theta, a single-edge boundary index."""
+    from lsmpqs_amd import hiplib
+    from lsmpqs_amd.abi import make_grid
+    path = hiplib.lib_path(np.float64)
+    if not os.path.exists(path):
+        pytest.skip("library not built")
+    lib = hiplib.load(np.float64)
+    g3 = make_grid((8, 8, 8), np.float64)
+    A3 = Arrays(g3, np.float64)
+    o = default_options(np.float64)
+    o.theta = 0.5
+    lib.drain_model2d(C.byref(o), A3.ptr(), C.byref(g3), DEVNULL)
+    assert "2D grid" in hiplib.last_error(lib)
+    g2 = mk.grid2d(lib)
+    A2 = Arrays(g2, np.float64)
+    o.use_var_theta = 1
+    lib.imbibe_model2d(C.byref(o), A2.ptr(), C.byref(g2), DEVNULL)
+    assert "use_var_theta" in hiplib.last_error(lib)
+    f = np.zeros(A2.shape)
+    lib.signedLinearExtrapolationBCqss(f.ctypes.data_as(C.c_void_p), C.byref(g2), 0)
+    assert "bdry_location_idx 9" in hiplib.last_error(lib)
