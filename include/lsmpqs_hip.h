@@ -158,6 +158,39 @@ void qss3dcomputereinitializationeqnrhssubcellfixorder1_(
     const QSSLIB_REAL *distance0, LSMPQS_GB6, const QSSLIB_REAL *dx, const QSSLIB_REAL *dy,
     const QSSLIB_REAL *dz);
 
+/* ---- kernel level, 2D (Fortran calling convention, 4-int ghost-box and
+ *      fill-box bounds; reference: src/spatial_derivatives/
+ *      qss_spatial_derivatives2d.f qss2dGetRHS2nd / 2ndconst / 3rd / 3rdconst
+ *      and qss2dSignedLinearExtrapolation, src/time_integration/
+ *      qss_tvd_runge_kutta2d.f).  The constant forms add no advective term
+ *      where the reference adds an unassigned one. ---- */
+#define LSMPQS_GB4 const int *, const int *, const int *, const int *
+void qss2dgetrhs2nd_(const QSSLIB_REAL *phi, const QSSLIB_REAL *vel_n, const QSSLIB_REAL *b,
+                     const QSSLIB_REAL *vel_x, const QSSLIB_REAL *vel_y,
+                     QSSLIB_REAL *max_H_over_dX, QSSLIB_REAL *lse_rhs, LSMPQS_GB4, LSMPQS_GB4,
+                     const QSSLIB_REAL *dx, const QSSLIB_REAL *dy);
+void qss2dgetrhs3rd_(const QSSLIB_REAL *phi, const QSSLIB_REAL *vel_n, const QSSLIB_REAL *b,
+                     const QSSLIB_REAL *vel_x, const QSSLIB_REAL *vel_y,
+                     QSSLIB_REAL *max_H_over_dX, QSSLIB_REAL *lse_rhs, LSMPQS_GB4, LSMPQS_GB4,
+                     const QSSLIB_REAL *dx, const QSSLIB_REAL *dy);
+void qss2dgetrhs2ndconst_(const QSSLIB_REAL *phi, const QSSLIB_REAL *vel_n, const QSSLIB_REAL *b,
+                          QSSLIB_REAL *max_H_over_dX, QSSLIB_REAL *lse_rhs, LSMPQS_GB4,
+                          LSMPQS_GB4, const QSSLIB_REAL *dx, const QSSLIB_REAL *dy);
+void qss2dgetrhs3rdconst_(const QSSLIB_REAL *phi, const QSSLIB_REAL *vel_n, const QSSLIB_REAL *b,
+                          QSSLIB_REAL *max_H_over_dX, QSSLIB_REAL *lse_rhs, LSMPQS_GB4,
+                          LSMPQS_GB4, const QSSLIB_REAL *dx, const QSSLIB_REAL *dy);
+void qss2drk1step_(QSSLIB_REAL *u_next, LSMPQS_GB4, const QSSLIB_REAL *u_cur, LSMPQS_GB4,
+                   const QSSLIB_REAL *rhs, LSMPQS_GB4, LSMPQS_GB4, const QSSLIB_REAL *dt);
+void qss2dtvdrk2stage2_(QSSLIB_REAL *u_stage2, LSMPQS_GB4, const QSSLIB_REAL *u_stage1,
+                        LSMPQS_GB4, const QSSLIB_REAL *u_cur, LSMPQS_GB4,
+                        const QSSLIB_REAL *rhs, LSMPQS_GB4, LSMPQS_GB4, const QSSLIB_REAL *dt);
+void qss2dtvdrk3stage3_(QSSLIB_REAL *u_next, LSMPQS_GB4, const QSSLIB_REAL *u_stage2,
+                        LSMPQS_GB4, const QSSLIB_REAL *u_cur, LSMPQS_GB4,
+                        const QSSLIB_REAL *rhs, LSMPQS_GB4, LSMPQS_GB4, const QSSLIB_REAL *dt);
+/* one edge (0..3 = x-, x+, y-, y+) over the whole ghost range of the other axis */
+void qss2dsignedlinearextrapolation_(QSSLIB_REAL *phi, LSMPQS_GB4, LSMPQS_GB4,
+                                     const int *bdry_location_idx);
+
 /* ---- trapping (reference: src/connectivity/connectivity.h:23-27) ---- */
 void trapComponents_mask(QSS_DataArrays *p, Grid *g, Options *o);
 /* labels of phi_bin (26-neighbourhood, each component labelled with its

@@ -160,7 +160,7 @@ __global__ void rhs2d_kernel(Rhs2Args a) {
   const int n = b.nfill();
   const real inv_dx = real(1.0) / a.dx, inv_dy = real(1.0) / a.dy;
   const real fx = real(0.5) / a.dx, fy = real(0.5) / a.dy;
-  const real dt = *a.dt;
+  const real dt = STAGE ? *a.dt : real(0);
   for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < n; f += gridDim.x * blockDim.x) {
     int i, j;
     fill_ij(b, f, i, j);
@@ -193,7 +193,8 @@ __global__ void rhs2d_kernel(Rhs2Args a) {
     const real temp_curv = (VAR ? a.bb[idx] : a.b_const) * curv;
     const real rhs = temp_norm + temp_curv + temp_vel;
     real val;
-    if (STAGE == 1) val = c + dt * rhs;
+    if (STAGE == 0) val = rhs;  // the right-hand side alone
+    else if (STAGE == 1) val = c + dt * rhs;
     else if (STAGE == 2) val = real(0.75) * a.un[idx] + real(0.25) * (c + dt * rhs);
     else val = real(1.0 / 3.0) * a.un[idx] + real(2.0 / 3.0) * (c + dt * rhs);
     a.out[idx] = val;
@@ -203,6 +204,27 @@ __global__ void rhs2d_kernel(Rhs2Args a) {
 // dt from H/dx at the last fill voxel (ihi, slab's last row) of each j-slab
 // (the reference resets max_H_over_dX inside its voxel loop), CFL 0.5:
 // dt = 0.5 / (max H + b_max/dx + U_max/dx)
+template <int S, bool VAR>
+__device__ real h2d_at(const Rhs2Args& a, int i, int j) {
+  const Box2& b = a.b;
+  const real inv_dx = real(1.0) / a.dx, inv_dy = real(1.0) / a.dy;
+  real d[4];
+  derivs2d<S>(a.in, b, i, j, inv_dx, inv_dy, d);
+  const real vn = VAR ? a.vn[b.at(i, j)] : a.a_const;
+  const real mdx = tmax(a.dx, a.dy);
+  const real max_dx_sq = mdx * mdx;
+  const real xc = tmax(fabs(d[0]), fabs(d[1])), yc = tmax(fabs(d[2]), fabs(d[3]));
+  const real ng = sqrt(xc * xc + yc * yc + max_dx_sq);
+  return fabs(vn) / ng * (xc * inv_dx + yc * inv_dy);
+}
+
+// H/dx at the last fill voxel (ihi, jhi), what one call of the reference's
+// 2D right-hand side returns
+template <int S, bool VAR>
+__global__ void h2d_kernel(Rhs2Args a, real* h) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *h = h2d_at<S, VAR>(a, a.b.ihi, a.b.jhi);
+}
+
 template <int S, bool VAR>
 __global__ void probe2d_kernel(Rhs2Args a, real bmax_over_dx, real umax_over_dx, real* dt_out) {
   __shared__ real hs[1024];
@@ -214,17 +236,7 @@ __global__ void probe2d_kernel(Rhs2Args a, real bmax_over_dx, real umax_over_dx,
     int jhi = b.jlo + (int)((int64_t)nsl * (t + 1) / a.nslabs) - 1;
     int jlo = b.jlo + (int)((int64_t)nsl * t / a.nslabs);
     if (jhi > b.jhi) jhi = b.jhi;
-    if (jlo <= jhi) {
-      const real inv_dx = real(1.0) / a.dx, inv_dy = real(1.0) / a.dy;
-      real d[4];
-      derivs2d<S>(a.in, b, b.ihi, jhi, inv_dx, inv_dy, d);
-      const real vn = VAR ? a.vn[b.at(b.ihi, jhi)] : a.a_const;
-      const real mdx = tmax(a.dx, a.dy);
-      const real max_dx_sq = mdx * mdx;
-      const real xc = tmax(fabs(d[0]), fabs(d[1])), yc = tmax(fabs(d[2]), fabs(d[3]));
-      const real ng = sqrt(xc * xc + yc * yc + max_dx_sq);
-      h = fabs(vn) / ng * (xc * inv_dx + yc * inv_dy);
-    }
+    if (jlo <= jhi) h = h2d_at<S, VAR>(a, b.ihi, jhi);
   }
   hs[t] = h;
   __syncthreads();
@@ -386,6 +398,42 @@ __global__ void circle2d_kernel(Box2 b, real* f, real x0, real y0, real dx, real
     // the reference calls the double sqrt and subtracts in double
     const real s2 = (x - cx) * (x - cx) + (y - cy) * (y - cy);
     f[l] = (real)(sqrt((double)s2) - (double)r);
+  }
+}
+
+// Kernel-level pieces (qss_tvd_runge_kutta2d.f; the one-edge signed linear
+// extrapolation of qss_spatial_derivatives2d.f): the fill box of b only.
+template <int STAGE>
+__global__ void rk2d_kernel(Box2 b, real* out, const real* u1, const real* un, const real* rhs,
+                            real dt) {
+  const int n = b.nfill();
+  for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < n; f += gridDim.x * blockDim.x) {
+    int i, j;
+    fill_ij(b, f, i, j);
+    const int idx = b.at(i, j);
+    if (STAGE == 1) out[idx] = u1[idx] + dt * rhs[idx];
+    else if (STAGE == 2) out[idx] = real(0.75) * un[idx] + real(0.25) * (u1[idx] + dt * rhs[idx]);
+    else out[idx] = real(1.0 / 3.0) * un[idx] + real(2.0 / 3.0) * (u1[idx] + dt * rhs[idx]);
+  }
+}
+
+// edge 0..3 = x-, x+, y-, y+: the ghost layers beyond the fill box on that
+// edge, over the whole ghost range of the other axis
+__global__ void edge2d_kernel(Box2 b, real* phi, int edge) {
+  const int ax = edge >> 1, hi = edge & 1;
+  const int cols = ax == 0 ? b.ny : b.nx;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < cols; c += gridDim.x * blockDim.x) {
+    auto at = [&](int w) { return ax == 0 ? b.at(w, c) : b.at(c, w); };
+    const int lo_f = ax == 0 ? b.ilo : b.jlo, hi_f = ax == 0 ? b.ihi : b.jhi;
+    const int nax = ax == 0 ? b.nx : b.ny;
+    const int e = hi ? hi_f : lo_f, e2 = hi ? hi_f - 1 : lo_f + 1;
+    const real pe = phi[at(e)];
+    const real slope = copysign(real(1), pe) * fabs(pe - phi[at(e2)]);
+    if (hi) {
+      for (int w = hi_f + 1; w < nax; ++w) phi[at(w)] = pe + slope * real(w - hi_f);
+    } else {
+      for (int w = 0; w < lo_f; ++w) phi[at(w)] = pe + slope * real(lo_f - w);
+    }
   }
 }
 
@@ -760,6 +808,92 @@ void labels_down(Run2& r, int* conn) {
   for (int64_t l = 0; l < r.n; ++l) conn[l] = lab[l] >= 0 ? lab[l] : 0;
 }
 
+// ghost-box bounds (4 ints) + fill box (4 ints) -> a Box2 indexed from the
+// ghost box's lower corner; the fill box plus a halo of h must lie inside
+Box2 kbox(const int* ilo_gb, const int* ihi_gb, const int* jlo_gb, const int* jhi_gb,
+          const int* ilo_fb, const int* ihi_fb, const int* jlo_fb, const int* jhi_fb, int h) {
+  Box2 b;
+  b.nx = *ihi_gb - *ilo_gb + 1;
+  b.ny = *jhi_gb - *jlo_gb + 1;
+  b.ilo = *ilo_fb - *ilo_gb; b.ihi = *ihi_fb - *ilo_gb;
+  b.jlo = *jlo_fb - *jlo_gb; b.jhi = *jhi_fb - *jlo_gb;
+  if (b.nx <= 0 || b.ny <= 0) throw std::runtime_error("empty ghost box");
+  if ((int64_t)b.nx * b.ny >= ((int64_t)1 << 31))
+    throw std::runtime_error("2D ghost box exceeds 2^31 voxels");
+  if (b.ilo > b.ihi || b.jlo > b.jhi) throw std::runtime_error("empty fill box");
+  if (b.ilo - h < 0 || b.jlo - h < 0 || b.ihi + h >= b.nx || b.jhi + h >= b.ny)
+    throw std::runtime_error("fill box plus stencil reaches outside the ghost box");
+  return b;
+}
+
+void same_gb2(const Box2& b, const int* i0, const int* i1, const int* j0, const int* j1,
+              const int* ilo_gb, const int* jlo_gb) {
+  if (*i0 != *ilo_gb || *j0 != *jlo_gb || *i1 - *i0 + 1 != b.nx || *j1 - *j0 + 1 != b.ny)
+    throw std::runtime_error("arrays of one call must share the ghost-box bounds");
+}
+
+struct Dev2 {
+  std::vector<void*> p;
+  ~Dev2() {
+    for (void* q : p) (void)hipFree(q);
+  }
+  real* put(const real* h, int64_t n) {
+    void* d = nullptr;
+    X2CHECK(hipMalloc(&d, (size_t)(n > 0 ? n : 1) * sizeof(real)));
+    p.push_back(d);
+    if (h) X2CHECK(hipMemcpy(d, h, (size_t)n * sizeof(real), hipMemcpyHostToDevice));
+    return (real*)d;
+  }
+  static void get(real* h, const real* d, int64_t n) {
+    X2CHECK(hipDeviceSynchronize());
+    X2CHECK(hipMemcpy(h, d, (size_t)n * sizeof(real), hipMemcpyDeviceToHost));
+  }
+};
+
+// one call of qss2dGetRHS{2nd,3rd}[const] over the given fill box (one slab)
+template <int S, bool VAR>
+void rhs2d_call(const real* phi, const real* vn, const real* bb, const real* vx, const real* vy,
+                real* max_H, real* lse_rhs, const Box2& b, real dx, real dy, real a0, real b0) {
+  const int64_t n = (int64_t)b.nx * b.ny;
+  Dev2 dv;
+  Rhs2Args a{};
+  a.b = b;
+  a.in = dv.put(phi, n);
+  if (VAR) {
+    a.vn = dv.put(vn, n);
+    a.bb = dv.put(bb, n);
+    a.vx = dv.put(vx, n);
+    a.vy = dv.put(vy, n);
+  }
+  a.a_const = a0;
+  a.b_const = b0;
+  a.dx = dx;
+  a.dy = dy;
+  a.nslabs = 1;
+  a.out = dv.put(lse_rhs, n);
+  real* h = dv.put(nullptr, 1);
+  hipLaunchKernelGGL((rhs2d_kernel<S, VAR, 0>), dim3(blocks2(b.nfill())), dim3(256), 0, 0, a);
+  X2CHECK(hipGetLastError());
+  hipLaunchKernelGGL((h2d_kernel<S, VAR>), dim3(1), dim3(64), 0, 0, a, h);
+  X2CHECK(hipGetLastError());
+  Dev2::get(lse_rhs, a.out, n);
+  Dev2::get(max_H, h, 1);
+}
+
+template <int STAGE>
+void rk2d_call(real* out, const real* u1, const real* un, const real* rhs, const Box2& b, real dt) {
+  const int64_t n = (int64_t)b.nx * b.ny;
+  Dev2 dv;
+  real* d_out = dv.put(out, n);
+  const real* d_u1 = dv.put(u1, n);
+  const real* d_un = STAGE == 1 ? nullptr : dv.put(un, n);
+  const real* d_rhs = dv.put(rhs, n);
+  hipLaunchKernelGGL((rk2d_kernel<STAGE>), dim3(blocks2(b.nfill())), dim3(256), 0, 0, b, d_out,
+                     d_u1, d_un, d_rhs, dt);
+  X2CHECK(hipGetLastError());
+  Dev2::get(out, d_out, n);
+}
+
 }  // namespace
 
 // 2D forms of the shared helper entry points (called from their C entry
@@ -877,6 +1011,105 @@ QSSLIB_REAL constCurvModel2d(Options* options, QSS_DataArrays* p, Grid* g, FILE*
 QSSLIB_REAL constCurvModel2dNoVar(Options* options, QSS_DataArrays* p, Grid* g, FILE* fp_out,
                                   QSSLIB_REAL a) {
   return ccm2_entry(options, p, g, fp_out, false, a);
+}
+
+// ---- kernel level, 2D (Fortran calling convention; reference:
+// qss_spatial_derivatives2d.f qss2dGetRHS2nd / 2ndconst / 3rd / 3rdconst and
+// the one-edge extrapolation, qss_tvd_runge_kutta2d.f) ----
+#define GB4 const int *ilo_gb, const int *ihi_gb, const int *jlo_gb, const int *jhi_gb
+#define FB4 const int *ilo_fb, const int *ihi_fb, const int *jlo_fb, const int *jhi_fb
+#define GB4N(p) const int *p##i0, const int *p##i1, const int *p##j0, const int *p##j1
+#define SAME2(b, p) same_gb2(b, p##i0, p##i1, p##j0, p##j1, ilo_gb, jlo_gb)
+#define TRY2 \
+  set_error("");  \
+  try {
+#define CATCH2                      \
+  }                                 \
+  catch (const std::exception& e) { \
+    set_error(e.what());            \
+  }
+
+void qss2dgetrhs2nd_(const QSSLIB_REAL* phi, const QSSLIB_REAL* vel_n, const QSSLIB_REAL* b,
+                     const QSSLIB_REAL* vx, const QSSLIB_REAL* vy, QSSLIB_REAL* max_H,
+                     QSSLIB_REAL* lse_rhs, GB4, FB4, const QSSLIB_REAL* dx, const QSSLIB_REAL* dy) {
+  TRY2 rhs2d_call<2, true>(phi, vel_n, b, vx, vy, max_H, lse_rhs,
+                           kbox(ilo_gb, ihi_gb, jlo_gb, jhi_gb, ilo_fb, ihi_fb, jlo_fb, jhi_fb, 2),
+                           *dx, *dy, 0, 0);
+  CATCH2
+}
+
+void qss2dgetrhs3rd_(const QSSLIB_REAL* phi, const QSSLIB_REAL* vel_n, const QSSLIB_REAL* b,
+                     const QSSLIB_REAL* vx, const QSSLIB_REAL* vy, QSSLIB_REAL* max_H,
+                     QSSLIB_REAL* lse_rhs, GB4, FB4, const QSSLIB_REAL* dx, const QSSLIB_REAL* dy) {
+  TRY2 rhs2d_call<3, true>(phi, vel_n, b, vx, vy, max_H, lse_rhs,
+                           kbox(ilo_gb, ihi_gb, jlo_gb, jhi_gb, ilo_fb, ihi_fb, jlo_fb, jhi_fb, 3),
+                           *dx, *dy, 0, 0);
+  CATCH2
+}
+
+void qss2dgetrhs2ndconst_(const QSSLIB_REAL* phi, const QSSLIB_REAL* vel_n, const QSSLIB_REAL* b,
+                          QSSLIB_REAL* max_H, QSSLIB_REAL* lse_rhs, GB4, FB4,
+                          const QSSLIB_REAL* dx, const QSSLIB_REAL* dy) {
+  TRY2 rhs2d_call<2, false>(phi, nullptr, nullptr, nullptr, nullptr, max_H, lse_rhs,
+                            kbox(ilo_gb, ihi_gb, jlo_gb, jhi_gb, ilo_fb, ihi_fb, jlo_fb, jhi_fb, 2),
+                            *dx, *dy, *vel_n, *b);
+  CATCH2
+}
+
+void qss2dgetrhs3rdconst_(const QSSLIB_REAL* phi, const QSSLIB_REAL* vel_n, const QSSLIB_REAL* b,
+                          QSSLIB_REAL* max_H, QSSLIB_REAL* lse_rhs, GB4, FB4,
+                          const QSSLIB_REAL* dx, const QSSLIB_REAL* dy) {
+  TRY2 rhs2d_call<3, false>(phi, nullptr, nullptr, nullptr, nullptr, max_H, lse_rhs,
+                            kbox(ilo_gb, ihi_gb, jlo_gb, jhi_gb, ilo_fb, ihi_fb, jlo_fb, jhi_fb, 3),
+                            *dx, *dy, *vel_n, *b);
+  CATCH2
+}
+
+void qss2drk1step_(QSSLIB_REAL* u_next, GB4, const QSSLIB_REAL* u_cur, GB4N(a),
+                   const QSSLIB_REAL* rhs, GB4N(c), FB4, const QSSLIB_REAL* dt) {
+  TRY2 Box2 bx = kbox(ilo_gb, ihi_gb, jlo_gb, jhi_gb, ilo_fb, ihi_fb, jlo_fb, jhi_fb, 0);
+  SAME2(bx, a);
+  SAME2(bx, c);
+  rk2d_call<1>(u_next, u_cur, nullptr, rhs, bx, *dt);
+  CATCH2
+}
+
+void qss2dtvdrk2stage2_(QSSLIB_REAL* u_stage2, GB4, const QSSLIB_REAL* u_stage1, GB4N(a),
+                        const QSSLIB_REAL* u_cur, GB4N(c), const QSSLIB_REAL* rhs, GB4N(e), FB4,
+                        const QSSLIB_REAL* dt) {
+  TRY2 Box2 bx = kbox(ilo_gb, ihi_gb, jlo_gb, jhi_gb, ilo_fb, ihi_fb, jlo_fb, jhi_fb, 0);
+  SAME2(bx, a);
+  SAME2(bx, c);
+  SAME2(bx, e);
+  rk2d_call<2>(u_stage2, u_stage1, u_cur, rhs, bx, *dt);
+  CATCH2
+}
+
+void qss2dtvdrk3stage3_(QSSLIB_REAL* u_next, GB4, const QSSLIB_REAL* u_stage2, GB4N(a),
+                        const QSSLIB_REAL* u_cur, GB4N(c), const QSSLIB_REAL* rhs, GB4N(e), FB4,
+                        const QSSLIB_REAL* dt) {
+  TRY2 Box2 bx = kbox(ilo_gb, ihi_gb, jlo_gb, jhi_gb, ilo_fb, ihi_fb, jlo_fb, jhi_fb, 0);
+  SAME2(bx, a);
+  SAME2(bx, c);
+  SAME2(bx, e);
+  rk2d_call<3>(u_next, u_stage2, u_cur, rhs, bx, *dt);
+  CATCH2
+}
+
+void qss2dsignedlinearextrapolation_(QSSLIB_REAL* phi, GB4, FB4, const int* bdry_location_idx) {
+  TRY2 const int edge = *bdry_location_idx;
+  if (edge < 0 || edge > 3) return;  // the reference's routine does nothing for other indices
+  Box2 bx = kbox(ilo_gb, ihi_gb, jlo_gb, jhi_gb, ilo_fb, ihi_fb, jlo_fb, jhi_fb, 0);
+  if ((edge >> 1) == 0 ? bx.ihi <= bx.ilo : bx.jhi <= bx.jlo)
+    throw std::runtime_error("extrapolation needs two fill voxels along the edge normal");
+  const int64_t n = (int64_t)bx.nx * bx.ny;
+  Dev2 dv;
+  real* d = dv.put(phi, n);
+  hipLaunchKernelGGL(edge2d_kernel, dim3(blocks2((edge >> 1) == 0 ? bx.ny : bx.nx)), dim3(256), 0,
+                     0, bx, d, edge);
+  X2CHECK(hipGetLastError());
+  Dev2::get(phi, d, n);
+  CATCH2
 }
 
 void findConnectivity2d(QSS_DataArrays* p, Grid* g) {

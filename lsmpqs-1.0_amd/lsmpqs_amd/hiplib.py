@@ -42,6 +42,8 @@ EXPORTS = (
     "imposemaskparallel_", "qss3dvolumeregionphilessthanzero_", "qss3dmaxnormdifflocal_",
     "getcentralgrad3dorder2_", "setvarnorm3d_", "setvarcurvadv3d_",
     "qss3dcomputedistanceforsubcellfix_", "qss3dcomputereinitializationeqnrhssubcellfixorder1_",
+    "qss2dgetrhs2nd_", "qss2dgetrhs3rd_", "qss2dgetrhs2ndconst_", "qss2dgetrhs3rdconst_",
+    "qss2drk1step_", "qss2dtvdrk2stage2_", "qss2dtvdrk3stage3_", "qss2dsignedlinearextrapolation_",
     "signedLinearExtrapolationBCqss", "initializeDisconnectedMasks", "createReservoirInlet3d",
     "setSpaceDerivFunc", "setThetaOverlap", "createOptionsDefault", "destroyOptions",
     "writeDataArrayQSS", "readDataArrayQSS", "allocateQSSDataArrays",

@@ -90,7 +90,7 @@ def test_kernel_entry_points_exported_and_declared():
     from lsmpqs_amd import hiplib
     hdr = open(os.path.join(os.path.dirname(HERE), "include", "lsmpqs_hip.h")).read()
     names = [n for n in hiplib.EXPORTS if n.endswith("_")]
-    assert len(names) == 17
+    assert len(names) == 17 + 8
     for dt in (np.float32, np.float64):
         path = hiplib.lib_path(dt)
         if not os.path.exists(path):
