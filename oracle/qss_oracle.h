@@ -123,6 +123,15 @@ typedef struct {
 } orc_stats;
 void orc_get_stats(orc_stats *s);
 
+/* Test switches (no counterpart in the reference).  orc_set_quirks: bits 1
+   (ENO3 x line reads phi(i+3,j,j)), 2 (WENO5 normal term with + sign), 4
+   (|vel_n| < 1e-5 reuses the previous voxel's normal term), all set by
+   default; a cleared bit gives the textbook scheme.  orc_set_connectivity:
+   26 (default, the reference's rule) or 6 (face neighbours).  Both return
+   the previous value. */
+unsigned orc_set_quirks(unsigned q);
+int orc_set_connectivity(int c);
+
 #ifdef __cplusplus
 }
 #endif

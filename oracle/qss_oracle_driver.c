@@ -96,6 +96,17 @@ static int uf_find(int *parent, int i)
   return r;
 }
 
+/* 26 (the reference's rule, connectivity.c:159-196) or 6 (face neighbours
+   only); set by orc_set_connectivity */
+static int orc_conn = 26;
+
+int orc_set_connectivity(int c)
+{
+  int old = orc_conn;
+  orc_conn = (c == 6) ? 6 : 26;
+  return old;
+}
+
 void orc_findConnectivity3d(QSS_DataArrays *p, Grid *g)
 {
   int nx = g->grid_dims_ghostbox[0], ny = g->grid_dims_ghostbox[1], nz = g->grid_dims_ghostbox[2];
@@ -114,6 +125,7 @@ void orc_findConnectivity3d(QSS_DataArrays *p, Grid *g)
             for (di = -1; di <= 1; di++) {
               int i2 = i + di, j2 = j + dj, k2 = k + dk, b, ra, rb;
               if (!(di || dj || dk)) continue;
+              if (orc_conn == 6 && (di != 0) + (dj != 0) + (dk != 0) != 1) continue;
               if (i2 < 0 || j2 < 0 || k2 < 0 || i2 >= nx || j2 >= ny || k2 >= nz) continue;
               b = i2 + j2 * nx + k2 * nx * ny;
               if (!p->phi_bin[b]) continue;

@@ -222,6 +222,22 @@ static R curvature_term(const R *phi, box_t bx, int i, int j, int k,
 
 enum { SCHEME_ENO2 = 2, SCHEME_ENO3 = 3, SCHEME_WENO5 = 5 };
 
+/* Reference-behaviour switches, all set by default (the reference's results):
+   1  the ENO3 x line reads phi(i+3,j,j)          (qss_spatial_derivatives3d.f:677,1454)
+   2  the WENO5 normal term enters with + sign     (:2276; - for ENO2/ENO3, :484)
+   4  where |vel_n| < 1e-5 the normal term of the previous voxel of the loop
+      is reused                                    (:459-486)
+   With a bit cleared the scheme is the textbook one: phi(i+3,j,k), -vel_n
+   |grad phi|_G for every scheme, and no normal term where |vel_n| < 1e-5. */
+static unsigned orc_quirk_bits = 7u;
+
+unsigned orc_set_quirks(unsigned q)
+{
+  unsigned old = orc_quirk_bits;
+  orc_quirk_bits = q & 7u;
+  return old;
+}
+
 /* Gather the 7-point line along each axis and apply the scheme.  The ENO3
    x-line reads phi(i+3,j,j) in place of phi(i+3,j,k), as the reference does
    (qss_spatial_derivatives3d.f:677,1454). */
@@ -240,7 +256,7 @@ static void upwind_derivs(const R *phi, box_t bx, int i, int j, int k, int schem
     ly[o + 3] = AT(phi, bx, i, jo, k);
     lz[o + 3] = AT(phi, bx, i, j, ko);
   }
-  if (scheme == SCHEME_ENO3) {
+  if (scheme == SCHEME_ENO3 && (orc_quirk_bits & 1u)) {
     /* linear address of phi(i+3,j,j); outside the array (possible only when
        the j range exceeds the k range) the reference reads undefined memory,
        the oracle reads 0 */
@@ -287,6 +303,8 @@ static void rhs_body(const R *phi, const R *vn_arr, R vn_s, const R *b_arr, R b_
         if (fabs(vn) >= (R)1.e-5) {
           R s = godunov_sq(vn, d[0], d[1], d[2], d[3], d[4], d[5]);
           temp_norm = norm_sign * vn * wsqrt(s);
+        } else if (!(orc_quirk_bits & 4u)) {
+          temp_norm = 0;
         }
         H = h_over_dx(vn, d[0], d[1], d[2], d[3], d[4], d[5], dx, dy, dz,
                       inv_dx, inv_dy, inv_dz);
@@ -353,7 +371,7 @@ void orc_getrhs3d5th(const R *phi, const R *vel_n, const R *b, const R *vx,
                      GBARGS, FBARGS, const R *dx, const R *dy, const R *dz)
 {
   rhs_body(phi, vel_n, 0, b, 0, vx, vy, vz, max_H, rhs, GB, FB,
-           *dx, *dy, *dz, SCHEME_WENO5, +1);
+           *dx, *dy, *dz, SCHEME_WENO5, (orc_quirk_bits & 2u) ? +1 : -1);
 }
 
 /* ------------------------------------------------------------------------ */

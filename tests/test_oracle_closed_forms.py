@@ -227,3 +227,128 @@ def test_overlap_thresholds(dtype, table):
         o.theta = deg * 3.14159 / 180
         op.oracle_lib(dtype).orc_setThetaOverlap(C.byref(o), C.byref(g))
         assert abs(o.overlap - want) < 1e-6, (deg, o.overlap)
+
+
+# ---------------------------------------------------------------------------
+# The textbook ("fixed") scheme behind the oracle's quirk switches
+# (orc_set_quirks): the batched subvolumes and the adjoint run with the
+# reference's quirks off, so each switch is pinned here on its own.
+
+class quirks:
+    """Context manager: the oracle's quirk bits set to `bits` inside."""
+
+    def __init__(self, bits):
+        self.bits = bits
+
+    def __enter__(self):
+        lib = op.oracle_lib(DT)
+        lib.orc_set_quirks.restype = C.c_uint
+        lib.orc_set_quirks.argtypes = [C.c_uint]
+        self.old = lib.orc_set_quirks(self.bits)
+
+    def __exit__(self, *exc):
+        op.oracle_lib(DT).orc_set_quirks(self.old)
+
+
+def noncubic(nx=14, ny=11, nz=7):
+    g = make_grid((nx, ny, nz), DT)
+    shape = (g.grid_dims_ghostbox[2], g.grid_dims_ghostbox[1], g.grid_dims_ghostbox[0])
+    z, y, x = np.meshgrid(*[np.arange(s, dtype=DT) for s in shape], indexing="ij")
+    return g, shape, x, y, z
+
+
+@pytest.mark.parametrize("name", ["getrhs3d2nd", "getrhs3d3rd", "getrhs3d5th"])
+def test_fixed_mode_plane_gives_minus_a_for_every_scheme(name):
+    """Quirks off: a tilted plane (|grad phi| = 1, kappa = 0) gives rhs = -a
+    for WENO5 as for ENO2/ENO3; with the sign switch back on WENO5 gives +a
+    (so the switch is what flips it)."""
+    g, shape, x, y, z = noncubic()
+    n = np.array([0.48, -0.6, 0.64])
+    n /= np.linalg.norm(n)
+    phi = n[0] * x + n[1] * y + n[2] * z - 3.1
+    a = 0.37
+    with quirks(0):
+        rhs, _ = rhs_var(name, phi, np.full(shape, a), np.full(shape, 0.05), g)
+    assert np.allclose(rhs[fill(g)], -a, atol=1e-12)
+    if name == "getrhs3d5th":
+        with quirks(2):
+            rhs, _ = rhs_var(name, phi, np.full(shape, a), np.full(shape, 0.05), g)
+        assert np.allclose(rhs[fill(g)], +a, atol=1e-12)
+
+
+def test_fixed_mode_eno3_exact_on_cubic_along_x_noncubic_box():
+    """Quirks off: ENO3 is exact for a cubic along x on a box whose y range
+    exceeds its z range; with the read switch on, the x line takes
+    phi(i+3,j,j), which is another voxel's value wherever j != k, and the
+    result is wrong there (and only there)."""
+    g, shape, x, y, z = noncubic()
+    # increasing in x with |phi_xx| falling to the right, so the upwind side
+    # (vn < 0 takes D+) widens its stencil to the right, where phi(i+3, .)
+    # enters through the rightmost third difference (0.02 for this cubic); a
+    # small slope in z makes the misread value lower that difference, and
+    # ENO then picks it, for j - k in 1..3
+    phi = -(40.0 - x) ** 3 / 300.0 - 0.01 * z
+    vn = np.full(shape, -1.0)                   # vn < 0: rhs = +|grad phi|
+    exact = np.sqrt((3 * (40.0 - x) ** 2 / 300.0) ** 2 + 0.01 ** 2)
+    with quirks(0):
+        rhs, _ = rhs_var("getrhs3d3rd", phi, vn, np.zeros(shape), g)
+    f = fill(g)
+    assert np.allclose(rhs[f], exact[f], rtol=1e-12)
+    with quirks(1):
+        rhs_q, _ = rhs_var("getrhs3d3rd", phi, vn, np.zeros(shape), g)
+    jj, kk = y[f], z[f]
+    off = ~np.isclose(rhs_q[f], exact[f], rtol=1e-9)
+    assert off.any()
+    assert not (off & (jj == kk)).any()
+
+
+def test_fixed_mode_small_speed_has_no_normal_term():
+    """Quirks off: where |vel_n| < 1e-5 the normal term is 0 (flat phi in
+    x, no curvature, no advection: rhs = 0 there); with the stale switch on
+    those voxels repeat the previous voxel's -vel_n |grad phi|."""
+    g, shape, x, y, z = noncubic()
+    phi = x - 5.2
+    vn = np.full(shape, 0.8)
+    vn[:, :, 8] = 3e-6                          # one x plane of tiny speeds
+    with quirks(0):
+        rhs, _ = rhs_var("getrhs3d2nd", phi, vn, np.zeros(shape), g)
+    inside = _fillmask(g, shape)
+    small = np.zeros(shape, bool)
+    small[:, :, 8] = True
+    assert np.all(rhs[small & inside] == 0.0)
+    assert np.allclose(rhs[~small & inside], -0.8, atol=1e-12)
+    with quirks(4):
+        rhs_q, _ = rhs_var("getrhs3d2nd", phi, vn, np.zeros(shape), g)
+    assert np.allclose(rhs_q[small & inside], -0.8, atol=1e-12)
+
+
+def _fillmask(g, shape):
+    m = np.zeros(shape, bool)
+    m[fill(g)] = True
+    return m
+
+
+def test_six_connected_labels_match_scipy():
+    """orc_set_connectivity(6): face-neighbour components, each labelled with
+    its largest linear index, against scipy's 6-connected labelling (and
+    different from the 26-connected partition on the same field)."""
+    g = make_grid((13, 11, 9), DT)
+    A = Arrays(g, DT)
+    rng = np.random.default_rng(5)
+    A["phi_bin"] = (rng.random(A.shape) < 0.35).astype(np.uint8)
+    lib = op.oracle_lib(DT)
+    res = {}
+    for conn, struct in ((6, None), (26, np.ones((3, 3, 3)))):
+        old = lib.orc_set_connectivity(conn)
+        try:
+            lib.orc_findConnectivity3d(A.ptr(), C.byref(g))
+        finally:
+            lib.orc_set_connectivity(old)
+        lab, nlab = ndimage.label(A["phi_bin"], structure=struct)
+        flat = lab.ravel()
+        top = np.full(nlab + 1, -1, dtype=np.int64)
+        np.maximum.at(top, flat, np.arange(flat.size))
+        want = np.where(flat > 0, top[flat], 0).reshape(lab.shape)
+        np.testing.assert_array_equal(A["connectivity"], want)
+        res[conn] = A["connectivity"].copy()
+    assert not np.array_equal(res[6], res[26])
