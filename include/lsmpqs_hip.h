@@ -363,6 +363,25 @@ int lsmpqs_init_from_mask(const QSSLIB_REAL *mask, Grid *g, Options *options, QS
 int lsmpqs_diff_steps(const QSSLIB_REAL *step_a, const QSSLIB_REAL *step_b, Grid *g,
                       QSSLIB_REAL *diff, QSSLIB_REAL *connected_only);
 
+/* Allocation-free volume of the region phi < 0.  No counterpart in the
+   reference: it is for callers that cannot allocate device memory (code
+   under HIP graph capture, tight serving loops, harnesses that forbid
+   hipMalloc).  The same FP64 sum as qss3dvolumeregionphilessthanzero_
+   (smoothed Heaviside of width eps, dV = dx dy dz) over DEVICE arrays:
+   phi is a device array over the ghost box gb = {ilo, ihi, jlo, jhi, klo,
+   khi} with fill box fb (same convention as the Fortran-ABI routine), and
+   *volume is a device scalar, written asynchronously on `stream` (a
+   hipStream_t; NULL: the null stream).  The call allocates nothing: its
+   reduction partials and result live in a library-owned 32.8 KB device
+   array, one per GPU, held until the process ends; the result does not
+   depend on the box size, which may be anything up to 2^31 ghost voxels.
+   Calls that overlap on one GPU (concurrent streams or host threads) race
+   on that array.  Returns 0, or -2 with the message in
+   lsmpqs_hip_last_error(). */
+int lsmpqs_volume_noalloc(QSSLIB_REAL *volume, const QSSLIB_REAL *phi, const int *gb,
+                          const int *fb, QSSLIB_REAL dx, QSSLIB_REAL dy, QSSLIB_REAL dz,
+                          QSSLIB_REAL eps, void *stream);
+
 /* free the device state kept for record p */
 void lsmpqs_release(QSS_DataArrays *p);
 

@@ -13,6 +13,7 @@
 // entry points do not use them.  Several arrays of one call must share the
 // same ghost-box bounds (the reference always passes one GB_DIMS).
 #include <math.h>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -337,6 +338,40 @@ __global__ void impose_planes_kernel(real* dst, const real* m, const real* src, 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free volume entry point: the per-block partial
+// sums and maxima of the reduction (2 REDUCE_BLOCKS doubles) and its FP64
+// result (2 doubles), 32.8 KB in all.  One copy per GPU, held until the
+// process ends.  In host code `scratch` names a host-side stand-in, not the
+// device array: host code uses only the address hipGetSymbolAddress returns
+// for the current device (scratch_ptr()).
+constexpr int SCRATCH_DOUBLES = 2 * REDUCE_BLOCKS + 4;
+__device__ __align__(256) double scratch[SCRATCH_DOUBLES];
+
+constexpr int MAX_DEVICES = 64;
+
+// Device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept.
+double* scratch_ptr() {
+  static std::mutex mu;
+  static double* ptr[MAX_DEVICES] = {};
+  int dev = 0;
+  FCHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES) throw std::runtime_error("device ordinal out of range");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptr[dev]) {
+    void* p = nullptr;
+    FCHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    ptr[dev] = (double*)p;
+  }
+  return ptr[dev];
+}
+
+// dst[0] = src[0] rounded to QSSLIB_REAL, or 0 for an empty fill box
+__global__ void store_real_kernel(const double* src, real* dst, int empty) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) dst[0] = empty ? real(0) : (real)src[0];
+}
+
 }  // namespace
 }  // namespace lsm
 
@@ -478,6 +513,32 @@ void qss3dvolumeregionphilessthanzero_(QSSLIB_REAL* volume, const QSSLIB_REAL* p
   get(h, out, 2);
   *volume = (QSSLIB_REAL)h[0];
   FCATCH
+}
+
+int lsmpqs_volume_noalloc(QSSLIB_REAL* volume, const QSSLIB_REAL* phi, const int* gb,
+                          const int* fb, QSSLIB_REAL dx, QSSLIB_REAL dy, QSSLIB_REAL dz,
+                          QSSLIB_REAL eps, void* stream) {
+  FTRY
+  if (!volume || !phi || !gb || !fb) throw std::runtime_error("lsmpqs_volume_noalloc: NULL argument");
+  FBox f = fbox(gb + 0, gb + 1, gb + 2, gb + 3, gb + 4, gb + 5, fb + 0, fb + 1, fb + 2, fb + 3,
+                fb + 4, fb + 5);
+  need_halo(f, 0);
+  if (reduce_partials() + 2 > SCRATCH_DOUBLES)
+    throw std::runtime_error("lsmpqs_volume_noalloc: reduction needs more than its " +
+                             std::to_string(SCRATCH_DOUBLES * sizeof(double)) + "-byte scratch");
+  double* part = scratch_ptr();
+  double* out = part + reduce_partials();
+  hipStream_t st = (hipStream_t)stream;
+  const int empty = nfill(f.b) == 0;
+  if (!empty) FCHECK(launch_outer_reduce(f.b, phi, nullptr, 0, eps, dx * dy * dz, part, out, st));
+  hipLaunchKernelGGL(store_real_kernel, dim3(1), dim3(64), 0, st, out, volume, empty);
+  FCHECK(hipGetLastError());
+  return 0;
+  }
+  catch (const std::exception& e) {
+    set_error(e.what());
+    return -2;
+  }
 }
 
 void qss3dmaxnormdifflocal_(QSSLIB_REAL* max_norm, const QSSLIB_REAL* field1, GB6,

@@ -141,7 +141,9 @@ hipError_t launch_floor(real* floor, const real* m, const real* mw, const real* 
 
 // FP64 reductions over the fill box.  out[0] = smoothed volume of {f1 < 0}
 // (width eps), out[1] = max |f1 - f2| over |f1| < zone (-1 if none; skipped
-// when f2 is NULL).  partials needs reduce_partials() doubles.
+// when f2 is NULL).  partials needs reduce_partials() = 2 REDUCE_BLOCKS
+// doubles whatever the box size (at most REDUCE_BLOCKS blocks per pass).
+constexpr int REDUCE_BLOCKS = 2048;
 int reduce_partials();
 hipError_t launch_outer_reduce(const Box& bx, const real* f1, const real* f2, real zone, real eps,
                                real dV, double* partials, double* out, hipStream_t st);

@@ -94,7 +94,7 @@ hipError_t launch_fill(real* dst, real v, int64_t n, hipStream_t st) {
 // Fill-box iteration helper: flat fill index -> (i,j,k)
 
 
-constexpr int RED_BLOCKS = 2048;
+constexpr int RED_BLOCKS = REDUCE_BLOCKS;
 int reduce_partials() { return 2 * RED_BLOCKS; }
 
 __device__ __forceinline__ double wave_sum(double v) {

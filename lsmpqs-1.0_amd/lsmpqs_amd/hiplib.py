@@ -54,7 +54,7 @@ EXPORTS = (
     "createGridSetDxDyDz", "createGridSetGridDims", "destroyGrid", "printGrid",
     "writeGridToBinaryFile", "readGridFromBinaryFile", "reinitializeMedium3d",
     "lsmpqs_batch_model3d", "lsmpqs_adjoint_run", "writeDataArrayRaw", "lsmpqs_mask_from_image",
-    "lsmpqs_init_from_mask", "lsmpqs_diff_steps",
+    "lsmpqs_init_from_mask", "lsmpqs_diff_steps", "lsmpqs_volume_noalloc",
 )
 
 
