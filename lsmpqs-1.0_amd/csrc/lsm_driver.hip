@@ -12,6 +12,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <algorithm>
+#include <chrono>
 #include <stdexcept>
 #include <vector>
 #include <memory>
@@ -344,6 +345,7 @@ static real ccm_dev(Session* s, Options* o, const Grid* g, FILE* fp, bool var, r
   vol_phi = s->h_red[0];
   satn = vol_phi / vol_pore;
   (void)bx;
+  std::unique_ptr<StepWriter> ckpt;
   while ((t < o->tmax) && (max_abs_err > (double)o->eps_stop)) {
     s->tm.outer_steps++;
     DCHECK(hipMemcpyAsync(s->phi_prev, s->phi, bytes(s), hipMemcpyDeviceToDevice, s->st));
@@ -356,6 +358,7 @@ static real ccm_dev(Session* s, Options* o, const Grid* g, FILE* fp, bool var, r
     (void)hipEventElapsedTime(&ms, s->ev_a, s->ev_b);
     s->tm.inner_ms += ms;
     if (fp) fprintf(fp, "1 GPU: Level set time = %lf\n", ms * 1e-3);
+    const auto t3 = std::chrono::steady_clock::now();
     t += dt_sub;
     if (var) DCHECK(launch_impose(s->phi, s->mask, s->phi, 0, s->n, s->st));
     if (o->check_connectivity) {
@@ -378,8 +381,24 @@ static real ccm_dev(Session* s, Options* o, const Grid* g, FILE* fp, bool var, r
     max_abs_err = s->h_red[1];
     satn_old = satn;
     satn = vol_phi / vol_pore;
-    if ((satn > (double)o->vol_frac_max) || (vol_phi <= (double)vsmall)) return -1;
+    if ((satn > (double)o->vol_frac_max) || (vol_phi <= (double)vsmall)) {
+      if (ckpt) ckpt->finish();
+      return -1;
+    }
+    if (o->checkpoint) {
+      // restart files of this outer step (gz, working directory), written by a
+      // worker thread from snapshots while the next outer step runs; a later
+      // step's files replace them in order
+      if (!ckpt) ckpt.reset(new StepWriter(bytes(s), g->grid_dims_ghostbox, 4));
+      ckpt->submit(s->st, s->phi, "checkpoint_phi");
+      ckpt->submit(s->st, s->phi_prev, "checkpoint_phi_prev");
+      ckpt->submit(s->st, s->mask_w, "mask_w");
+      ckpt->submit(s->st, s->mask_nw, "mask_nw");
+    }
+    const double conn_s =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t3).count();
     if (fp) {
+      fprintf(fp, "connectivity time = %lf\n", conn_s);
       fprintf(fp, "t = %4.1f\t", (double)t);
       fprintf(fp, "max_abs_err = %4.3f,\t", max_abs_err);
       fprintf(fp, "nw phase satn pct = %4.3f\n", satn * 100);
@@ -389,6 +408,7 @@ static real ccm_dev(Session* s, Options* o, const Grid* g, FILE* fp, bool var, r
       if (fabsf((float)(100 * (satn - satn_old))) < 0.001) break;
     }
   }
+  if (ckpt) ckpt->finish();
   return t;
 }
 

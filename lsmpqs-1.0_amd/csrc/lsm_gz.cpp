@@ -78,9 +78,11 @@ bool write_array_gz(const std::string& path, const int dims[3], const void* data
   return gz_write(path, {{dims, 3 * sizeof(int)}, {data, bytes}}, 0);
 }
 
-StepWriter::StepWriter(size_t bytes, const int dims[3]) : bytes_(bytes) {
+StepWriter::StepWriter(size_t bytes, const int dims[3], int slots)
+    : bytes_(bytes), nslots_(slots < 1 ? 1 : (slots > MAX_SLOTS ? MAX_SLOTS : slots)) {
   for (int a = 0; a < 3; ++a) dims_[a] = dims[a];
-  for (Slot& s : slots_) {
+  for (int k = 0; k < nslots_; ++k) {
+    Slot& s = slots_[k];
     if (hipHostMalloc(&s.host, bytes_, hipHostMallocDefault) != hipSuccess ||
         hipEventCreateWithFlags(&s.copied, hipEventDisableTiming) != hipSuccess) {
       for (Slot& t : slots_) {
@@ -106,12 +108,18 @@ StepWriter::~StepWriter() {
   }
 }
 
+StepWriter::Slot* StepWriter::free_slot() {
+  for (int k = 0; k < nslots_; ++k)
+    if (!slots_[k].busy) return &slots_[k];
+  return nullptr;
+}
+
 void StepWriter::submit(hipStream_t st, const void* device_field, const std::string& name) {
   std::unique_lock<std::mutex> lk(mu_);
-  cv_free_.wait(lk, [&] { return !slots_[0].busy || !slots_[1].busy || !error_.empty(); });
+  cv_free_.wait(lk, [&] { return free_slot() != nullptr || !error_.empty(); });
   if (!error_.empty()) throw std::runtime_error(error_);
-  const int k = slots_[0].busy ? 1 : 0;
-  Slot& s = slots_[k];
+  Slot& s = *free_slot();
+  const int k = (int)(&s - slots_);
   if (hipMemcpyAsync(s.host, device_field, bytes_, hipMemcpyDeviceToHost, st) != hipSuccess ||
       hipEventRecord(s.copied, st) != hipSuccess)
     throw std::runtime_error("step-file writer: snapshot copy failed for " + name);

@@ -26,15 +26,17 @@ bool gz_write(const std::string& path, const std::vector<GzPart>& parts, int thr
 // Reference array-file layout: int32 dims[3], then the values.
 bool write_array_gz(const std::string& path, const int dims[3], const void* data, size_t bytes);
 
-// Asynchronous data_step_N files.  submit() enqueues a device-to-host copy of
-// the field into one of two pinned snapshot buffers on the caller's stream
-// (so later kernels on that stream cannot change the snapshot) and returns;
-// a worker thread waits for the copy and writes `name`.gz.  With both
-// buffers busy, submit() waits for the older file to finish.  finish()
-// waits for every file and reports the first failure.
+// Asynchronous array files (data_step_N, the checkpoint files).  submit()
+// enqueues a device-to-host copy of the field into one of `slots` pinned
+// snapshot buffers on the caller's stream (so later kernels on that stream
+// cannot change the snapshot) and returns; a worker thread waits for the copy
+// and writes `name`.gz, in submission order.  With every buffer busy,
+// submit() waits for the oldest file to finish.  finish() waits for every
+// file and reports the first failure.
 class StepWriter {
  public:
-  StepWriter(size_t bytes, const int dims[3]);
+  static constexpr int MAX_SLOTS = 4;
+  StepWriter(size_t bytes, const int dims[3], int slots = 2);
   ~StepWriter();
   void submit(hipStream_t st, const void* device_field, const std::string& name);
   void finish();
@@ -51,9 +53,11 @@ class StepWriter {
   };
   void run();
 
+  Slot* free_slot();
   size_t bytes_;
   int dims_[3];
-  Slot slots_[2];
+  int nslots_;
+  Slot slots_[MAX_SLOTS];
   std::deque<Job> jobs_;
   std::mutex mu_;
   std::condition_variable cv_work_, cv_free_;
