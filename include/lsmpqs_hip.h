@@ -312,7 +312,8 @@ int lsmpqs_batch_model3d(int n, Options **options, QSS_DataArrays **p, Grid **g,
    library uses its fast WENO5 and curvature arithmetic.  theta: per-voxel
    contact angle (radians) on the ghost box; p->phi the initial state (ghost
    values used as given), p->mask the pore mask.  On return p->phi holds the
-   final state, dloss_dtheta the gradient on the fill box and 0 on ghosts,
+   final state, p->normal_velocity (when not NULL) the per-voxel vel_n of the
+   forward map on the fill box, dloss_dtheta the gradient on the fill box and 0 on ghosts,
    args the loss, S_w, the time step and d loss / d b (dloss_dtheta NULL:
    loss only, no reverse pass, dloss_db = 0).  The
    reverse pass transposes the linearisation of every stage with the

@@ -713,6 +713,8 @@ extern "C" int lsmpqs_adjoint_run(Options* o, QSS_DataArrays* p, Grid* g, const 
     args->loss = r * r;
     args->dt_used = (double)dt;
     ACHECK(hipMemcpyAsync(p->phi, u, bytes, hipMemcpyDeviceToHost, s->st));
+    if (p->normal_velocity)  // the per-voxel vel_n of the forward map
+      ACHECK(hipMemcpyAsync(p->normal_velocity, s->vn, bytes, hipMemcpyDeviceToHost, s->st));
     if (!dloss_dtheta) {  // loss only
       args->dloss_db = 0;
       ACHECK(hipStreamSynchronize(s->st));

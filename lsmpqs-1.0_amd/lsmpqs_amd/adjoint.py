@@ -77,12 +77,16 @@ def saturation_loss(phi0, mask, theta, grid, options, nsteps, a0, b, target_sw, 
     a.checkpoint_every = int(checkpoint_every or max(1, int(round(np.sqrt(nsteps)))))
     a.dt, a.a0, a.b, a.target_sw = float(dt), float(a0), float(b), float(target_sw)
     a.phi_cap = float(phi_cap)
-    rc = lib.lsmpqs_adjoint_run(C.addressof(options), C.addressof(rec), C.addressof(grid),
-                                theta.ctypes.data,
-                                dtheta.ctypes.data if gradient else None, C.byref(a))
-    if rc != 0:
-        raise hiplib.LsmpqsError(hiplib.last_error(lib))
-    lib.lsmpqs_release(C.c_void_p(C.addressof(rec)))
+    try:
+        rc = lib.lsmpqs_adjoint_run(C.addressof(options), C.addressof(rec), C.addressof(grid),
+                                    theta.ctypes.data,
+                                    dtheta.ctypes.data if gradient else None, C.byref(a))
+        if rc != 0:
+            raise hiplib.LsmpqsError(hiplib.last_error(lib))
+    finally:
+        # the device session is keyed on this temporary record: free it on
+        # every path, so a later record at the same address starts clean
+        lib.lsmpqs_release(C.c_void_p(C.addressof(rec)))
     return AdjointResult(a, dtheta, phi)
 
 

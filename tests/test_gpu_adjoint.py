@@ -243,3 +243,59 @@ def test_contact_angle_field_128():
     assert rel <= 2e-3
     assert derr <= FD_TOL
     assert abs(r32.dloss_db - r64.dloss_db) <= 1e-3 * abs(r64.dloss_db)
+
+
+def _theta_fixture():
+    import os
+    import sys
+    here = os.path.dirname(os.path.abspath(__file__))
+    sys.path.insert(0, os.path.join(here, "golden"))
+    import make_theta_coeff as mt
+    return mt, np.load(os.path.join(here, "golden", "theta_coeff_f64.npz"))
+
+
+def test_theta_fixture_is_the_reference_output():
+    """The committed setVarNorm3dTheta vectors are the reference build's (CPU)."""
+    import oracle_py as op
+    if not op.have_ref(np.float64):
+        pytest.skip("reference build not present")
+    mt, d = _theta_fixture()
+    g, mask, _, theta = mt.inputs()
+    np.testing.assert_array_equal(mt.reference_var_a(op.ref_lib(np.float64), g, mask, theta),
+                                  d["var_a"])
+
+
+@pytest.mark.gpu
+def test_angle_field_speed_matches_reference_setvarnorm3dtheta():
+    """The per-voxel contact-angle speed the adjoint's forward map uses
+    (vel_n = h1 a0 - s h2 cos(theta) |grad mask|) against the reference
+    build's setVarNorm3dTheta on the same mask and angle field
+    (tests/golden/theta_coeff_f64.npz, tests/golden/make_theta_coeff.py).
+    FP64; cos and the Heaviside's sin are the device's and glibc's, so the
+    tolerance is a few ulps of the speed."""
+    import ctypes as C
+    from lsmpqs_amd import hiplib
+    from lsmpqs_amd.abi import Arrays
+    from lsmpqs_amd.adjoint import AdjointArgs, _lib
+    print("\n[adjoint] setVarNorm3dTheta speed vs reference, 20^3 fp64", flush=True)
+    mt, d = _theta_fixture()
+    g, mask, phi0, theta = mt.inputs()
+    lib = _lib(np.float64)
+    A = Arrays(g, np.float64)
+    A["mask"] = mask
+    A["phi"] = phi0
+    A["normal_velocity"] = -7.0
+    th = np.ascontiguousarray(theta)
+    o = default_options(np.float64)
+    o.order_space_accur, o.order_time_accur = 5, 3
+    a = AdjointArgs()
+    a.nsteps, a.checkpoint_every = 1, 1
+    a.dt, a.a0, a.b, a.target_sw = 0.0, float(d["a0"]), B, TARGET
+    rc = lib.lsmpqs_adjoint_run(C.addressof(o), C.addressof(A.rec), C.addressof(g),
+                                th.ctypes.data, None, C.byref(a))
+    lib.lsmpqs_release(A.ptr())
+    assert rc == 0, hiplib.last_error(lib)
+    f = fill_of(g)
+    diff = float(np.max(np.abs(A["normal_velocity"][f] - d["var_a"][f])))
+    print(f"  max|vel_n - reference| = {diff:.2e}", flush=True)
+    assert diff <= 1e-13
