@@ -22,11 +22,17 @@ def ghost_coords(grid):
     return out
 
 
-def sphere_pack_mask(grid, radius, porosity, seed, clip=6.0, max_spheres=10_000_000):
+def sphere_pack_mask(grid, radius, porosity, seed, clip=6.0, max_spheres=10_000_000,
+                     device=None):
     """Signed distance to a Boolean sphere pack, > 0 in solid, float64.
 
     radius: a number, or a (rmin, rmax) pair for radii uniform in [rmin, rmax].
+    device: None builds the mask with numpy; a torch device ("cuda") builds
+    it there from the same seeded spheres (for packs of 1e9 voxels) and
+    returns it to the host.
     Returns (mask, centres (m,3), radii (m,), achieved fill-box porosity)."""
+    if device is not None:
+        return _sphere_pack_mask_torch(grid, radius, porosity, seed, clip, max_spheres, device)
     rng = np.random.default_rng(seed)
     xs, ys, zs = ghost_coords(grid)
     dx = float(grid.dx[0])
@@ -74,6 +80,63 @@ def sphere_pack_mask(grid, radius, porosity, seed, clip=6.0, max_spheres=10_000_
             break
         add(step)
     return mask, np.array(centres), np.array(radii), phi_now
+
+
+def _sphere_pack_mask_torch(grid, radius, porosity, seed, clip, max_spheres, device):
+    """sphere_pack_mask with the mask held and updated as a torch tensor on
+    `device`; the spheres come from the same seeded generator in the same
+    order."""
+    import torch
+    rng = np.random.default_rng(seed)
+    coords = ghost_coords(grid)
+    xs, ys, zs = (torch.from_numpy(c).to(device) for c in coords)
+    axes_np = coords
+    dx = float(grid.dx[0])
+    N = [grid.grid_dims_ghostbox[a] for a in range(3)]
+    mask = torch.full((N[2], N[1], N[0]), -clip, dtype=torch.float64, device=device)
+    lo = [float(grid.x_lo[a]) for a in range(3)]
+    hi = [float(grid.x_hi[a]) for a in range(3)]
+    fb = (slice(grid.klo_fb, grid.khi_fb + 1), slice(grid.jlo_fb, grid.jhi_fb + 1),
+          slice(grid.ilo_fb, grid.ihi_fb + 1))
+    rmin, rmax = (float(radius), float(radius)) if np.isscalar(radius) else \
+        (float(radius[0]), float(radius[1]))
+    vol_box = np.prod([hi[a] - lo[a] for a in range(3)])
+    er3 = (rmax ** 4 - rmin ** 4) / (4 * (rmax - rmin)) if rmax > rmin else rmin ** 3
+    expected = int(np.ceil(-np.log(porosity) * vol_box / (4.0 / 3.0 * np.pi * er3)))
+    centres, radii = [], []
+
+    def add(count):
+        for _ in range(count):
+            c = np.array([rng.uniform(lo[a], hi[a]) for a in range(3)])
+            r = rmin if rmax == rmin else rng.uniform(rmin, rmax)
+            centres.append(c)
+            radii.append(r)
+            reach = r + clip
+            sl = []
+            for a in range(3):
+                i0 = max(0, int(np.floor((c[a] - reach - axes_np[a][0]) / dx)))
+                i1 = min(N[a], int(np.ceil((c[a] + reach - axes_np[a][0]) / dx)) + 1)
+                sl.append((i0, i1))
+            if any(i1 <= i0 for i0, i1 in sl):
+                continue
+            X = xs[sl[0][0]:sl[0][1]] - c[0]
+            Y = ys[sl[1][0]:sl[1][1]] - c[1]
+            Z = zs[sl[2][0]:sl[2][1]] - c[2]
+            d = r - torch.sqrt(Z[:, None, None] ** 2 + Y[None, :, None] ** 2
+                               + X[None, None, :] ** 2)
+            view = mask[sl[2][0]:sl[2][1], sl[1][0]:sl[1][1], sl[0][0]:sl[0][1]]
+            torch.maximum(view, d, out=view)
+
+    add(int(0.9 * expected))
+    step = max(1, expected // 50)
+    while True:
+        phi_now = float((mask[fb] < 0).sum().item()) / float(mask[fb].numel())
+        if phi_now <= porosity or len(radii) >= max_spheres:
+            break
+        add(step)
+    out = mask.cpu().numpy()
+    del mask
+    return out, np.array(centres), np.array(radii), phi_now
 
 
 def initial_phi(grid, mask):

@@ -1,0 +1,164 @@
+"""Writes tests/golden/ref512_<case>.npz by running the REFERENCE build
+(oracle/_ref/libqssref_f64.so, compiled from /root/reference by
+oracle/Makefile.ref) on one curvature step of a 512^3 drainage case.
+
+    python tests/golden/make_ref512.py c2|c3 [threads]
+
+Cases (seeded Boolean sphere packs, HJ-WENO5 + TVD-RK3, trapping on,
+eps_stop 1e-4 dx, pseudo-time 10 per curvature step, one drainage
+curvature step from a = 0.02 with b = 0.04):
+  c2  radius U[6,12],  porosity 0.36, seed 1002, contact angle 30 deg
+  c3  radius U[10,20], porosity 0.35, seed 1003, contact angle 39.999 deg
+      (overlap 0.3 dx in both precisions; at exactly 40 deg the
+      reference's threshold test, qss_general_util.c:160-172, rounds
+      differently in its FP32 and FP64 builds)
+The full fields are 1.1 GB each in FP64, so the fixture keeps what a
+comparison needs (data only), as ref256_drain_imbibe.npz does:
+  sat_drain        FP64 smoothed non-wetting saturation
+  plane_sum_drain  per-z-plane sums of phi over the fill box
+  idx, val_drain   4096 seeded voxels of the |phi| < 3 band and their phi
+  nneg_drain       fill voxels with phi < 0
+  nw_trap_drain, w_trap_drain  voxels with mask_nw > 0, mask_w > 0
+  outer_drain      outer steps in the log
+  iface_idx        every fill voxel with phi < 0 that has a face neighbour
+                   with phi >= 0, or the reverse (the interface voxels)
+  wall_s           wall time of the reference's drain_model3d
+"""
+import ctypes as C
+import os
+import resource
+import shutil
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path[:0] = [os.path.join(ROOT, "lsmpqs-1.0_amd"), os.path.join(ROOT, "oracle"),
+                os.path.join(ROOT, "tests")]
+
+import oracle_py as op  # noqa: E402
+from lsm_cases import arrays, fill, saturation  # noqa: E402
+from lsmpqs_amd.abi import default_options  # noqa: E402
+from lsmpqs_amd.inputs import pack_case  # noqa: E402
+
+N = 512
+NSAMPLE = 4096
+CASES = {
+    "c2": {"radius": (6.0, 12.0), "porosity": 0.36, "seed": 1002, "theta_deg": 30.0},
+    "c3": {"radius": (10.0, 20.0), "porosity": 0.35, "seed": 1003, "theta_deg": 39.999},
+}
+
+
+def out_path(name):
+    return os.path.join(HERE, f"ref512_{name}.npz")
+
+
+def case_options(name, dtype):
+    """Options of the case (shared with the GPU test)."""
+    c = CASES[name]
+    o = default_options(dtype)
+    o.order_space_accur = 5
+    o.order_time_accur = 3
+    o.theta = c["theta_deg"] * 3.14159 / 180
+    o.a = 0.02
+    o.b = 0.04
+    o.dc = 0.25
+    o.amax = o.a + o.b * o.dc * 0.5
+    o.tmax = 10.0
+    o.eps_stop = 1e-4
+    o.check_connectivity = 1
+    return o
+
+
+def case_inputs(name, dtype):
+    c = CASES[name]
+    return pack_case((N, N, N), c["radius"], c["porosity"], c["seed"], dtype)
+
+
+def interface_voxels(phi, g):
+    """Sorted linear indices of the fill voxels whose phase (phi < 0)
+    differs from that of a face neighbour."""
+    neg = phi < 0
+    f = fill(g)
+    iface = np.zeros(neg.shape, dtype=bool)
+    for ax in range(3):
+        d = neg != np.roll(neg, 1, axis=ax)
+        iface |= d | np.roll(d, -1, axis=ax)
+    keep = np.zeros(neg.shape, dtype=bool)
+    keep[f] = True
+    return np.flatnonzero(iface & keep)
+
+
+def band_sample(phi, g, seed):
+    f = np.zeros(phi.shape, dtype=bool)
+    f[fill(g)] = True
+    cand = np.flatnonzero((np.abs(phi) < 3.0) & f)
+    return np.sort(np.random.default_rng(seed).choice(cand, NSAMPLE, replace=False))
+
+
+def main():
+    name = sys.argv[1]
+    T = int(sys.argv[2]) if len(sys.argv) > 2 else 8
+    dt = np.float64
+    t0 = time.perf_counter()
+    g, mask, phi0, info = case_inputs(name, dt)
+    print(f"{name}: pack of {info['spheres']} spheres, porosity {info['porosity']:.4f} "
+          f"({time.perf_counter() - t0:.0f} s)", flush=True)
+    A = arrays(g, mask, phi0, dt)
+    del mask, phi0
+    # The reference's unionCoords3d reads phi_bin at the neighbour's index
+    # before testing that the neighbour lies in the box
+    # (src/connectivity/connectivity.c:138-146), up to nx*ny + nx + 1 bytes
+    # before the array; it discards what it reads there, but the bytes must be
+    # readable, so phi_bin gets that much room in front of it.
+    nx, ny = g.grid_dims_ghostbox[0], g.grid_dims_ghostbox[1]
+    pad = nx * ny + nx + 64
+    buf = np.zeros(A["phi_bin"].size + 2 * pad, dtype=np.uint8)
+    A.a["phi_bin"] = buf[pad:pad + A["phi_bin"].size].reshape(A["phi_bin"].shape)
+    A.rec.phi_bin = A.a["phi_bin"].ctypes.data
+    # its find() recurses along parent chains, which grow long at 512^3
+    resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+    o = case_options(name, dt)
+    lib = op.ref_lib(dt)
+    op.set_omp_threads(T)
+    libc = C.CDLL("libc.so.6")
+    libc.fopen.restype = C.c_void_p
+    libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
+    libc.fclose.argtypes = [C.c_void_p]
+    work = tempfile.mkdtemp(prefix=f"lsmref512_{name}_")
+    cwd = os.getcwd()
+    out = {"threads": T, "n": N, "seed": CASES[name]["seed"]}
+    os.chdir(work)
+    try:
+        path = os.path.join(work, "drain.log")
+        fp = C.c_void_p(libc.fopen(path.encode(), b"w"))
+        w0 = time.perf_counter()
+        lib.drain_model3d(C.byref(o), A.ptr(), C.byref(g), fp)
+        out["wall_s"] = time.perf_counter() - w0
+        libc.fclose(fp)
+        phi = A["phi"]
+        f = fill(g)
+        out["idx"] = band_sample(phi, g, CASES[name]["seed"])
+        out["val_drain"] = phi.ravel()[out["idx"]]
+        out["sat_drain"] = saturation(phi, A["mask"], g)
+        out["plane_sum_drain"] = phi[f].sum(axis=(1, 2))
+        out["nneg_drain"] = int((phi[f] < 0).sum())
+        out["nw_trap_drain"] = int((A["mask_nw"][f] > 0).sum())
+        out["w_trap_drain"] = int((A["mask_w"][f] > 0).sum())
+        out["iface_idx"] = interface_voxels(phi, g).astype(np.int64)
+        out["outer_drain"] = sum(1 for line in open(path) if "nw phase satn" in line)
+        out["overlap"] = float(o.overlap)
+        print(f"{name}: S_nw {out['sat_drain']:.8f}, outer {out['outer_drain']}, "
+              f"wall {out['wall_s']:.0f} s", flush=True)
+    finally:
+        os.chdir(cwd)
+        shutil.rmtree(work, ignore_errors=True)
+    np.savez_compressed(out_path(name), **out)
+    print("wrote", out_path(name))
+
+
+if __name__ == "__main__":
+    main()
