@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cstdint>
 
+#include "lsm_coeff.h"
 #include "lsm_kernels.h"
 
 namespace lsm {
@@ -361,41 +362,28 @@ __device__ __forceinline__ int conn_value(const int* L, const int* T, int64_t id
   return t >= 0 ? L[t] : 0;
 }
 
-// phi = merge(phi, mask_nw); bin = phi < 0, plus the inlet plane forced to 1
+// phi = merge(phi, mask_nw); bin = phi < 0 (force_bin_kernel then sets the
+// inlet voxels)
 __global__ void trap_nw_bin_kernel(TrapArgs a) {
-  const Box bx = a.bx;
-  int64_t n = bx.size();
-  int ci = (int)(0.5 * (bx.ilo + bx.ihi)), ck = (int)(0.5 * (bx.klo + bx.khi));
-  int cj = (int)(0.5 * (bx.jlo + bx.jhi));
+  const int64_t n = a.bx.size();
   for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n;
        l += (int64_t)gridDim.x * blockDim.x) {
-    real mn = a.mask_nw[l];
     real p = a.phi[l];
-    if (mn > real(0)) { p = -mn; a.phi[l] = p; }
-    uint8_t b = p < real(0) ? 1 : 0;
-    int64_t t = l / bx.nx;
-    int i = (int)(l - t * bx.nx), j = (int)(t % bx.ny), k = (int)(t / bx.ny);
-    if (a.dims2) {
-      // 2D: the centre point, or the whole ghost column i = ilo - 1
-      if (a.center_inlet) {
-        if (i == ci && j == cj) b = 1;
-      } else if (i == bx.ilo - 1) {
-        b = 1;
-      }
-    } else if (a.center_inlet) {
-      if (i == ci && k == ck) b = 1;
-    } else if (i == bx.ilo - 1 && j >= bx.jlo && j <= bx.jhi && k >= bx.klo && k <= bx.khi) {
-      b = 1;
+    if (a.impose_mask) {  // phi = max(phi, mask) first (the var path's imposition)
+      const real m = a.impose_mask[l];
+      if (m > p) p = m;
     }
-    a.bin[l] = b;
+    real mn = a.mask_nw[l];
+    if (mn > real(0)) p = -mn;
+    if (a.impose_mask || mn > real(0)) a.phi[l] = p;
+    a.bin[l] = p < real(0) ? 1 : 0;
   }
 }
 
 // mask_nw from the non-wetting labels; s = max(-phi, mask, mask_nw);
-// bin = s < 0, plus the outlet plane(s) forced to 1
+// bin = s < 0 (force_bin_kernel then sets the outlet voxels)
 __global__ void trap_w_bin_kernel(TrapArgs a) {
-  const Box bx = a.bx;
-  int64_t n = bx.size();
+  const int64_t n = a.bx.size();
   int main_val = conn_value(a.labels, a.troot, a.nw_ind);
   for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n;
        l += (int64_t)gridDim.x * blockDim.x) {
@@ -409,23 +397,84 @@ __global__ void trap_w_bin_kernel(TrapArgs a) {
     s = (m > s) ? m : s;
     s = (mn > s) ? mn : s;
     a.s[l] = s;
-    uint8_t b = s < real(0) ? 1 : 0;
-    int64_t t = l / bx.nx;
-    int i = (int)(l - t * bx.nx), j = (int)(t % bx.ny), k = (int)(t / bx.ny);
-    if (a.dims2) {
-      // 2D: the four ghost lines next to the fill box, or (as the reference
-      // does) the fill column i = ilo + 1
-      if (a.center_inlet) {
-        if (i == bx.ilo - 1 || i == bx.ihi + 1 || j == bx.jlo - 1 || j == bx.jhi + 1) b = 1;
-      } else if (i == bx.ilo + 1 && j >= bx.jlo && j <= bx.jhi) {
-        b = 1;
-      }
-    } else if (a.center_inlet) {
-      if (k == bx.klo - 1 || k == bx.khi + 1 || i == bx.ilo - 1 || i == bx.ihi + 1) b = 1;
-    } else if (i == bx.ihi + 1 && j >= bx.jlo && j <= bx.jhi && k >= bx.klo && k <= bx.khi) {
-      b = 1;
+    a.bin[l] = s < real(0) ? 1 : 0;
+  }
+}
+
+// The voxels the reference forces into the binary field before labelling
+// (trapComponents.c:185-194 inlet, :290-299 outlet, :196-211 and :297-337 in
+// 2D): up to four boxes of the ghost box, inclusive bounds.
+struct ForceSet {
+  int n;
+  int lo[4][3], hi[4][3];
+  void add(const Box& bx, int i0, int i1, int j0, int j1, int k0, int k1) {
+    i0 = std::max(i0, 0); j0 = std::max(j0, 0); k0 = std::max(k0, 0);
+    i1 = std::min(i1, bx.nx - 1); j1 = std::min(j1, bx.ny - 1); k1 = std::min(k1, bx.nz - 1);
+    if (i0 > i1 || j0 > j1 || k0 > k1) return;
+    lo[n][0] = i0; lo[n][1] = j0; lo[n][2] = k0;
+    hi[n][0] = i1; hi[n][1] = j1; hi[n][2] = k1;
+    ++n;
+  }
+};
+
+// non-wetting phase: the inlet (the ghost plane i = ilo - 1 over the fill
+// box's j, k; 2D the whole ghost column; with a centre inlet the line
+// i = ci, k = ck, in 2D the point i = ci, j = cj)
+static ForceSet nw_force(const TrapArgs& a) {
+  const Box& b = a.bx;
+  const int ci = (int)(0.5 * (b.ilo + b.ihi)), cj = (int)(0.5 * (b.jlo + b.jhi));
+  const int ck = (int)(0.5 * (b.klo + b.khi));
+  const int all = 1 << 30;
+  ForceSet f{};
+  if (a.dims2) {
+    if (a.center_inlet) f.add(b, ci, ci, cj, cj, 0, all);
+    else f.add(b, b.ilo - 1, b.ilo - 1, 0, all, 0, all);
+  } else if (a.center_inlet) {
+    f.add(b, ci, ci, 0, all, ck, ck);
+  } else {
+    f.add(b, b.ilo - 1, b.ilo - 1, b.jlo, b.jhi, b.klo, b.khi);
+  }
+  return f;
+}
+
+// wetting phase: the outlet (the ghost plane i = ihi + 1 over the fill box's
+// j, k; with a centre inlet the ghost planes k = klo - 1, k = khi + 1,
+// i = ilo - 1, i = ihi + 1; 2D the fill column i = ilo + 1, or with a centre
+// inlet the four ghost lines)
+static ForceSet w_force(const TrapArgs& a) {
+  const Box& b = a.bx;
+  const int all = 1 << 30;
+  ForceSet f{};
+  if (a.dims2) {
+    if (a.center_inlet) {
+      f.add(b, b.ilo - 1, b.ilo - 1, 0, all, 0, all);
+      f.add(b, b.ihi + 1, b.ihi + 1, 0, all, 0, all);
+      f.add(b, 0, all, b.jlo - 1, b.jlo - 1, 0, all);
+      f.add(b, 0, all, b.jhi + 1, b.jhi + 1, 0, all);
+    } else {
+      f.add(b, b.ilo + 1, b.ilo + 1, b.jlo, b.jhi, 0, all);
     }
-    a.bin[l] = b;
+  } else if (a.center_inlet) {
+    f.add(b, 0, all, 0, all, b.klo - 1, b.klo - 1);
+    f.add(b, 0, all, 0, all, b.khi + 1, b.khi + 1);
+    f.add(b, b.ilo - 1, b.ilo - 1, 0, all, 0, all);
+    f.add(b, b.ihi + 1, b.ihi + 1, 0, all, 0, all);
+  } else {
+    f.add(b, b.ihi + 1, b.ihi + 1, b.jlo, b.jhi, b.klo, b.khi);
+  }
+  return f;
+}
+
+__global__ void force_bin_kernel(uint8_t* bin, Box bx, ForceSet f) {
+  for (int q = 0; q < f.n; ++q) {
+    const unsigned nx = (unsigned)(f.hi[q][0] - f.lo[q][0] + 1);
+    const unsigned ny = (unsigned)(f.hi[q][1] - f.lo[q][1] + 1);
+    const unsigned cnt = nx * ny * (unsigned)(f.hi[q][2] - f.lo[q][2] + 1);
+    for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < cnt; t += gridDim.x * blockDim.x) {
+      const unsigned r = t / nx, kk = r / ny;
+      bin[bx.at(f.lo[q][0] + (int)(t - r * nx), f.lo[q][1] + (int)(r - kk * ny),
+                f.lo[q][2] + (int)kk)] = 1;
+    }
   }
 }
 
@@ -444,10 +493,12 @@ __global__ void trap_w_mask_kernel(TrapArgs a) {
 hipError_t launch_trap_components(const TrapArgs& a, int conn, hipStream_t st) {
   int64_t n = a.bx.size();
   hipLaunchKernelGGL(trap_nw_bin_kernel, dim3(grid_for(n)), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(force_bin_kernel, dim3(256), dim3(256), 0, st, a.bin, a.bx, nw_force(a));
   hipError_t e = launch_label_components(a.bx, a.bin, a.labels, a.troot, conn, st, a.roots,
                                          a.roots_cap, false);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(trap_w_bin_kernel, dim3(grid_for(n)), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(force_bin_kernel, dim3(256), dim3(256), 0, st, a.bin, a.bx, w_force(a));
   e = launch_label_components(a.bx, a.bin, a.labels, a.troot, conn, st, a.roots, a.roots_cap,
                               a.full_labels);
   if (e != hipSuccess) return e;

@@ -23,6 +23,63 @@ struct FillIter {
   }
 };
 
+// Row-wise iteration for grid-stride kernels launched with 64 x 4 blocks:
+// each wave takes whole x rows (j, k) of the box, b * 4 + ty, then
+// + gridDim.x * 4, ..., and its lanes the row's voxels 64 apart, so every
+// wave reads contiguous memory and the row's (j, k) costs one 32-bit
+// division per row instead of 64-bit divisions per voxel.  b is the block's
+// XCD-contiguous number (xcd_tile): the blocks of one XCD take neighbouring
+// rows and planes, so a stencil's j +- 1 and k +- 1 rows are read through
+// one L2.  FILL: the fill box; otherwise the whole ghost box.
+// f(idx, i, j, k).
+constexpr int ROW_TX = 64, ROW_TY = 4;
+
+template <bool FILL>
+__host__ __device__ inline void row_extent(const Box& b, int& i0, int& nxr, int& j0, int& nyr,
+                                           int& k0, int& nzr) {
+  if (FILL) {
+    i0 = b.ilo; nxr = b.ihi - b.ilo + 1;
+    j0 = b.jlo; nyr = b.jhi - b.jlo + 1;
+    k0 = b.klo; nzr = b.khi - b.klo + 1;
+  } else {
+    i0 = 0; nxr = b.nx; j0 = 0; nyr = b.ny; k0 = 0; nzr = b.nz;
+  }
+  if (nxr <= 0 || nyr <= 0 || nzr <= 0) nxr = nyr = nzr = 0;
+}
+
+// rows of the box (0 when it is empty)
+template <bool FILL>
+inline int64_t box_rows(const Box& b) {
+  int i0, nxr, j0, nyr, k0, nzr;
+  row_extent<FILL>(b, i0, nxr, j0, nyr, k0, nzr);
+  return (int64_t)nyr * nzr;
+}
+
+// blocks of a row-wise launch: one block per 4 rows, at most cap
+template <bool FILL>
+inline int row_blocks(const Box& b, int cap) {
+  int64_t g = (box_rows<FILL>(b) + ROW_TY - 1) / ROW_TY;
+  return (int)(g < 1 ? 1 : (g > cap ? cap : g));
+}
+
+template <bool FILL, typename F>
+__device__ __forceinline__ void for_rows(const Box& b, F&& f) {
+  int i0, nxr, j0, nyr, k0, nzr;
+  row_extent<FILL>(b, i0, nxr, j0, nyr, k0, nzr);
+  const unsigned nrows = (unsigned)nyr * (unsigned)nzr;
+  const int64_t plane = (int64_t)b.nx * b.ny;
+  const unsigned blk = (unsigned)xcd_tile(blockIdx.x, gridDim.x);
+  for (unsigned r = blk * ROW_TY + threadIdx.y; r < nrows; r += gridDim.x * ROW_TY) {
+    const unsigned kk = r / (unsigned)nyr, jj = r - kk * (unsigned)nyr;
+    const int j = j0 + (int)jj, k = k0 + (int)kk;
+    const int64_t base = (int64_t)k * plane + (int64_t)j * b.nx;
+    for (int x = threadIdx.x; x < nxr; x += ROW_TX) {
+      const int i = i0 + x;
+      f(base + i, i, j, k);
+    }
+  }
+}
+
 // Heaviside weights, |grad mask| and the smoothed sign of the mask of one
 // voxel (reference: qss_spatial_derivatives3d.f:13-87,178-298).
 __device__ __forceinline__ void coeff_common(real m, real mx, real my, real mz, real dx, real& h1,

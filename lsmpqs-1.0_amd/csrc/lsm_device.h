@@ -31,6 +31,17 @@ struct Box {
   }
 };
 
+// Workgroups are handed to the eight XCDs round-robin by linear id, and each
+// XCD has its own L2.  Block b is given tile number xcd * share + b / 8 (the
+// shares differ by at most one when the count is not a multiple of 8), so an
+// XCD works through a contiguous run of tiles, x fastest, and the halo rows
+// and columns neighbouring tiles share are fetched into one L2 instead of
+// several.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  const int q = nb >> 3, r = nb & 7, xcd = b & 7, slot = b >> 3;
+  return xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
+}
+
 // HJ-ENO2 one-sided derivatives from p[0..6] = phi(-3..+3) (:386-456)
 template <typename T>
 __device__ __forceinline__ void eno2(const T* p, T inv_d, T& plus, T& minus) {

@@ -54,16 +54,39 @@ static int rhs_scheme(const Options* o, bool var_path) {
 }
 
 // ---------------------------------------------------------------------------
+// Outer reductions requested from reinit_dev: volume of {f < 0} and the
+// local max-norm against prev, into s->red (see launch_outer_reduce)
+struct ReduceReq {
+  const real* prev;
+  real zone, eps, dV;
+};
+
 // Reinitialization with the subcell fix (reference:
 // src/reinitialization/reinitialize_top.c:297-377).  `f` is replaced by the
 // result; A, B and S serve as next-iterate, phi0 copy and distance buffers.
-static void reinit_dev(Session* s, real*& f, const Grid* g, const Options* o) {
+// With `red`, the outer reductions of the result are left in s->red (fused
+// into the single-iteration pass when there is one).
+static void reinit_dev(Session* s, real*& f, const Grid* g, const Options* o,
+                       const ReduceReq* red = nullptr) {
   const Box& bx = s->bx;
   real t_r = 0, dt_r = real(0.5) * g->dx[0];
-  if (!(t_r < o->tmax_r)) return;
+  auto reduce_after = [&]() {
+    if (red)
+      DCHECK(launch_outer_reduce(bx, f, red->prev, red->zone, red->eps, red->dV, s->part, s->red,
+                                 s->st));
+  };
+  if (!(t_r < o->tmax_r)) {
+    reduce_after();
+    return;
+  }
   if (!(dt_r < o->tmax_r)) {
     // one iteration (tmax_r = 0.4 at dx = 1): data = phi0, fused first step
-    DCHECK(launch_reinit_first_step(bx, s->A, f, dt_r, g->dx[0], g->dx[1], g->dx[2], s->st));
+    if (red)
+      DCHECK(launch_reinit_first_step_reduce(bx, s->A, f, dt_r, g->dx[0], g->dx[1], g->dx[2],
+                                             red->prev, red->zone, red->eps, red->dV, s->part,
+                                             s->red, s->st));
+    else
+      DCHECK(launch_reinit_first_step(bx, s->A, f, dt_r, g->dx[0], g->dx[1], g->dx[2], s->st));
     BcArgs ba{};
     ba.bx = bx; ba.src = s->A; ba.dst = s->A;
     DCHECK(launch_bc(ba, false, s->st));
@@ -82,6 +105,7 @@ static void reinit_dev(Session* s, real*& f, const Grid* g, const Options* o) {
     std::swap(f, s->A);
     t_r = t_r + dt_r;
   }
+  reduce_after();
 }
 
 // Smoothed-sign reinitialization of f (reference:
@@ -112,9 +136,12 @@ static void reinit_medium_dev(Session* s, real*& f, const Grid* g, const Options
 // Trapped-phase masks (reference: src/connectivity/trapComponents.c:156-375)
 // full_labels: leave the wetting-phase labels resolved at every voxel (for
 // callers that read them back); otherwise only tile roots carry them
-static void trap_dev(Session* s, const Grid* g, const Options* o, bool full_labels = false) {
+// impose: phi = max(phi, impose) first, fused into the first trapping pass
+static void trap_dev(Session* s, const Grid* g, const Options* o, bool full_labels = false,
+                     const real* impose = nullptr) {
   ensure_labels(s);
   TrapArgs ta{};
+  ta.impose_mask = impose;
   ta.bx = s->bx;
   ta.phi = s->phi; ta.mask = s->mask; ta.mask_nw = s->mask_nw; ta.mask_w = s->mask_w;
   ta.s = s->S; ta.bin = s->bin; ta.labels = s->labels; ta.troot = s->troot;
@@ -231,14 +258,17 @@ StepArgs make_step_args(Session* s, const StepCfg& c) {
   }
   a.stage[0].in = s->phi; a.stage[0].un = s->phi; a.stage[0].out = s->A; a.stage[0].pre = nullptr;
   a.bc[0].src = s->A; a.bc[0].dst = s->A;
+  // the new state goes to phi (in place, pointwise) or, with s->redirect, to
+  // phi_prev's buffer
+  real* res = s->redirect ? s->phi_prev : s->phi;
   if (c.order == 2) {
-    a.stage[1].in = s->A; a.stage[1].un = s->phi; a.stage[1].out = s->phi; a.stage[1].pre = s->B;
-    a.bc[1].src = s->B; a.bc[1].dst = s->phi; a.bc[1].floor = s->floor;
+    a.stage[1].in = s->A; a.stage[1].un = s->phi; a.stage[1].out = res; a.stage[1].pre = s->B;
+    a.bc[1].src = s->B; a.bc[1].dst = res; a.bc[1].floor = s->floor;
   } else {
     a.stage[1].in = s->A; a.stage[1].un = s->phi; a.stage[1].out = s->B; a.stage[1].pre = nullptr;
     a.bc[1].src = s->B; a.bc[1].dst = s->B;
-    a.stage[2].in = s->B; a.stage[2].un = s->phi; a.stage[2].out = s->phi; a.stage[2].pre = s->A;
-    a.bc[2].src = s->A; a.bc[2].dst = s->phi; a.bc[2].floor = s->floor;
+    a.stage[2].in = s->B; a.stage[2].un = s->phi; a.stage[2].out = res; a.stage[2].pre = s->A;
+    a.bc[2].src = s->A; a.bc[2].dst = res; a.bc[2].floor = s->floor;
   }
   return a;
 }
@@ -307,6 +337,10 @@ static real inner_loop(Session* s, const StepCfg& c, real tplot, real* last_dt) 
       DCHECK(hipEventSynchronize(s->ev_dt));
       dt = *s->h_dt;
     }
+    if (s->redirect) {  // the step wrote phi_prev's buffer: it is the new phi
+      std::swap(s->phi, s->phi_prev);
+      s->redirect = false;
+    }
     if (!(dt > real(0)) || !isfinite((double)dt))
       throw std::runtime_error("non-positive or non-finite time step");
     profile_collect(s, c.order);
@@ -318,10 +352,13 @@ static real inner_loop(Session* s, const StepCfg& c, real tplot, real* last_dt) 
 }
 
 // outer reductions: out[0] = volume of {f < 0}, out[1] = local max-norm diff vs f2
-static void outer_reduce(Session* s, const real* f, const real* f2, real zone, real eps, real dV) {
-  DCHECK(launch_outer_reduce(s->bx, f, f2, zone, eps, dV, s->part, s->red, s->st));
+static void read_reduce(Session* s) {
   DCHECK(hipMemcpyAsync(s->h_red, s->red, 2 * sizeof(double), hipMemcpyDeviceToHost, s->st));
   sync(s);
+}
+static void outer_reduce(Session* s, const real* f, const real* f2, real zone, real eps, real dV) {
+  DCHECK(launch_outer_reduce(s->bx, f, f2, zone, eps, dV, s->part, s->red, s->st));
+  read_reduce(s);
 }
 
 // Outer loop of constCurvModel3d / constCurvModel3dNoVar
@@ -333,6 +370,7 @@ static real ccm_dev(Session* s, Options* o, const Grid* g, FILE* fp, bool var, r
   real dV = g->dx[0] * g->dx[1] * g->dx[2];
   double max_abs_err = 100, vol_phi, vol_pore, satn, satn_old;
   real t = 0;
+  s->redirect = false;
   StepCfg c = make_cfg(s, o, g, var, a_const);
   if (s->profile && s->stage_ev.empty()) {
     s->stage_ev.resize(8);
@@ -348,10 +386,17 @@ static real ccm_dev(Session* s, Options* o, const Grid* g, FILE* fp, bool var, r
   std::unique_ptr<StepWriter> ckpt;
   while ((t < o->tmax) && (max_abs_err > (double)o->eps_stop)) {
     s->tm.outer_steps++;
-    DCHECK(hipMemcpyAsync(s->phi_prev, s->phi, bytes(s), hipMemcpyDeviceToDevice, s->st));
+    // phi_prev = phi: the first inner step writes its result to phi_prev's
+    // buffer and the two are swapped (RK1 copies)
+    if (c.order >= 2) s->redirect = true;
+    else DCHECK(hipMemcpyAsync(s->phi_prev, s->phi, bytes(s), hipMemcpyDeviceToDevice, s->st));
     DCHECK(hipEventRecord(s->ev_a, s->st));
     real last_dt = 0;
     real dt_sub = inner_loop(s, c, o->tplot, &last_dt);
+    if (s->redirect) {  // no inner step ran
+      s->redirect = false;
+      DCHECK(hipMemcpyAsync(s->phi_prev, s->phi, bytes(s), hipMemcpyDeviceToDevice, s->st));
+    }
     DCHECK(hipEventRecord(s->ev_b, s->st));
     sync(s);
     float ms = 0;
@@ -360,9 +405,10 @@ static real ccm_dev(Session* s, Options* o, const Grid* g, FILE* fp, bool var, r
     if (fp) fprintf(fp, "1 GPU: Level set time = %lf\n", ms * 1e-3);
     const auto t3 = std::chrono::steady_clock::now();
     t += dt_sub;
-    if (var) DCHECK(launch_impose(s->phi, s->mask, s->phi, 0, s->n, s->st));
+    if (var && !o->check_connectivity)
+      DCHECK(launch_impose(s->phi, s->mask, s->phi, 0, s->n, s->st));
     if (o->check_connectivity) {
-      trap_dev(s, g, o);
+      trap_dev(s, g, o, false, var ? s->mask : nullptr);
       reinit_dev(s, s->mask_w, g, o);
       reinit_dev(s, s->mask_nw, g, o);
       if (var && o->conserve_imbibe) {
@@ -370,13 +416,14 @@ static real ccm_dev(Session* s, Options* o, const Grid* g, FILE* fp, bool var, r
                                     g->dx[0], s->st));
         build_stale(s, g);
       }
-      DCHECK(launch_merge(s->phi, s->mask_nw, s->n, s->st));
-      update_floor(s, c);
+      DCHECK(launch_merge_floor(s->phi, s->floor, s->mask, s->mask_w, s->mask_nw, c.overlap,
+                                s->n, s->st));
     }
     if (fp) fprintf(fp, "Reinitializing....");
-    reinit_dev(s, s->phi, g, o);
+    const ReduceReq rq{s->phi_prev, o->err_check_zone, eps, dV};
+    reinit_dev(s, s->phi, g, o, &rq);
     if (fp) fprintf(fp, "Reinitialized\n");
-    outer_reduce(s, s->phi, s->phi_prev, o->err_check_zone, eps, dV);
+    read_reduce(s);
     vol_phi = s->h_red[0];
     max_abs_err = s->h_red[1];
     satn_old = satn;
@@ -830,6 +877,7 @@ int lsmpqs_bench_inner_steps(Options* o, QSS_DataArrays* p, Grid* g, int nsteps,
     for (auto& e : s->stage_ev) DCHECK(hipEventCreate(&e));
   }
   s->tm = Timers();
+  s->redirect = false;
   StepCfg c = make_cfg(s, o, g, var, a);
   update_floor(s, c);
   DCHECK(hipEventRecord(s->ev_a, s->st));

@@ -177,6 +177,17 @@ hipError_t launch_reinit_rhs(const Box& bx, real* rhs, const real* data, const r
 hipError_t launch_reinit_first_step(const Box& bx, real* next, const real* phi0, real dt_r,
                                     real dx, real dy, real dz, hipStream_t st);
 
+// launch_reinit_first_step plus the outer reductions of its result (as
+// launch_outer_reduce(bx, next, prev, zone, eps, dV, ...), same sums)
+hipError_t launch_reinit_first_step_reduce(const Box& bx, real* next, const real* phi0, real dt_r,
+                                           real dx, real dy, real dz, const real* prev, real zone,
+                                           real eps, real dV, double* partials, double* out,
+                                           hipStream_t st);
+// phi = -mask_nw where mask_nw > 0, and the imposition floor with the
+// trapped-phase masks (launch_merge then launch_floor with conn = 1), one pass
+hipError_t launch_merge_floor(real* phi, real* floor, const real* m, const real* mw,
+                              const real* mnw, real overlap, int64_t n, hipStream_t st);
+
 // Coefficient fields of the variable-coefficient path.
 hipError_t launch_central_grad(const Box& bx, real* gx, real* gy, real* gz, const real* f,
                                real dx, real dy, real dz, hipStream_t st);
@@ -216,6 +227,7 @@ struct TrapArgs {
   int center_inlet;
   real fill;             // -20 dx
   int dims2;             // 2D grid on a one-plane box: the 2D inlet/outlet rules
+  const real* impose_mask;  // not NULL: phi = max(phi, impose_mask) first, in the same pass
 };
 hipError_t launch_trap_components(const TrapArgs& a, int connectivity, hipStream_t st);
 // labels: largest linear index of the component on foreground, -1 on

@@ -55,6 +55,11 @@ struct Session {
   hipEvent_t ev_dt = nullptr, ev_a = nullptr, ev_b = nullptr;
   std::vector<hipEvent_t> stage_ev;  // timing pairs when profiling is on
   bool profile = false;
+  // the next inner step writes its result into phi_prev's buffer, after
+  // which phi and phi_prev are swapped: phi_prev keeps the state before the
+  // step without a copy (RK orders 2 and 3, whose last stage can write to a
+  // buffer other than its u^n)
+  bool redirect = false;
   Timers tm;
 };
 

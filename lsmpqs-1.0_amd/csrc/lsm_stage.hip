@@ -117,17 +117,6 @@ __device__ __forceinline__ HaloSlot halo_slot(const Box& bx, int i0, int j0, int
   return h;
 }
 
-// Workgroups are handed to the eight XCDs round-robin by linear id, and each
-// XCD has its own L2.  Block b is given tile number xcd * share + b / 8 (the
-// shares differ by at most one when the count is not a multiple of 8), so an
-// XCD works through a contiguous run of tiles, x fastest, and the halo rows
-// and columns neighbouring tiles share are fetched into one L2 instead of
-// several.
-__device__ __forceinline__ int xcd_tile(int b, int nb) {
-  const int q = nb >> 3, r = nb & 7, xcd = b & 7, slot = b >> 3;
-  return xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
-}
-
 // Pointwise inputs of one voxel: coefficients (variable path), u^n (stages
 // 2 and 3), imposition floor (last stage).
 template <typename T>

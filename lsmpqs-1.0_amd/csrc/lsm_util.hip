@@ -90,10 +90,6 @@ hipError_t launch_fill(real* dst, real v, int64_t n, hipStream_t st) {
   return hipGetLastError();
 }
 
-// ---------------------------------------------------------------------------
-// Fill-box iteration helper: flat fill index -> (i,j,k)
-
-
 constexpr int RED_BLOCKS = REDUCE_BLOCKS;
 int reduce_partials() { return 2 * RED_BLOCKS; }
 
@@ -113,14 +109,10 @@ __global__ __launch_bounds__(256) void outer_reduce_kernel(Box bx, const real* f
                                                            real zone, real eps, real dV,
                                                            double* part) {
   const real pi = real(3.14159265358979323846), one_over_pi = real(0.31830988618379);
-  FillIter it(bx);
   double vol = 0.0;
   real mx = real(-1);
-  for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < it.n;
-       f += (int64_t)gridDim.x * blockDim.x) {
-    int i, j, k;
-    it.ijk(f, i, j, k);
-    int64_t idx = bx.at(i, j, k);
+  for_rows<true>(bx, [&](int64_t idx, int i, int j, int k) {
+    (void)i; (void)j; (void)k;
     real p = f1[idx];
     real q = p / eps;
     if (p < -eps) vol += (double)dV;
@@ -129,15 +121,15 @@ __global__ __launch_bounds__(256) void outer_reduce_kernel(Box bx, const real* f
       real d = fabs(p - f2[idx]);
       if (d > mx) mx = d;
     }
-  }
-  __shared__ double s_sum[4], s_max[4];
+  });
+  __shared__ double s_sum[ROW_TY], s_max[ROW_TY];
   double ws = wave_sum(vol), wm = wave_max((double)mx);
-  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int lane = threadIdx.x, wid = threadIdx.y;  // 64 x 4 blocks: one wave per y
   if (lane == 0) { s_sum[wid] = ws; s_max[wid] = wm; }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (lane == 0 && wid == 0) {
     double a = 0.0, m = -1.0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) { a += s_sum[w]; m = fmax(m, s_max[w]); }
+    for (int w = 0; w < ROW_TY; ++w) { a += s_sum[w]; m = fmax(m, s_max[w]); }
     part[blockIdx.x] = a;
     part[RED_BLOCKS + blockIdx.x] = m;
   }
@@ -171,11 +163,9 @@ __global__ __launch_bounds__(256) void finish_reduce_kernel(const double* part, 
 
 hipError_t launch_outer_reduce(const Box& bx, const real* f1, const real* f2, real zone, real eps,
                                real dV, double* part, double* out, hipStream_t st) {
-  FillIter it(bx);
-  int nb = (int)std::min<int64_t>((it.n + 255) / 256, RED_BLOCKS);
-  if (nb < 1) nb = 1;
-  hipLaunchKernelGGL(outer_reduce_kernel, dim3(nb), dim3(256), 0, st, bx, f1, f2, zone, eps, dV,
-                     part);
+  const int nb = row_blocks<true>(bx, RED_BLOCKS);
+  hipLaunchKernelGGL(outer_reduce_kernel, dim3(nb), dim3(ROW_TX, ROW_TY), 0, st, bx, f1, f2, zone,
+                     eps, dV, part);
   hipLaunchKernelGGL(finish_reduce_kernel, dim3(1), dim3(256), 0, st, part, nb, out, 0);
   return hipGetLastError();
 }
@@ -184,26 +174,22 @@ hipError_t launch_outer_reduce(const Box& bx, const real* f1, const real* f2, re
 __global__ __launch_bounds__(256) void coeff_reduce_kernel(Box bx, const real* b, const real* vx,
                                                            const real* vy, const real* vz,
                                                            real inv, double* part) {
-  FillIter it(bx);
   real bm = real(-1), um = real(-1);
-  for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < it.n;
-       f += (int64_t)gridDim.x * blockDim.x) {
-    int i, j, k;
-    it.ijk(f, i, j, k);
-    int64_t idx = bx.at(i, j, k);
+  for_rows<true>(bx, [&](int64_t idx, int i, int j, int k) {
+    (void)i; (void)j; (void)k;
     real bv = b[idx];
     if (bv > bm) bm = bv;
     real u = fabs(vx[idx]) * inv + fabs(vy[idx]) * inv + fabs(vz[idx]) * inv;
     if (u > um) um = u;
-  }
-  __shared__ double s0[4], s1[4];
+  });
+  __shared__ double s0[ROW_TY], s1[ROW_TY];
   double a = wave_max((double)bm), m = wave_max((double)um);
-  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int lane = threadIdx.x, wid = threadIdx.y;  // 64 x 4 blocks: one wave per y
   if (lane == 0) { s0[wid] = a; s1[wid] = m; }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (lane == 0 && wid == 0) {
     double x = s0[0], y = s1[0];
-    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) { x = fmax(x, s0[w]); y = fmax(y, s1[w]); }
+    for (int w = 1; w < ROW_TY; ++w) { x = fmax(x, s0[w]); y = fmax(y, s1[w]); }
     part[blockIdx.x] = x;
     part[RED_BLOCKS + blockIdx.x] = y;
   }
@@ -212,10 +198,9 @@ __global__ __launch_bounds__(256) void coeff_reduce_kernel(Box bx, const real* b
 hipError_t launch_coeff_reduce(const Box& bx, const real* b, const real* vx, const real* vy,
                                const real* vz, real inv, double* part, double* out,
                                hipStream_t st) {
-  FillIter it(bx);
-  int nb = (int)std::min<int64_t>((it.n + 255) / 256, RED_BLOCKS);
-  if (nb < 1) nb = 1;
-  hipLaunchKernelGGL(coeff_reduce_kernel, dim3(nb), dim3(256), 0, st, bx, b, vx, vy, vz, inv, part);
+  const int nb = row_blocks<true>(bx, RED_BLOCKS);
+  hipLaunchKernelGGL(coeff_reduce_kernel, dim3(nb), dim3(ROW_TX, ROW_TY), 0, st, bx, b, vx, vy, vz,
+                     inv, part);
   hipLaunchKernelGGL(finish_reduce_kernel, dim3(1), dim3(256), 0, st, part, nb, out, 1);
   return hipGetLastError();
 }
@@ -263,22 +248,17 @@ __device__ __forceinline__ real reinit_rhs(real c, const real* gp, const real* g
 
 __global__ void reinit_distance_kernel(Box bx, real* dist0, const real* phi0, real dx, real dy,
                                        real dz) {
-  FillIter it(bx);
   real flag = real(-1000.0) * tmax(tmax(dx, dy), dz);
-  for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < it.n;
-       f += (int64_t)gridDim.x * blockDim.x) {
-    int i, j, k;
-    it.ijk(f, i, j, k);
-    int64_t idx = bx.at(i, j, k);
+  for_rows<true>(bx, [&](int64_t idx, int i, int j, int k) {
+    (void)i; (void)j; (void)k;
     int64_t sy = bx.nx, sz = (int64_t)bx.nx * bx.ny;
     dist0[idx] = reinit_d0(phi0[idx], phi0[idx + 1], phi0[idx - 1], phi0[idx + sy],
                            phi0[idx - sy], phi0[idx + sz], phi0[idx - sz], flag);
-  }
+  });
 }
 hipError_t launch_reinit_distance(const Box& bx, real* dist0, const real* phi0, real dx, real dy,
                                   real dz, hipStream_t st) {
-  FillIter it(bx);
-  hipLaunchKernelGGL(reinit_distance_kernel, dim3(stream_blocks(it.n)), dim3(256), 0, st, bx,
+  hipLaunchKernelGGL(reinit_distance_kernel, dim3(row_blocks<true>(bx, 16384)), dim3(ROW_TX, ROW_TY), 0, st, bx,
                      dist0, phi0, dx, dy, dz);
   return hipGetLastError();
 }
@@ -287,16 +267,12 @@ hipError_t launch_reinit_distance(const Box& bx, real* dist0, const real* phi0, 
 // the first-order subcell-fix right-hand side)
 __global__ void reinit_step_kernel(Box bx, real* next, const real* data, const real* phi0,
                                    const real* dist0, real dt_r, real dx, real dy, real dz) {
-  FillIter it(bx);
   real max_dx = tmax(tmax(dx, dy), dz);
   real flag = real(-1000.0) * max_dx;
   real inv_dx = real(1.0) / dx, inv_dy = real(1.0) / dy, inv_dz = real(1.0) / dz;
   int64_t sy = bx.nx, sz = (int64_t)bx.nx * bx.ny;
-  for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < it.n;
-       f += (int64_t)gridDim.x * blockDim.x) {
-    int i, j, k;
-    it.ijk(f, i, j, k);
-    int64_t idx = bx.at(i, j, k);
+  for_rows<true>(bx, [&](int64_t idx, int i, int j, int k) {
+    (void)i; (void)j; (void)k;
     real c = data[idx];
     real gp[3], gm[3];
     gp[0] = (data[idx + 1] - c) * inv_dx;
@@ -306,13 +282,12 @@ __global__ void reinit_step_kernel(Box bx, real* next, const real* data, const r
     gp[2] = (data[idx + sz] - c) * inv_dz;
     gm[2] = (c - data[idx - sz]) * inv_dz;
     next[idx] = c + dt_r * reinit_rhs(c, gp, gm, phi0[idx], dist0[idx], flag, max_dx);
-  }
+  });
 }
 hipError_t launch_reinit_step(const Box& bx, real* next, const real* data, const real* phi0,
                               const real* dist0, real dt_r, real dx, real dy, real dz,
                               hipStream_t st) {
-  FillIter it(bx);
-  hipLaunchKernelGGL(reinit_step_kernel, dim3(stream_blocks(it.n)), dim3(256), 0, st, bx, next,
+  hipLaunchKernelGGL(reinit_step_kernel, dim3(row_blocks<true>(bx, 16384)), dim3(ROW_TX, ROW_TY), 0, st, bx, next,
                      data, phi0, dist0, dt_r, dx, dy, dz);
   return hipGetLastError();
 }
@@ -320,16 +295,12 @@ hipError_t launch_reinit_step(const Box& bx, real* next, const real* data, const
 // the right-hand side alone (qss3dComputeReinitializationEqnRHSSubcellFixOrder1)
 __global__ void reinit_rhs_kernel(Box bx, real* rhs, const real* data, const real* phi0,
                                   const real* dist0, real dx, real dy, real dz) {
-  FillIter it(bx);
   real max_dx = tmax(tmax(dx, dy), dz);
   real flag = real(-1000.0) * max_dx;
   real inv_dx = real(1.0) / dx, inv_dy = real(1.0) / dy, inv_dz = real(1.0) / dz;
   int64_t sy = bx.nx, sz = (int64_t)bx.nx * bx.ny;
-  for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < it.n;
-       f += (int64_t)gridDim.x * blockDim.x) {
-    int i, j, k;
-    it.ijk(f, i, j, k);
-    int64_t idx = bx.at(i, j, k);
+  for_rows<true>(bx, [&](int64_t idx, int i, int j, int k) {
+    (void)i; (void)j; (void)k;
     real c = data[idx];
     real gp[3], gm[3];
     gp[0] = (data[idx + 1] - c) * inv_dx;
@@ -339,12 +310,11 @@ __global__ void reinit_rhs_kernel(Box bx, real* rhs, const real* data, const rea
     gp[2] = (data[idx + sz] - c) * inv_dz;
     gm[2] = (c - data[idx - sz]) * inv_dz;
     rhs[idx] = reinit_rhs(c, gp, gm, phi0[idx], dist0[idx], flag, max_dx);
-  }
+  });
 }
 hipError_t launch_reinit_rhs(const Box& bx, real* rhs, const real* data, const real* phi0,
                              const real* dist0, real dx, real dy, real dz, hipStream_t st) {
-  FillIter it(bx);
-  hipLaunchKernelGGL(reinit_rhs_kernel, dim3(stream_blocks(it.n)), dim3(256), 0, st, bx, rhs,
+  hipLaunchKernelGGL(reinit_rhs_kernel, dim3(row_blocks<true>(bx, 16384)), dim3(ROW_TX, ROW_TY), 0, st, bx, rhs,
                      data, phi0, dist0, dx, dy, dz);
   return hipGetLastError();
 }
@@ -354,16 +324,12 @@ hipError_t launch_reinit_rhs(const Box& bx, real* rhs, const real* data, const r
 // distance kernel followed by the step kernel on that first iteration.
 __global__ void reinit_first_step_kernel(Box bx, real* next, const real* phi0, real dt_r, real dx,
                                          real dy, real dz) {
-  FillIter it(bx);
   real max_dx = tmax(tmax(dx, dy), dz);
   real flag = real(-1000.0) * max_dx;
   real inv_dx = real(1.0) / dx, inv_dy = real(1.0) / dy, inv_dz = real(1.0) / dz;
   int64_t sy = bx.nx, sz = (int64_t)bx.nx * bx.ny;
-  for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < it.n;
-       f += (int64_t)gridDim.x * blockDim.x) {
-    int i, j, k;
-    it.ijk(f, i, j, k);
-    int64_t idx = bx.at(i, j, k);
+  for_rows<true>(bx, [&](int64_t idx, int i, int j, int k) {
+    (void)i; (void)j; (void)k;
     real c = phi0[idx];
     real xp = phi0[idx + 1], xm = phi0[idx - 1];
     real yp = phi0[idx + sy], ym = phi0[idx - sy];
@@ -372,12 +338,89 @@ __global__ void reinit_first_step_kernel(Box bx, real* next, const real* phi0, r
     real gp[3] = {(xp - c) * inv_dx, (yp - c) * inv_dy, (zp - c) * inv_dz};
     real gm[3] = {(c - xm) * inv_dx, (c - ym) * inv_dy, (c - zm) * inv_dz};
     next[idx] = c + dt_r * reinit_rhs(c, gp, gm, c, d0, flag, max_dx);
+  });
+}
+// reinit_first_step_kernel fused with the outer reductions of its result:
+// per-block FP64 partials of the smoothed volume of {next < 0} and of the
+// local max-norm |next - prev| over |next| < zone, in the same order as
+// outer_reduce_kernel (same launch shape), so the sums are the same bits.
+__global__ __launch_bounds__(256) void reinit_first_step_reduce_kernel(
+    Box bx, real* next, const real* phi0, real dt_r, real dx, real dy, real dz, const real* prev,
+    real zone, real eps, real dV, double* part) {
+  const real pi = real(3.14159265358979323846), one_over_pi = real(0.31830988618379);
+  real max_dx = tmax(tmax(dx, dy), dz);
+  real flag = real(-1000.0) * max_dx;
+  real inv_dx = real(1.0) / dx, inv_dy = real(1.0) / dy, inv_dz = real(1.0) / dz;
+  int64_t sy = bx.nx, sz = (int64_t)bx.nx * bx.ny;
+  double vol = 0.0;
+  real mx = real(-1);
+  for_rows<true>(bx, [&](int64_t idx, int i, int j, int k) {
+    (void)i; (void)j; (void)k;
+    real c = phi0[idx];
+    real xp = phi0[idx + 1], xm = phi0[idx - 1];
+    real yp = phi0[idx + sy], ym = phi0[idx - sy];
+    real zp = phi0[idx + sz], zm = phi0[idx - sz];
+    real d0 = reinit_d0(c, xp, xm, yp, ym, zp, zm, flag);
+    real gp[3] = {(xp - c) * inv_dx, (yp - c) * inv_dy, (zp - c) * inv_dz};
+    real gm[3] = {(c - xm) * inv_dx, (c - ym) * inv_dy, (c - zm) * inv_dz};
+    const real p = c + dt_r * reinit_rhs(c, gp, gm, c, d0, flag, max_dx);
+    next[idx] = p;
+    real q = p / eps;
+    if (p < -eps) vol += (double)dV;
+    else if (p < eps) vol += (double)(real(0.5) * (real(1) - q - one_over_pi * sin(pi * q)) * dV);
+    if (fabs(p) < zone) {
+      real d = fabs(p - prev[idx]);
+      if (d > mx) mx = d;
+    }
+  });
+  __shared__ double s_sum[ROW_TY], s_max[ROW_TY];
+  double ws = wave_sum(vol), wm = wave_max((double)mx);
+  const int lane = threadIdx.x, wid = threadIdx.y;
+  if (lane == 0) { s_sum[wid] = ws; s_max[wid] = wm; }
+  __syncthreads();
+  if (lane == 0 && wid == 0) {
+    double a = 0.0, m = -1.0;
+    for (int w = 0; w < ROW_TY; ++w) { a += s_sum[w]; m = fmax(m, s_max[w]); }
+    part[blockIdx.x] = a;
+    part[RED_BLOCKS + blockIdx.x] = m;
   }
 }
+hipError_t launch_reinit_first_step_reduce(const Box& bx, real* next, const real* phi0, real dt_r,
+                                           real dx, real dy, real dz, const real* prev, real zone,
+                                           real eps, real dV, double* part, double* out,
+                                           hipStream_t st) {
+  const int nb = row_blocks<true>(bx, RED_BLOCKS);
+  hipLaunchKernelGGL(reinit_first_step_reduce_kernel, dim3(nb), dim3(ROW_TX, ROW_TY), 0, st, bx,
+                     next, phi0, dt_r, dx, dy, dz, prev, zone, eps, dV, part);
+  hipLaunchKernelGGL(finish_reduce_kernel, dim3(1), dim3(256), 0, st, part, nb, out, 0);
+  return hipGetLastError();
+}
+
+// merge and imposition floor in one pass: phi = -mask_nw where mask_nw > 0
+// (merge_kernel) and floor = max(mask - overlap, mask_w, mask_nw) (floor_kernel)
+__global__ void merge_floor_kernel(real* phi, real* f, const real* m, const real* mw,
+                                   const real* mnw, real ov, int64_t n) {
+  for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n;
+       l += (int64_t)gridDim.x * blockDim.x) {
+    const real b = mnw[l];
+    if (b > real(0)) phi[l] = -b;
+    real v = m[l] - ov;
+    real a = mw[l];
+    v = (a > v) ? a : v;
+    v = (b > v) ? b : v;
+    f[l] = v;
+  }
+}
+hipError_t launch_merge_floor(real* phi, real* floor, const real* m, const real* mw,
+                              const real* mnw, real overlap, int64_t n, hipStream_t st) {
+  hipLaunchKernelGGL(merge_floor_kernel, dim3(stream_blocks(n)), dim3(256), 0, st, phi, floor, m,
+                     mw, mnw, overlap, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_reinit_first_step(const Box& bx, real* next, const real* phi0, real dt_r,
                                     real dx, real dy, real dz, hipStream_t st) {
-  FillIter it(bx);
-  hipLaunchKernelGGL(reinit_first_step_kernel, dim3(stream_blocks(it.n)), dim3(256), 0, st, bx,
+  hipLaunchKernelGGL(reinit_first_step_kernel, dim3(row_blocks<true>(bx, 16384)), dim3(ROW_TX, ROW_TY), 0, st, bx,
                      next, phi0, dt_r, dx, dy, dz);
   return hipGetLastError();
 }
@@ -387,22 +430,17 @@ hipError_t launch_reinit_first_step(const Box& bx, real* next, const real* phi0,
 
 __global__ void central_grad_kernel(Box bx, real* gx, real* gy, real* gz, const real* f, real fx,
                                     real fy, real fz) {
-  FillIter it(bx);
   int64_t sy = bx.nx, sz = (int64_t)bx.nx * bx.ny;
-  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < it.n;
-       q += (int64_t)gridDim.x * blockDim.x) {
-    int i, j, k;
-    it.ijk(q, i, j, k);
-    int64_t idx = bx.at(i, j, k);
+  for_rows<true>(bx, [&](int64_t idx, int i, int j, int k) {
+    (void)i; (void)j; (void)k;
     gx[idx] = (f[idx + 1] - f[idx - 1]) * fx;
     gy[idx] = (f[idx + sy] - f[idx - sy]) * fy;
     gz[idx] = (f[idx + sz] - f[idx - sz]) * fz;
-  }
+  });
 }
 hipError_t launch_central_grad(const Box& bx, real* gx, real* gy, real* gz, const real* f,
                                real dx, real dy, real dz, hipStream_t st) {
-  FillIter it(bx);
-  hipLaunchKernelGGL(central_grad_kernel, dim3(stream_blocks(it.n)), dim3(256), 0, st, bx, gx, gy,
+  hipLaunchKernelGGL(central_grad_kernel, dim3(row_blocks<true>(bx, 16384)), dim3(ROW_TX, ROW_TY), 0, st, bx, gx, gy,
                      gz, f, real(0.5) / dx, real(0.5) / dy, real(0.5) / dz);
   return hipGetLastError();
 }
@@ -410,22 +448,17 @@ hipError_t launch_central_grad(const Box& bx, real* gx, real* gy, real* gz, cons
 
 __global__ void var_norm_kernel(Box bx, real* vn, const real* mask, const real* gx, const real* gy,
                                 const real* gz, real a0, real cos_theta, real dx) {
-  FillIter it(bx);
-  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < it.n;
-       q += (int64_t)gridDim.x * blockDim.x) {
-    int i, j, k;
-    it.ijk(q, i, j, k);
-    int64_t idx = bx.at(i, j, k);
+  for_rows<true>(bx, [&](int64_t idx, int i, int j, int k) {
+    (void)i; (void)j; (void)k;
     real h1, h2, ag, s;
     coeff_common(mask[idx], gx[idx], gy[idx], gz[idx], dx, h1, h2, ag, s);
     vn[idx] = h1 * a0 - s * h2 * real(1) * cos_theta * ag;
-  }
+  });
 }
 hipError_t launch_var_norm(const Box& bx, real* vn, const real* mask, const real* gx,
                            const real* gy, const real* gz, real a0, real cos_theta, real dx,
                            hipStream_t st) {
-  FillIter it(bx);
-  hipLaunchKernelGGL(var_norm_kernel, dim3(stream_blocks(it.n)), dim3(256), 0, st, bx, vn, mask,
+  hipLaunchKernelGGL(var_norm_kernel, dim3(row_blocks<true>(bx, 16384)), dim3(ROW_TX, ROW_TY), 0, st, bx, vn, mask,
                      gx, gy, gz, a0, cos_theta, dx);
   return hipGetLastError();
 }
@@ -433,12 +466,8 @@ hipError_t launch_var_norm(const Box& bx, real* vn, const real* mask, const real
 __global__ void var_curv_adv_kernel(Box bx, real* b, real* vx, real* vy, real* vz,
                                     const real* mask, const real* gx, const real* gy,
                                     const real* gz, real const_b, real dx) {
-  FillIter it(bx);
-  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < it.n;
-       q += (int64_t)gridDim.x * blockDim.x) {
-    int i, j, k;
-    it.ijk(q, i, j, k);
-    int64_t idx = bx.at(i, j, k);
+  for_rows<true>(bx, [&](int64_t idx, int i, int j, int k) {
+    (void)i; (void)j; (void)k;
     real h1, h2, ag, s;
     real mxv = gx[idx], myv = gy[idx], mzv = gz[idx];
     coeff_common(mask[idx], mxv, myv, mzv, dx, h1, h2, ag, s);
@@ -446,13 +475,12 @@ __global__ void var_curv_adv_kernel(Box bx, real* b, real* vx, real* vy, real* v
     vx[idx] = s * real(1) * mxv * h2;
     vy[idx] = s * real(1) * myv * h2;
     vz[idx] = s * real(1) * mzv * h2;
-  }
+  });
 }
 hipError_t launch_var_curv_adv(const Box& bx, real* b, real* vx, real* vy, real* vz,
                                const real* mask, const real* gx, const real* gy, const real* gz,
                                real const_b, real dx, hipStream_t st) {
-  FillIter it(bx);
-  hipLaunchKernelGGL(var_curv_adv_kernel, dim3(stream_blocks(it.n)), dim3(256), 0, st, bx, b, vx,
+  hipLaunchKernelGGL(var_curv_adv_kernel, dim3(row_blocks<true>(bx, 16384)), dim3(ROW_TX, ROW_TY), 0, st, bx, b, vx,
                      vy, vz, mask, gx, gy, gz, const_b, dx);
   return hipGetLastError();
 }
@@ -460,43 +488,33 @@ hipError_t launch_var_curv_adv(const Box& bx, real* b, real* vx, real* vy, real*
 // reference: src/utilities/qss_util3d.f:499-581
 __global__ void trap_velocity_kernel(Box bx, const real* phi_w, real* vn, real* b, real* vx,
                                      real* vy, real* vz, real eps_var) {
-  FillIter it(bx);
-  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < it.n;
-       q += (int64_t)gridDim.x * blockDim.x) {
-    int i, j, k;
-    it.ijk(q, i, j, k);
-    int64_t idx = bx.at(i, j, k);
+  for_rows<true>(bx, [&](int64_t idx, int i, int j, int k) {
+    (void)i; (void)j; (void)k;
     if (phi_w[idx] > eps_var) {
       vn[idx] = real(0); b[idx] = real(0);
       vx[idx] = real(0); vy[idx] = real(0); vz[idx] = real(0);
     }
-  }
+  });
 }
 hipError_t launch_trap_velocity(const Box& bx, const real* phi_w, real* vn, real* b, real* vx,
                                 real* vy, real* vz, real dx, hipStream_t st) {
-  FillIter it(bx);
-  hipLaunchKernelGGL(trap_velocity_kernel, dim3(stream_blocks(it.n)), dim3(256), 0, st, bx, phi_w,
+  hipLaunchKernelGGL(trap_velocity_kernel, dim3(row_blocks<true>(bx, 16384)), dim3(ROW_TX, ROW_TY), 0, st, bx, phi_w,
                      vn, b, vx, vy, vz, real(1.5) * dx);
   return hipGetLastError();
 }
 
 __global__ void collect_stale_kernel(Box bx, const real* vn, int64_t* list, int* count, int cap) {
-  FillIter it(bx);
-  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < it.n;
-       q += (int64_t)gridDim.x * blockDim.x) {
-    int i, j, k;
-    it.ijk(q, i, j, k);
-    int64_t idx = bx.at(i, j, k);
+  for_rows<true>(bx, [&](int64_t idx, int i, int j, int k) {
+    (void)i; (void)j; (void)k;
     if (fabs(vn[idx]) < real(1.e-5)) {
       int pos = atomicAdd(count, 1);
       if (pos < cap) list[pos] = idx;
     }
-  }
+  });
 }
 hipError_t launch_collect_stale(const Box& bx, const real* vn, int64_t* list, int* count, int cap,
                                 hipStream_t st) {
-  FillIter it(bx);
-  hipLaunchKernelGGL(collect_stale_kernel, dim3(stream_blocks(it.n)), dim3(256), 0, st, bx, vn,
+  hipLaunchKernelGGL(collect_stale_kernel, dim3(row_blocks<true>(bx, 16384)), dim3(ROW_TX, ROW_TY), 0, st, bx, vn,
                      list, count, cap);
   return hipGetLastError();
 }
