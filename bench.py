@@ -10,6 +10,13 @@ library; the FP64 library is measured on the same input and reported beside
 it.  Inputs are device-resident before the timed region (set up by the
 library's own drain_model3d set-up on the GPU).
 
+Beside it, in the same JSON line: the ENO2 "RK2" and ENO3 RK3 step rates
+(contact angle 0) on the same pack; the drainage-curve time on the
+workloads of lsmpqs_amd/workloads.py (256^3 drain + imbibe 25 + 25 steps,
+512^3 FP32 vs FP64 40 steps in reference mode and FP32 with the quirks off,
+64^3 ENO2 10 steps against the reference build); the adjoint; the batched
+subvolumes; and the reference build's CPU rate.
+
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512] [--no-cpu]
 """
 import argparse
@@ -44,33 +51,40 @@ def lean_arrays(dtype, mask, phi0):
     return rec, keep
 
 
-def options(dtype, a, b):
+def options(dtype, a, b, space=5, time_order=3, theta_deg=THETA_DEG):
     o = default_options(dtype)
-    o.order_space_accur = 5
-    o.order_time_accur = 3
-    o.theta = THETA_DEG * 3.14159 / 180
+    o.order_space_accur = space
+    o.order_time_accur = time_order
+    o.theta = theta_deg * 3.14159 / 180
     o.a = a
     o.b = b
     o.check_connectivity = 1
     return o
 
 
-def gpu_leg(dtype, g64, mask, phi0, steps, warmup, a, b):
+def gpu_leg(dtype, g64, mask, phi0, steps, warmup, a, b, space=5, time_order=3,
+            theta_deg=THETA_DEG):
+    """K inner steps of constCurvModel3d's loop on the device state the
+    library's own drainage set-up left (time-step probe, fused RHS + RK
+    stages, ghost extrapolation, imposition), timed with HIP events on the
+    library's stream; then a shorter pass with events around every stage
+    kernel for the per-kernel durations."""
     lib = hiplib.load(dtype)
     g = make_grid(tuple(g64.grid_dims), dtype)
     rec, keep = lean_arrays(dtype, mask, phi0)
-    o = options(dtype, a, b)
+    o = options(dtype, a, b, space, time_order, theta_deg)
+    var = 1 if theta_deg > 0 else 0
     lib.lsmpqs_set_dt_slabs(16)
     if lib.lsmpqs_model_prepare(C.byref(o), C.byref(rec), C.byref(g), 1) != 0:
         raise hiplib.LsmpqsError(hiplib.last_error(lib))
     t = hiplib.Timing()
     if warmup:
-        if lib.lsmpqs_bench_inner_steps(C.byref(o), C.byref(rec), C.byref(g), warmup, 1, a, 0,
+        if lib.lsmpqs_bench_inner_steps(C.byref(o), C.byref(rec), C.byref(g), warmup, var, a, 0,
                                         C.byref(t)) != 0:
             raise hiplib.LsmpqsError(hiplib.last_error(lib))
     t = hiplib.Timing()
     w0 = time.perf_counter()
-    rc = lib.lsmpqs_bench_inner_steps(C.byref(o), C.byref(rec), C.byref(g), steps, 1, a, 0,
+    rc = lib.lsmpqs_bench_inner_steps(C.byref(o), C.byref(rec), C.byref(g), steps, var, a, 0,
                                       C.byref(t))
     wall = time.perf_counter() - w0
     if rc != 0:
@@ -78,8 +92,8 @@ def gpu_leg(dtype, g64, mask, phi0, steps, warmup, a, b):
     # per-kernel durations in a separate pass (events around each stage launch)
     tp = hiplib.Timing()
     tp.profile = 1
-    if lib.lsmpqs_bench_inner_steps(C.byref(o), C.byref(rec), C.byref(g), max(3, steps // 2), 1, a,
-                                    0, C.byref(tp)) != 0:
+    if lib.lsmpqs_bench_inner_steps(C.byref(o), C.byref(rec), C.byref(g), max(3, steps // 2), var,
+                                    a, 0, C.byref(tp)) != 0:
         raise hiplib.LsmpqsError(hiplib.last_error(lib))
     lib.lsmpqs_release(C.byref(rec))
     nvox = g64.grid_dims[0] * g64.grid_dims[1] * g64.grid_dims[2]
@@ -132,6 +146,31 @@ def stage_roofline(res, itemsize, key, n):
     return out
 
 
+# Algorithmic bytes per voxel-update of a whole inner step (fields read and
+# written once per stage, coefficients derived from the mask, trapped-phase
+# masks imposed): contact-angle path RK3 13 fields, theta 0 RK3 11, theta 0
+# "RK2" 8.
+STEP_FIELDS = {("var", 3): 13, ("const", 3): 11, ("const", 2): 8}
+
+
+def scheme_leg(dtype, g64, mask, phi0, steps, warmup, space, time_order):
+    """Inner-step rate of a contact-angle-0 scheme (the constant-coefficient
+    right-hand sides: ENO2 "RK2", ENO3 RK3) on the same pack: voxel-updates/s
+    and the whole step's algorithmic HBM rate against 8 TB/s."""
+    r = gpu_leg(dtype, g64, mask, phi0, steps, warmup, 0.02, 0.04, space, time_order, 0.0)
+    itemsize = np.dtype(dtype).itemsize
+    ms = r["elapsed_ms"] / steps
+    vu = r["nvox"] / (ms * 1e-3)
+    fields = STEP_FIELDS[("const", time_order)]
+    ach = fields * itemsize * vu / 1e9
+    return {"scheme": {2: "ENO2", 3: "ENO3"}[space] + ("+RK2" if time_order == 2 else "+TVD-RK3"),
+            "ms_per_step": round(ms, 4), "voxel_updates_per_s": round(vu, 1),
+            "stage_avg_ms": [round(x, 4) for x in r["stage_avg_ms"][:time_order]],
+            "step_roofline": {"bound": "hbm", "alg_bytes_per_voxel_update": fields * itemsize,
+                              "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": round(ach / HBM_PEAK_GBS, 4)}}
+
+
 def curve_leg(dtype, g64, mask, phi0, curv_steps, tmax, a, b, step_files=False):
     """Drainage-curve time: drain_model3d end to end on the GPU (set-up,
     curv_steps curvature steps, each running outer steps until the reference's
@@ -163,6 +202,141 @@ def curve_leg(dtype, g64, mask, phi0, curv_steps, tmax, a, b, step_files=False):
     return {"wall_s": round(wall, 3), "outer_steps": int(st.outer_steps),
             "inner_steps": int(st.inner_steps), "inner_loop_s": round(st.inner_loop_ms * 1e-3, 3),
             "nw_voxels": int((keep[0] < 0).sum())}
+
+
+def model_leg(dtype, g64, mask, phi0, o, drain_steps, imbibe_steps, slabs=16, quirks=None):
+    """drain_model3d over drain_steps curvature steps, then (imbibe_steps > 0)
+    imbibe_model3d from the last drainage pressure, on the GPU from host
+    arrays (step files off): host wall time of each curve, outer and inner
+    steps, GPU time in inner loops, final non-wetting saturation."""
+    from lsmpqs_amd.workloads import saturation_of
+    lib = hiplib.load(dtype)
+    g = make_grid(tuple(g64.grid_dims), dtype)
+    rec, keep = lean_arrays(dtype, mask, phi0)
+    _, _, DataArrays = types_for(dtype)
+    extra = {f: np.zeros(keep[0].shape, dtype) for f in ("mask_w", "mask_nw")}
+    rec.mask_w, rec.mask_nw = extra["mask_w"].ctypes.data, extra["mask_nw"].ctypes.data
+    lib.lsmpqs_set_dt_slabs(slabs)
+    os.environ["LSMPQS_NO_STEP_FILES"] = "1"
+    old = lib.lsmpqs_set_quirks(quirks) if quirks is not None else None
+    out = {}
+    try:
+        for name, steps, fn in (("drain", drain_steps, lib.drain_model3d),
+                                ("imbibe", imbibe_steps, lib.imbibe_model3d)):
+            if steps <= 0:
+                continue
+            if name == "drain":
+                o.amax = o.a + o.b * o.dc * (steps - 0.5)
+            else:
+                o.a = o.a + o.b * o.dc * (drain_steps - 1)
+                o.amin = o.a - o.b * o.dc * (steps - 0.5)
+            w0 = time.perf_counter()
+            fn(C.byref(o), C.byref(rec), C.byref(g), None)
+            wall = time.perf_counter() - w0
+            hiplib.check(lib)
+            st = hiplib.Timing()
+            lib.lsmpqs_session_stats(C.byref(rec), C.byref(g), C.byref(st))
+            out[name] = {"curvature_steps": steps, "wall_s": round(wall, 3),
+                         "outer_steps": int(st.outer_steps), "inner_steps": int(st.inner_steps),
+                         "inner_loop_s": round(st.inner_loop_ms * 1e-3, 3),
+                         "sat_nw": round(saturation_of(keep[0], keep[1], g), 8)}
+    finally:
+        if old is not None:
+            lib.lsmpqs_set_quirks(old)
+        lib.lsmpqs_release(C.byref(rec))
+    return out, keep[0]
+
+
+def curves_leg(n_pair=256, n_prec=512):
+    """The drainage-curve half of the metric on three workloads:
+    * the 256^3 pack of radius U[6,12] (porosity 0.36, seed 1002), WENO5 +
+      TVD-RK3, 30 deg: 25 drainage then 25 imbibition curvature steps,
+      eps_stop 1e-4 dx, pseudo-time 20 per step, FP32 and FP64;
+    * the 512^3 pack of radius U[10,20] (porosity 0.35, seed 1003), WENO5 +
+      TVD-RK3, 39.999 deg, 40 drainage steps, pseudo-time 20 per step: FP32
+      and FP64 in reference mode (precision_leg, with the FP32-vs-FP64
+      errors), and FP32 with the reference quirks off, whose WENO5 normal term
+      has the physical sign, so the non-wetting phase invades and trapping
+      works on a moving front."""
+    from lsmpqs_amd import workloads as W
+    spec = W.WENO5_DRAIN_IMBIBE
+    g64 = make_grid((n_pair,) * 3, np.float64)
+    mask, _, _, por = sphere_pack_mask(g64, spec["radius"], spec["porosity"], spec["seed"])
+    phi0 = initial_phi(g64, mask)
+    pair = {"grid": n_pair, "porosity": round(float(por), 4), "tmax_per_step": 20.0}
+    for key, dt_ in (("f32", np.float32), ("f64", np.float64)):
+        o = W.weno5_options(spec, dt_)
+        o.tmax = 20.0
+        pair[key], _ = model_leg(dt_, g64, mask, phi0, o, spec["steps"], spec["steps"])
+        print(f"# curve pair {key}: {pair[key]}", file=sys.stderr, flush=True)
+    del mask, phi0
+    prec = precision_leg(n_prec, W.WENO5_PRECISION["steps"], 20.0)
+    print(f"# precision: {prec}", file=sys.stderr, flush=True)
+    spec3 = W.WENO5_PRECISION
+    g3 = make_grid((n_prec,) * 3, np.float64)
+    mask, _, _, _ = sphere_pack_mask(g3, spec3["radius"], spec3["porosity"], spec3["seed"])
+    phi0 = initial_phi(g3, mask)
+    o = W.weno5_options(spec3, np.float32)
+    o.tmax = 20.0
+    fixed, _ = model_leg(np.float32, g3, mask, phi0, o, spec3["steps"], 0, quirks=0)
+    fixed["grid"] = n_prec
+    print(f"# fixed-mode curve: {fixed}", file=sys.stderr, flush=True)
+    return {"weno5_drain25_imbibe25": pair, "weno5_precision_drain40": prec,
+            "weno5_fixed_mode_drain40_f32": fixed}
+
+
+def drain64_leg(threads):
+    """The 64^3 ENO2 drainage curve (radius 8, porosity 0.40, 10 curvature
+    steps from kappa 0.05 to 0.30, reinitialization every 20 time steps,
+    trapping on): wall time of drain_model3d on the GPU (FP64 and FP32) and
+    of the reference build's drain_model3d (OpenMP, `threads` threads, FP64
+    and FP32 as shipped), the same `threads` time-step slabs on both."""
+    import tempfile
+    import oracle_py as op
+    from lsmpqs_amd import workloads as W
+    from lsmpqs_amd.abi import Arrays
+    from lsmpqs_amd.workloads import saturation_of
+    libc = C.CDLL("libc.so.6")
+    libc.fopen.restype = C.c_void_p
+    libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
+    devnull = C.c_void_p(libc.fopen(b"/dev/null", b"w"))
+    out = {"grid": 64, "curvature_steps": W.DRAIN64["steps"], "threads": threads}
+    finals = {}
+    for kind in ("gpu", "reference"):
+        if kind == "reference" and not op.have_ref(np.float64):
+            continue
+        for key, dt_ in (("f64", np.float64), ("f32", np.float32)):
+            g, mask, phi0, _ = W.drain64_inputs(dt_)
+            A = Arrays(g, dt_)
+            A["mask"], A["phi"] = mask, phi0
+            o = W.drain64_options(dt_)
+            cwd = os.getcwd()
+            if kind == "gpu":
+                lib = hiplib.load(dt_)
+                lib.lsmpqs_set_dt_slabs(threads)
+                os.environ["LSMPQS_NO_STEP_FILES"] = "1"
+            else:
+                op.pad_phi_bin(A, g)
+                lib = op.ref_lib(dt_)
+                op.set_omp_threads(threads)
+                os.chdir(tempfile.mkdtemp(prefix="lsm_ref64_"))
+            try:
+                w0 = time.perf_counter()
+                lib.drain_model3d(C.byref(o), A.ptr(), C.byref(g), devnull)
+                wall = time.perf_counter() - w0
+            finally:
+                os.chdir(cwd)
+            if kind == "gpu":
+                hiplib.check(lib)
+                lib.lsmpqs_release(A.ptr())
+            finals[(kind, key)] = A["phi"].astype(np.float64)
+            out[f"{kind}_{key}_s"] = round(wall, 3)
+            out[f"{kind}_{key}_sat_nw"] = round(saturation_of(A["phi"], A["mask"], g), 8)
+    if ("reference", "f64") in finals:
+        out["max_abs_diff_gpu_f64_vs_reference_f64"] = float(
+            np.max(np.abs(finals[("gpu", "f64")] - finals[("reference", "f64")])))
+        out["reference_f64_over_gpu_f64"] = round(out["reference_f64_s"] / out["gpu_f64_s"], 2)
+    return out
 
 
 def adjoint_leg(dtype, n=128, steps=200, checkpoint_every=20):
@@ -457,77 +631,111 @@ def model2d_leg(n=1024, curv_steps=2, tmax=4.0, seed=1006, cpu=True):
     return out
 
 
+def cpu_threads():
+    """Host threads for the CPU baseline: the CPUs this process may run on,
+    capped by OMP_NUM_THREADS when it is set (the box's CPU share for one
+    GPU), and the CPU model name."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return n, model, len(os.sched_getaffinity(0))
+
+
 def cpu_leg(n, steps=8):
-    """The reference build (oracle/_ref, OpenMP, FP64) on a bounded sample:
-    one outer step of constCurvModel3d at n^3 whose pseudo-time window holds
-    `steps` inner steps; the time is the reference's own inner-loop timer
-    (the "Level set time" line it writes to its log)."""
+    """The reference build (oracle/_ref, OpenMP) on a bounded sample of the
+    headline workload: one outer step of constCurvModel3d at n^3 (WENO5 +
+    TVD-RK3, 30 deg, trapping off) whose pseudo-time window holds `steps`
+    inner steps, in FP32 (as the reference ships) and FP64; the time is the
+    reference's own inner-loop timer (its "Level set time" log line).  dt is
+    formed as the reference forms it (H/dx at each slab's last voxel), from
+    the reference's own right-hand side at those voxels."""
     import tempfile
     import oracle_py as op
     from lsmpqs_amd.abi import Arrays
-    dt = np.float64
-    if not op.have_ref(dt):
+    if not op.have_ref(np.float64):
         return None
-    threads = int(os.environ.get("OMP_NUM_THREADS", "16") or 16)
+    threads, model, visible = cpu_threads()
     op.set_omp_threads(threads)
-    lib = op.ref_lib(dt)
-    g = make_grid((n, n, n), dt)
-    mask, _, _, _ = sphere_pack_mask(g, (6, 12), 0.36, 1002)
-    A = Arrays(g, dt)
-    A["mask"] = mask
-    A["phi"] = initial_phi(g, mask)
-    o = options(dt, 0.02, 0.04)
-    gb, fb = op.gb6(g), op.fb6(g)
-    op.fcall(lib, "getcentralgrad3dorder2_", dt, A["mask_x"], A["mask_y"], A["mask_z"], *gb,
-             A["mask"], *gb, *fb, 1.0, 1.0, 1.0)
-    bm, um = op.Scalar(0.0, C.c_double), op.Scalar(0.0, C.c_double)
-    op.fcall(lib, "setvarcurvadv3d_", dt, A["mask"], A["mask_x"], A["mask_y"], A["mask_z"],
-             A["curvature_coeff"], A["external_velocity_x"], A["external_velocity_y"],
-             A["external_velocity_z"], bm, um, 0.04, *gb, *fb, 1.0)
-    op.fcall(lib, "setvarnorm3d_", dt, A["mask"], A["mask_x"], A["mask_y"], A["mask_z"],
-             A["normal_velocity"], 0.02, float(o.theta), *gb, *fb, 1.0)
-    o.b_max_over_dx, o.max_U_over_dx = bm.value, um.value
-    # first time step exactly as the reference forms it: H/dx at each slab's last voxel
-    nsl = g.khi_fb - g.klo_fb + 1
-    H = -1.0
-    for t in range(threads):
-        klo = g.klo_fb + nsl * t // threads
-        khi = min(g.klo_fb + nsl * (t + 1) // threads - 1, g.khi_fb)
-        if klo > khi:
-            continue
-        mh = op.Scalar(0.0, C.c_double)
-        rhs = np.zeros_like(A["phi"])
-        op.fcall(lib, "getrhs3d5th_", dt, A["phi"], A["normal_velocity"], A["curvature_coeff"],
-                 A["external_velocity_x"], A["external_velocity_y"], A["external_velocity_z"], mh,
-                 rhs, *gb, g.ilo_fb, g.ihi_fb, g.jlo_fb, g.jhi_fb, klo, khi, 1.0, 1.0, 1.0)
-        H = max(H, mh.value)
-    dt1 = 0.5 / (H + o.b_max_over_dx + o.max_U_over_dx)
-    o.check_connectivity = 0
-    lib.setSpaceDerivFunc(C.byref(g), C.byref(o))
-    o.tmax = 1e-30
-    o.tplot = dt1 * (steps - 0.5)
-    log = tempfile.NamedTemporaryFile(prefix="lsm_ref_", suffix=".log", delete=False).name
-    libc = C.CDLL("libc.so.6")
-    libc.fopen.restype = C.c_void_p
-    libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
-    libc.fclose.argtypes = [C.c_void_p]
-    fp = libc.fopen(log.encode(), b"w")
-    lib.constCurvModel3d.restype = C.c_double
-    lib.constCurvModel3d(C.byref(o), A.ptr(), C.byref(g), C.c_void_p(fp))
-    libc.fclose(C.c_void_p(fp))
-    secs = None
-    for line in open(log):
-        if "Level set time" in line:
-            secs = float(line.split("=")[1])
-    os.unlink(log)
-    if not secs:
-        return None
-    per_step = secs / steps
-    return {"value": round(n ** 3 / per_step, 1), "unit": "voxel-updates/s", "cores": threads,
+    g64 = make_grid((n, n, n), np.float64)
+    mask64, _, _, _ = sphere_pack_mask(g64, (6, 12), 0.36, 1002)
+    phi64 = initial_phi(g64, mask64)
+    res = {}
+    for key, dt in (("f32", np.float32), ("f64", np.float64)):
+        lib = op.ref_lib(dt)
+        R = C.c_double if dt == np.float64 else C.c_float
+        g = make_grid((n, n, n), dt)
+        A = Arrays(g, dt)
+        A["mask"] = mask64
+        A["phi"] = phi64
+        o = options(dt, 0.02, 0.04)
+        gb, fb = op.gb6(g), op.fb6(g)
+        op.fcall(lib, "getcentralgrad3dorder2_", dt, A["mask_x"], A["mask_y"], A["mask_z"], *gb,
+                 A["mask"], *gb, *fb, 1.0, 1.0, 1.0)
+        bm, um = op.Scalar(0.0, R), op.Scalar(0.0, R)
+        op.fcall(lib, "setvarcurvadv3d_", dt, A["mask"], A["mask_x"], A["mask_y"], A["mask_z"],
+                 A["curvature_coeff"], A["external_velocity_x"], A["external_velocity_y"],
+                 A["external_velocity_z"], bm, um, 0.04, *gb, *fb, 1.0)
+        op.fcall(lib, "setvarnorm3d_", dt, A["mask"], A["mask_x"], A["mask_y"], A["mask_z"],
+                 A["normal_velocity"], 0.02, float(o.theta), *gb, *fb, 1.0)
+        o.b_max_over_dx, o.max_U_over_dx = bm.value, um.value
+        # the first dt as the reference forms it: its right-hand side routine
+        # returns H/dx of the last voxel of the range, so each slab's value
+        # comes from that one voxel
+        nsl = g.khi_fb - g.klo_fb + 1
+        H = -1.0
+        for t in range(threads):
+            klo = g.klo_fb + nsl * t // threads
+            khi = min(g.klo_fb + nsl * (t + 1) // threads - 1, g.khi_fb)
+            if klo > khi:
+                continue
+            mh = op.Scalar(0.0, R)
+            op.fcall(lib, "getrhs3d5th_", dt, A["phi"], A["normal_velocity"], A["curvature_coeff"],
+                     A["external_velocity_x"], A["external_velocity_y"],
+                     A["external_velocity_z"], mh, A["lse_rhs"], *gb, g.ihi_fb, g.ihi_fb,
+                     g.jhi_fb, g.jhi_fb, khi, khi, 1.0, 1.0, 1.0)
+            H = max(H, mh.value)
+        dt1 = 0.5 / (H + o.b_max_over_dx + o.max_U_over_dx)
+        o.check_connectivity = 0
+        lib.setSpaceDerivFunc(C.byref(g), C.byref(o))
+        o.tmax = 1e-30
+        o.tplot = dt1 * (steps - 0.5)
+        log = tempfile.NamedTemporaryFile(prefix="lsm_ref_", suffix=".log", delete=False).name
+        libc = C.CDLL("libc.so.6")
+        libc.fopen.restype = C.c_void_p
+        libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
+        libc.fclose.argtypes = [C.c_void_p]
+        fp = libc.fopen(log.encode(), b"w")
+        lib.constCurvModel3d.restype = R
+        lib.constCurvModel3d(C.byref(o), A.ptr(), C.byref(g), C.c_void_p(fp))
+        libc.fclose(C.c_void_p(fp))
+        secs = None
+        for line in open(log):
+            if "Level set time" in line:
+                secs = float(line.split("=")[1])
+        os.unlink(log)
+        del A
+        if not secs:
+            return None
+        res[key] = {"value": round(n ** 3 * steps / secs, 1), "ms_per_step": round(secs / steps * 1e3, 2)}
+        print(f"# cpu reference {key}: {res[key]}", file=sys.stderr, flush=True)
+    return {"value": res["f32"]["value"], "unit": "voxel-updates/s", "cores": threads,
             "kind": "reference",
-            "sample": f"reference constCurvModel3d (OpenMP {threads} threads, FP64) at {n}^3, "
-                      f"WENO5+TVD-RK3 theta 30: {steps} inner steps of one outer step, timed by "
-                      f"the reference's own inner-loop timer ({per_step * 1e3:.1f} ms/step)"}
+            "sample": f"reference constCurvModel3d (OpenMP, {threads} threads, FP32 as shipped) at "
+                      f"{n}^3 on the headline pack, WENO5 + TVD-RK3, 30 deg: {steps} inner steps "
+                      f"of one outer step, timed by the reference's own inner-loop timer; the "
+                      f"512^3 box would need 24 GB of host arrays and a minute of the "
+                      f"reference's serial set-up per precision",
+            "f64_value": res["f64"]["value"], "ms_per_step": {k: v["ms_per_step"] for k, v in res.items()},
+            "cpu_model": model, "cpus_visible": visible}
 
 
 def calibration_copy(mbytes):
@@ -552,14 +760,16 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=512)
-    ap.add_argument("--cpu-n", type=int, default=128)
+    ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--curve-n", type=int, default=256)
     ap.add_argument("--curve-steps", type=int, default=3)
     ap.add_argument("--curve-tmax", type=float, default=20.0)
     ap.add_argument("--precisions", default="f32,f64")
     ap.add_argument("--calib-copy", type=int, default=0, help="MiB copied once before the run")
-    ap.add_argument("--curve-only", action="store_true", help="only the drainage-curve leg")
+    ap.add_argument("--curve-only", action="store_true",
+                    help="only a drainage curve on the headline pack (--curve-n, --curve-steps, "
+                         "--curve-tmax)")
     ap.add_argument("--curve-step-files", action="store_true",
                     help="drainage-curve leg with data_step_N.gz files written")
     ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint leg")
@@ -569,9 +779,15 @@ def main():
                     help="only the 2D drainage leg (--model2d-n, --curve-steps, --curve-tmax)")
     ap.add_argument("--model2d-n", type=int, default=2048)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-subvolume leg")
+    ap.add_argument("--no-curves", action="store_true",
+                    help="skip the drainage-curve workloads (256^3 drain+imbibe, 512^3 precision "
+                         "and fixed-mode curves, 64^3 ENO2 curve against the reference)")
+    ap.add_argument("--no-schemes", action="store_true", help="skip the ENO2 / ENO3 step rates")
     ap.add_argument("--precision-only", action="store_true",
                     help="only the FP32-vs-FP64 drainage-curve comparison (--n, --curve-steps, "
                          "--curve-tmax)")
+    ap.add_argument("--stage-only", action="store_true",
+                    help="only the headline inner steps in FP32 (for kernel profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -606,7 +822,6 @@ def main():
                                  args.curve_step_files)
         print(json.dumps({"drainage_curve": out, "grid": args.curve_n}), flush=True)
         return
-    precs = args.precisions.split(",")
     n = args.n
     g64 = make_grid((n, n, n), np.float64)
     t0 = time.perf_counter()
@@ -617,25 +832,28 @@ def main():
     a, b = 0.02, 0.04
     r32 = gpu_leg(np.float32, g64, mask, phi0, args.steps, args.warmup, a, b)
     print(f"# fp32 done: {r32['elapsed_ms'] / args.steps:.3f} ms/step", file=sys.stderr, flush=True)
-    if "f64" not in precs:
+    if args.stage_only or "f64" not in args.precisions.split(","):
         print(json.dumps({"f32_ms_per_step": r32["elapsed_ms"] / args.steps,
                           "stage_avg_ms": r32["stage_avg_ms"]}), flush=True)
         return
     r64 = gpu_leg(np.float64, g64, mask, phi0, args.steps, args.warmup, a, b)
     print(f"# fp64 done: {r64['elapsed_ms'] / args.steps:.3f} ms/step", file=sys.stderr, flush=True)
+    schemes = None
+    if not args.no_schemes:
+        schemes = {}
+        for (sp, to) in ((2, 2), (3, 3)):
+            for key, dt_ in (("f32", np.float32), ("f64", np.float64)):
+                r = scheme_leg(dt_, g64, mask, phi0, args.steps, args.warmup, sp, to)
+                schemes[f"{r['scheme']}_{key}"] = r
+                print(f"# scheme {r['scheme']} {key}: {r}", file=sys.stderr, flush=True)
     del mask, phi0
-    curve = None
-    if args.curve_n > 0:
-        gc = make_grid((args.curve_n,) * 3, np.float64)
-        mc, _, _, _ = sphere_pack_mask(gc, (6, 12), 0.36, 1002)
-        pc = initial_phi(gc, mc)
-        curve = {"grid": args.curve_n, "curvature_steps": args.curve_steps, "tmax": args.curve_tmax}
-        for dt_, key in ((np.float32, "f32"), (np.float64, "f64")):
-            curve[key] = curve_leg(dt_, gc, mc, pc, args.curve_steps, args.curve_tmax, a, b)
-            print(f"# curve {key}: {curve[key]}", file=sys.stderr, flush=True)
-        del mc, pc
     v32 = r32["nvox"] * args.steps / (r32["elapsed_ms"] * 1e-3)
     v64 = r64["nvox"] * args.steps / (r64["elapsed_ms"] * 1e-3)
+    curves = None
+    if not args.no_curves:
+        curves = curves_leg(256, n)
+        curves["eno2_drain64_vs_reference"] = drain64_leg(cpu_threads()[0])
+        print(f"# drain64: {curves['eno2_drain64_vs_reference']}", file=sys.stderr, flush=True)
     adjoint = None
     if not args.no_adjoint:
         adjoint = {"f32": adjoint_leg(np.float32), "f64": adjoint_leg(np.float64)}
@@ -646,7 +864,8 @@ def main():
         print(f"# batch: {batch}", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_leg(args.cpu_n)
     line = {
-        "metric": "voxel-updates/s per RK3 step (WENO5, theta 30, trapped masks), FP32 vs FP64, 1 MI355X",
+        "metric": "voxel-updates/s per RK3 step (WENO5, theta 30, trapped masks) & drainage-curve "
+                  "time, FP32 vs FP64, 1 MI355X",
         "value": round(v32, 1),
         "unit": "voxel-updates/s",
         "n_gpus": 1,
@@ -658,14 +877,15 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded Boolean sphere pack, radius U[6,12], porosity 0.36, seed 1002)",
-        "config": {"workload": f"{n}^3 drainage step, WENO5 + TVD-RK3, theta 30 deg, dx 1, "
+        "config": {"workload": f"{n}^3 drainage inner step, WENO5 + TVD-RK3, theta 30 deg, dx 1, "
                                f"16 dt slabs, trapping masks on", "grid": n, "scheme": "WENO5+TVD-RK3"},
         "roofline": stage_roofline(r32, 4, "f32", n),
         "cpu_baseline": cpu,
         "fp64": {"value": round(v64, 1), "ms_per_step": round(r64["elapsed_ms"] / args.steps, 4),
                  "roofline": stage_roofline(r64, 8, "f64", n)},
         "fp32_over_fp64": round(v32 / v64, 3),
-        "drainage_curve": curve,
+        "schemes": schemes,
+        "drainage_curves": curves,
         "adjoint": adjoint,
         "batch": batch,
         "stage_avg_ms": {"f32": [round(x, 4) for x in r32["stage_avg_ms"]],

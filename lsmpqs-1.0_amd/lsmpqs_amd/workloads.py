@@ -80,3 +80,21 @@ def weno5_options(spec, dtype=np.float64, steps=None, drain=True):
 
 def weno5_inputs(spec, n, dtype=np.float64):
     return pack_case((n, n, n), spec["radius"], spec["porosity"], spec["seed"], dtype)
+
+
+def saturation_of(phi, mask, grid, eps=1.5):
+    """Non-wetting saturation V(phi < 0) / V(mask < 0) over the fill box with
+    the reference's smoothed Heaviside of width eps dx, summed in FP64
+    (reference: src/utilities/qss_util3d.f:288-358,
+    src/level_set_evolution/constCurvModel3d.c:66-76,292-295)."""
+    f = (slice(grid.klo_fb, grid.khi_fb + 1), slice(grid.jlo_fb, grid.jhi_fb + 1),
+         slice(grid.ilo_fb, grid.ihi_fb + 1))
+    w = eps * float(grid.dx[0])
+
+    def vol(a):
+        a = np.asarray(a[f], dtype=np.float64)
+        q = a / w
+        v = np.where(a < -w, 1.0, np.where(a < w, 0.5 * (1 - q - np.sin(np.pi * q) / np.pi), 0.0))
+        return float(v.sum())
+
+    return vol(phi) / vol(mask)

@@ -50,6 +50,22 @@ def ref_lib(dtype):
     return _LIBS[key]
 
 
+def pad_phi_bin(A, g):
+    """Give A's phi_bin array readable room in front of it before the
+    reference build labels it: the reference's unionCoords3d reads phi_bin
+    at a neighbour's index before testing that the neighbour lies in the box
+    (src/connectivity/connectivity.c:138-146), up to nx*ny + nx + 1 bytes
+    before the array.  It discards those bytes; they only have to be
+    mapped.  A is a lsmpqs_amd.abi.Arrays record; returns A."""
+    nx, ny = g.grid_dims_ghostbox[0], g.grid_dims_ghostbox[1]
+    pad = nx * ny + nx + 64
+    n = A["phi_bin"].size
+    buf = np.zeros(n + 2 * pad, dtype=np.uint8)
+    A.a["phi_bin"] = buf[pad:pad + n].reshape(A["phi_bin"].shape)
+    A.rec.phi_bin = A.a["phi_bin"].ctypes.data
+    return A
+
+
 def set_omp_threads(n):
     """Thread count the reference build's OpenMP team will use."""
     gomp = C.CDLL("libgomp.so.1")

@@ -58,7 +58,7 @@ def run(lib, name, dtype, workdir, extra=()):
     """Run drain_model3d of case `name` in workdir; returns (arrays of the
     four checkpoint files, connectivity-time lines, outer steps)."""
     g, mask, phi0 = case(N, dtype)
-    A = arrays(g, mask, phi0, dtype)
+    A = op.pad_phi_bin(arrays(g, mask, phi0, dtype), g)
     o = case_options(name, dtype)
     libc = C.CDLL("libc.so.6")
     libc.fopen.restype = C.c_void_p

@@ -109,16 +109,7 @@ def main():
           f"({time.perf_counter() - t0:.0f} s)", flush=True)
     A = arrays(g, mask, phi0, dt)
     del mask, phi0
-    # The reference's unionCoords3d reads phi_bin at the neighbour's index
-    # before testing that the neighbour lies in the box
-    # (src/connectivity/connectivity.c:138-146), up to nx*ny + nx + 1 bytes
-    # before the array; it discards what it reads there, but the bytes must be
-    # readable, so phi_bin gets that much room in front of it.
-    nx, ny = g.grid_dims_ghostbox[0], g.grid_dims_ghostbox[1]
-    pad = nx * ny + nx + 64
-    buf = np.zeros(A["phi_bin"].size + 2 * pad, dtype=np.uint8)
-    A.a["phi_bin"] = buf[pad:pad + A["phi_bin"].size].reshape(A["phi_bin"].shape)
-    A.rec.phi_bin = A.a["phi_bin"].ctypes.data
+    op.pad_phi_bin(A, g)   # the reference reads in front of phi_bin while labelling
     # its find() recurses along parent chains, which grow long at 512^3
     resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
     o = case_options(name, dt)

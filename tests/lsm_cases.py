@@ -78,6 +78,7 @@ def run_model(kind, dtype, n, opts, T=4, drain=True, imbibe_after=False, **case_
         f_d, f_i = lib.orc_drain_model3d, lib.orc_imbibe_model3d
         extra = (C.c_int(T),)
     elif kind == "ref":
+        op.pad_phi_bin(A, g)
         lib = op.ref_lib(dtype)
         op.set_omp_threads(T)
         f_d, f_i = lib.drain_model3d, lib.imbibe_model3d
