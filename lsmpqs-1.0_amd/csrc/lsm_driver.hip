@@ -273,16 +273,120 @@ StepArgs make_step_args(Session* s, const StepCfg& c) {
   return a;
 }
 
-static void inner_step(Session* s, const StepCfg& c) {
-  StepArgs a = make_step_args(s, c);
+// Inner steps replayed as HIP graphs (LSMPQS_GRAPHS=1).  A step is two
+// graphs: the time-step probe with the read-back of dt into pinned memory,
+// then the RK stages with their ghost extrapolations; the host records ev_dt
+// between them, so it learns dt while the stages run, as with direct
+// launches.  A graph pair is keyed by the step's kernel arguments (buffers,
+// coefficients, scheme); the session's buffers rotate between a few
+// arrangements (phi and phi_prev swap every outer step, the order-1 result
+// lands in B), so a session keeps a few pairs and replaces the least
+// recently used.  No memset is recorded.  Off by default: on MI355X with
+// ROCm 7 the two graph launches of a step cost more host time than its eight
+// direct launches (64^3 ENO2 drainage, 844 steps: 55 ms of inner-loop time
+// with graphs against 38 ms without), and the direct launches already keep
+// the GPU busy while the host waits for dt.
+struct StepGraph {
+  StepArgs key;
+  int scheme = 0;
+  bool var = false;
+  hipGraphExec_t probe = nullptr, stages = nullptr;
+  uint64_t used = 0;
+};
+struct GraphCache {
+  static constexpr int kMax = 8;
+  std::vector<StepGraph> g;
+  uint64_t clock = 0;
+};
+
+static void destroy_graph(StepGraph& e) {
+  if (e.probe) (void)hipGraphExecDestroy(e.probe);
+  if (e.stages) (void)hipGraphExecDestroy(e.stages);
+  e.probe = e.stages = nullptr;
+}
+
+void graph_cache_free(GraphCache* c) {
+  for (StepGraph& e : c->g) destroy_graph(e);
+  delete c;
+}
+
+static bool graphs_enabled() {
+  const char* e = getenv("LSMPQS_GRAPHS");
+  return e && *e && atoi(e) != 0;
+}
+
+static void enqueue_probe(Session* s, const StepArgs& a, const StepCfg& c) {
   DCHECK(launch_probe(a.probe, c.scheme, s->st));
   DCHECK(hipMemcpyAsync(s->h_dt, s->dt, sizeof(real), hipMemcpyDeviceToHost, s->st));
-  DCHECK(hipEventRecord(s->ev_dt, s->st));
+}
+
+static void enqueue_stages(Session* s, const StepArgs& a, const StepCfg& c) {
   const int nst = c.order <= 1 ? 1 : (c.order == 2 ? 2 : 3);
   for (int i = 0; i < nst; ++i) {
     const bool fin = i == nst - 1;
     stage_timed(s, a.stage[i], c, i + 1, fin);
     DCHECK(launch_bc(a.bc[i], fin, s->st));
+  }
+}
+
+// capture what `body` enqueues on s->st into an executable graph
+template <typename F>
+static hipGraphExec_t capture_on(Session* s, F&& body) {
+  DCHECK(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+  hipGraph_t graph = nullptr;
+  try {
+    body();
+  } catch (...) {
+    (void)hipStreamEndCapture(s->st, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    throw;
+  }
+  DCHECK(hipStreamEndCapture(s->st, &graph));
+  hipGraphExec_t exec = nullptr;
+  hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  DCHECK(e);
+  return exec;
+}
+
+static StepGraph& step_graph(Session* s, const StepArgs& a, const StepCfg& c) {
+  if (!s->graphs) s->graphs = new GraphCache();
+  GraphCache& gc = *s->graphs;
+  for (StepGraph& e : gc.g)
+    if (e.scheme == c.scheme && e.var == c.var && !memcmp(&e.key, &a, sizeof(StepArgs))) {
+      e.used = ++gc.clock;
+      return e;
+    }
+  StepGraph* slot;
+  if ((int)gc.g.size() < GraphCache::kMax) {
+    gc.g.emplace_back();
+    slot = &gc.g.back();
+  } else {
+    slot = &gc.g[0];
+    for (StepGraph& e : gc.g)
+      if (e.used < slot->used) slot = &e;
+    destroy_graph(*slot);
+  }
+  slot->key = a;
+  slot->scheme = c.scheme;
+  slot->var = c.var;
+  slot->probe = capture_on(s, [&] { enqueue_probe(s, a, c); });
+  slot->stages = capture_on(s, [&] { enqueue_stages(s, a, c); });
+  slot->used = ++gc.clock;
+  return *slot;
+}
+
+static void inner_step(Session* s, const StepCfg& c) {
+  StepArgs a = make_step_args(s, c);
+  if (!s->profile && graphs_enabled()) {
+    StepGraph& g = step_graph(s, a, c);
+    DCHECK(hipGraphLaunch(g.probe, s->st));
+    DCHECK(hipEventRecord(s->ev_dt, s->st));
+    DCHECK(hipGraphLaunch(g.stages, s->st));
+  } else {
+    enqueue_probe(s, a, c);
+    DCHECK(hipEventRecord(s->ev_dt, s->st));
+    enqueue_stages(s, a, c);
   }
   if (c.order <= 1) std::swap(s->phi, s->B);
 }

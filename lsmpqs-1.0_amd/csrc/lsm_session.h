@@ -27,6 +27,9 @@ struct Timers {
   int64_t outer_steps = 0;
 };
 
+struct GraphCache;  // replayed inner-step graphs of a session (lsm_driver.hip)
+void graph_cache_free(GraphCache* c);
+
 struct Session {
   const QSS_DataArrays* key = nullptr;
   Box bx{};
@@ -60,6 +63,7 @@ struct Session {
   // step without a copy (RK orders 2 and 3, whose last stage can write to a
   // buffer other than its u^n)
   bool redirect = false;
+  GraphCache* graphs = nullptr;
   Timers tm;
 };
 

@@ -67,6 +67,10 @@ static void dalloc(P*& p, int64_t count) {
 }
 
 static void free_all(Session* s) {
+  if (s->graphs) {
+    graph_cache_free(s->graphs);
+    s->graphs = nullptr;
+  }
   void* bufs[] = {s->phi, s->A, s->B, s->S, s->D, s->phi_prev, s->mask, s->mask_w, s->mask_nw,
                   s->floor, s->phi_extra, s->vn, s->b, s->vx, s->vy, s->vz, s->gx, s->gy, s->gz,
                   s->labels, s->troot, s->roots, s->bin, s->part, s->red, s->dt, s->count,

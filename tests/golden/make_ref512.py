@@ -20,8 +20,6 @@ comparison needs (data only), as ref256_drain_imbibe.npz does:
   nneg_drain       fill voxels with phi < 0
   nw_trap_drain, w_trap_drain  voxels with mask_nw > 0, mask_w > 0
   outer_drain      outer steps in the log
-  iface_idx        every fill voxel with phi < 0 that has a face neighbour
-                   with phi >= 0, or the reverse (the interface voxels)
   wall_s           wall time of the reference's drain_model3d
 """
 import ctypes as C
@@ -78,20 +76,6 @@ def case_inputs(name, dtype):
     return pack_case((N, N, N), c["radius"], c["porosity"], c["seed"], dtype)
 
 
-def interface_voxels(phi, g):
-    """Sorted linear indices of the fill voxels whose phase (phi < 0)
-    differs from that of a face neighbour."""
-    neg = phi < 0
-    f = fill(g)
-    iface = np.zeros(neg.shape, dtype=bool)
-    for ax in range(3):
-        d = neg != np.roll(neg, 1, axis=ax)
-        iface |= d | np.roll(d, -1, axis=ax)
-    keep = np.zeros(neg.shape, dtype=bool)
-    keep[f] = True
-    return np.flatnonzero(iface & keep)
-
-
 def band_sample(phi, g, seed):
     f = np.zeros(phi.shape, dtype=bool)
     f[fill(g)] = True
@@ -139,7 +123,6 @@ def main():
         out["nneg_drain"] = int((phi[f] < 0).sum())
         out["nw_trap_drain"] = int((A["mask_nw"][f] > 0).sum())
         out["w_trap_drain"] = int((A["mask_w"][f] > 0).sum())
-        out["iface_idx"] = interface_voxels(phi, g).astype(np.int64)
         out["outer_drain"] = sum(1 for line in open(path) if "nw phase satn" in line)
         out["overlap"] = float(o.overlap)
         print(f"{name}: S_nw {out['sat_drain']:.8f}, outer {out['outer_drain']}, "

@@ -218,3 +218,22 @@ def test_labels_of_blob_fields_are_stable(shape):
             hiplib.check(lib)
             assert np.array_equal(A["connectivity"], want), (thr, rep)
         lib.lsmpqs_release(A.ptr())
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_graph_replayed_steps_equal_direct_launches(dt, monkeypatch):
+    """Inner steps replayed from captured HIP graphs (LSMPQS_GRAPHS=1)
+    against the same steps launched directly (the default): drainage then
+    imbibition, RK3 and "RK2" (phi and phi_prev swap buffers every outer
+    step, so the cached graphs rotate), bit for bit in phi and the trapped
+    masks; every replay is followed by the drivers' own eager kernels and
+    synchronizations."""
+    print(f"case graphs vs direct launches {np.dtype(dt).name}", flush=True)
+    for opts in (dict(space=5, time=3, theta_deg=30), dict(space=2, time=2, theta_deg=0)):
+        res = []
+        for graphs in ("0", "1"):
+            monkeypatch.setenv("LSMPQS_GRAPHS", graphs)
+            A, _, _ = run_model("hip", dt, 24, opts, imbibe_after=True)
+            res.append(A)
+        for f in ("phi", "mask_w", "mask_nw"):
+            assert np.array_equal(res[0][f], res[1][f]), (opts, f)

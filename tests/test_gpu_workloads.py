@@ -6,8 +6,9 @@
   the reference's 26-connected trapping and with 6-connected trapping (the
   6-connected labels also against scipy);
 * 512^3 pack (radius U[6,12], porosity 0.36), WENO5 + TVD-RK3, 30 deg, FP64:
-  one drainage curvature step against the reference build's run
-  (tests/golden/ref512_c2.npz, tests/golden/make_ref512.py);
+  one drainage curvature step (17 outer steps) against the reference build's
+  run (tests/golden/ref512_c2.npz, tests/golden/make_ref512.py; 27 minutes
+  on 8 threads here);
 * 512^3 pack (radius U[10,20], porosity 0.35), WENO5 + TVD-RK3, 39.999 deg:
   the FP64 GPU against the reference build's FP64 run
   (tests/golden/ref512_c3.npz) and the FP32 GPU against it (saturation
@@ -196,12 +197,19 @@ def test_drain512_precision_pack_fp64_and_fp32_against_reference():
     assert abs(float(o32.overlap) - float(ref["overlap"])) < 1e-7
     sat32, band32, neg32 = _compare_with_fixture(keep32, g32, ref, "fp32 gpu", outer32,
                                                  exact=False)
-    iface = ref["iface_idx"]
-    mism = int((neg32.ravel()[iface] != neg64.ravel()[iface]).sum())
-    print(f"  fp32: interface voxels in the other phase {mism} of {iface.size} "
-          f"({mism / iface.size:.2e})", flush=True)
+    # interface voxels of the FP64 state (a face neighbour in the other
+    # phase), which the fixture pins through its counts and band samples
+    f = fill(g64)
+    n64 = neg64[f]
+    iface = np.zeros_like(n64)
+    for ax in range(3):
+        d = n64 != np.roll(n64, 1, axis=ax)
+        iface |= d | np.roll(d, -1, axis=ax)
+    mism = int((neg32[f] != n64).sum())
+    print(f"  fp32: voxels in the other phase {mism} of {int(iface.sum())} interface voxels "
+          f"({mism / max(1, int(iface.sum())):.2e})", flush=True)
     assert abs(sat32 - float(ref["sat_drain"])) <= 1e-4
-    assert mism <= 0.01 * iface.size
+    assert mism <= 0.01 * int(iface.sum())
 
 
 def test_fp32_against_fp64_at_1024():
