@@ -788,6 +788,9 @@ def main():
                          "--curve-tmax)")
     ap.add_argument("--stage-only", action="store_true",
                     help="only the headline inner steps in FP32 (for kernel profiles)")
+    ap.add_argument("--schemes-only", action="store_true",
+                    help="only the ENO2 / ENO3 step rates (--n, --steps, --warmup; for kernel "
+                         "profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -830,6 +833,15 @@ def main():
     print(f"# input: {n}^3 pack, {len(radii)} spheres, porosity {por:.4f}, "
           f"{time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
     a, b = 0.02, 0.04
+    if args.schemes_only:
+        out = {}
+        for (sp, to) in ((2, 2), (3, 3)):
+            for key in args.precisions.split(","):
+                dt_ = np.float32 if key == "f32" else np.float64
+                r = scheme_leg(dt_, g64, mask, phi0, args.steps, args.warmup, sp, to)
+                out[f"{r['scheme']}_{key}"] = r
+        print(json.dumps({"schemes": out}), flush=True)
+        return
     r32 = gpu_leg(np.float32, g64, mask, phi0, args.steps, args.warmup, a, b)
     print(f"# fp32 done: {r32['elapsed_ms'] / args.steps:.3f} ms/step", file=sys.stderr, flush=True)
     if args.stage_only or "f64" not in args.precisions.split(","):

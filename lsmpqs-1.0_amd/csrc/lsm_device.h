@@ -85,6 +85,38 @@ __device__ __forceinline__ void eno3(const T* p, T inv_plus, T inv_minus, T& plu
   minus = v * inv_minus;
 }
 
+// Fast single-precision HJ-ENO2 / HJ-ENO3: the same stencil choices as eno2
+// / eno3, with FMA contraction, selections instead of branches, the
+// comparisons the two sides share evaluated once, and the one-sided values
+// left unscaled (times dx): the caller applies sign(vel_n)/dx per component,
+// as for the fast WENO5.  Rounding differs from the reference's expression
+// order by a few ulp, so a stencil choice at a near-tie can differ.
+template <typename T>
+__host__ __device__ __forceinline__ void eno2_fast_raw(const T* p, T& plus, T& minus) {
+#pragma clang fp contract(fast)
+  const T D1 = p[3] - p[2], D1p = p[4] - p[3], D1m = p[2] - p[1], D1pp = p[5] - p[4];
+  const T D2 = D1p - D1, D2p = D1pp - D1p, D2m = D1 - D1m;
+  plus = T(-0.5) * (fabs(D2) < fabs(D2p) ? D2 : D2p) + D1p;
+  minus = T(0.5) * (fabs(D2m) < fabs(D2) ? D2m : D2) + D1;
+}
+
+template <typename T>
+__host__ __device__ __forceinline__ void eno3_fast_raw(const T* p, T& plus, T& minus) {
+#pragma clang fp contract(fast)
+  const T D1 = p[3] - p[2], D1p = p[4] - p[3], D1m = p[2] - p[1];
+  const T D1mm = p[1] - p[0], D1pp = p[5] - p[4], D1ppp = p[6] - p[5];
+  const T D2 = D1p - D1, D2p = D1pp - D1p, D2m = D1 - D1m;
+  const T D2pp = D1ppp - D1pp, D2mm = D1m - D1mm;
+  const T D3 = D2 - D2m, D3p = D2p - D2, D3pp = D2pp - D2p, D3m = D2m - D2mm;
+  const bool pl = fabs(D2) < fabs(D2p);    // plus side: stencil widens to the left
+  const bool mi = fabs(D2m) < fabs(D2);    // minus side: stencil widens to the left
+  const bool c0 = fabs(D3) < fabs(D3p);    // shared by both sides
+  const T s3p = pl ? (c0 ? D3 : D3p) : (fabs(D3p) < fabs(D3pp) ? D3p : D3pp);
+  const T s3m = mi ? (fabs(D3m) < fabs(D3) ? D3m : D3) : (c0 ? D3 : D3p);
+  plus = (pl ? T(-1.0 / 6.0) : T(1.0 / 3.0)) * s3p + (T(-0.5) * (pl ? D2 : D2p) + D1p);
+  minus = (mi ? T(1.0 / 3.0) : T(-1.0 / 6.0)) * s3m + (T(0.5) * (mi ? D2m : D2) + D1);
+}
+
 // HJ-WENO5 combination with linear weights 0.1/0.6/0.3 (:1960-2035)
 template <typename T>
 __host__ __device__ __forceinline__ T weno5_combine(T v1, T v2, T v3, T v4, T v5) {

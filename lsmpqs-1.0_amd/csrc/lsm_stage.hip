@@ -140,6 +140,7 @@ struct PointIn {
 template <int S, bool VAR, int STAGE, bool FINAL, bool FAST>
 __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile) {
   constexpr bool FW = FAST && S == WENO5;
+  constexpr bool FE = FAST && S != WENO5;  // fast ENO2 / ENO3 (eno*_fast_raw)
   __shared__ real pl[NSLOT][PY * PX];
   __shared__ real xp[FW ? 2 : 1][FW ? TY : 1][TX + 1];  // phi_x^+ by face, double-buffered
   __shared__ real yp[FW ? 2 : 1][FW ? TY + 1 : 1][TX];  // phi_y^+ by face
@@ -289,25 +290,39 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile) {
           ly[o] = P[me + (o - 3) * PX];
         }
         if (S == ENO3 && a.eno3_quirk) lx[6] = eno3_quirk_read(in, bx, i, j);
-        axis_derivs<real, S>(lx, inv_dx, inv_dx, d[0], d[1]);
-        axis_derivs<real, S>(ly, inv_dy, inv_dx, d[2], d[3]);
-        axis_derivs<real, S>(w, inv_dz, inv_dx, d[4], d[5]);
+        if constexpr (FE) {  // unscaled; sign(vel_n)/dx applied below
+          if constexpr (S == ENO2) {
+            eno2_fast_raw(lx, d[0], d[1]);
+            eno2_fast_raw(ly, d[2], d[3]);
+            eno2_fast_raw(w, d[4], d[5]);
+          } else {
+            eno3_fast_raw(lx, d[0], d[1]);
+            eno3_fast_raw(ly, d[2], d[3]);
+            eno3_fast_raw(w, d[4], d[5]);
+          }
+        } else {
+          axis_derivs<real, S>(lx, inv_dx, inv_dx, d[0], d[1]);
+          axis_derivs<real, S>(ly, inv_dy, inv_dx, d[2], d[3]);
+          axis_derivs<real, S>(w, inv_dz, inv_dx, d[4], d[5]);
+        }
       }
 
       real adv_sign = real(-1);
-      if constexpr (FW) {
-        // one-sided derivatives times sign(vel_n), so that the Godunov
-        // selection needs no per-axis sign products
+      if constexpr (FW || FE) {
+        // one-sided derivatives times sign(vel_n)/dx, so that the Godunov
+        // selection needs no per-axis sign products; the reference scales
+        // the ENO3 y- and z- minus sides by 1/dx (qss_spatial_derivatives3d.f:823,900)
         const real sg = vn > real(0) ? real(1) : real(-1);
         const real gx = sg * inv_dx, gy = sg * inv_dy, gz = sg * inv_dz;
+        const real gym = S == ENO3 ? gx : gy, gzm = S == ENO3 ? gx : gz;
         d[0] *= gx; d[1] *= gx;
-        d[2] *= gy; d[3] *= gy;
-        d[4] *= gz; d[5] *= gz;
+        d[2] *= gy; d[3] *= gym;
+        d[4] *= gz; d[5] *= gzm;
         adv_sign = -sg;
       }
       real temp_norm;
       if (fabs(vn) >= real(1.e-5)) {
-        real g2 = FW ? godunov_sq_signed(d) : godunov_sq(vn, d);
+        real g2 = (FW || FE) ? godunov_sq_signed(d) : godunov_sq(vn, d);
         real gn;
         if constexpr (FAST) gn = __builtin_amdgcn_sqrtf(g2);
         else gn = sqrt(g2);
@@ -315,22 +330,29 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile) {
       } else {
         temp_norm = (VAR && a.nstale > 0) ? stale_norm<S>(a, idx) : real(0);
       }
-      const real* Pp = pl[sp];
-      const real* Pm = pl[sm];
-      real xy[4] = {P[me + PX + 1], P[me - PX - 1], P[me + PX - 1], P[me - PX + 1]};
-      real xz[4] = {Pp[me + 1], Pm[me - 1], Pp[me - 1], Pm[me + 1]};
-      real yz[4] = {Pp[me + PX], Pm[me - PX], Pp[me - PX], Pm[me + PX]};
-      real curv;
-      if constexpr (FAST) {
-        if (iso)
-          curv = curvature_iso(lx[3], lx[1], lx[2], lx[4], lx[5], ly[1], ly[2], ly[4], ly[5], w[1],
-                               w[2], w[4], w[5], xy, xz, yz, f2, curv_thr);
-        else
-          curv = curvature_fast(lx[3], lx[1], lx[2], lx[4], lx[5], ly[1], ly[2], ly[4], ly[5],
-                                w[1], w[2], w[4], w[5], xy, xz, yz, fx, fy, fz);
-      } else
-        curv = curvature(lx[3], lx[1], lx[2], lx[4], lx[5], ly[1], ly[2], ly[4], ly[5], w[1],
-                         w[2], w[4], w[5], xy, xz, yz, fx, fy, fz);
+      // On the contact-angle path the curvature coefficient is h1 b, exactly
+      // 0 deeper than 1.5 dx inside the solid, where the curvature term
+      // b kappa |grad phi| is 0 whatever kappa is (up to the sign of a zero
+      // term): a wave whose voxels all have b = 0 skips it.
+      real curv = real(0);
+      if (!VAR || bv != real(0)) {
+        const real* Pp = pl[sp];
+        const real* Pm = pl[sm];
+        real xy[4] = {P[me + PX + 1], P[me - PX - 1], P[me + PX - 1], P[me - PX + 1]};
+        real xz[4] = {Pp[me + 1], Pm[me - 1], Pp[me - 1], Pm[me + 1]};
+        real yz[4] = {Pp[me + PX], Pm[me - PX], Pp[me - PX], Pm[me + PX]};
+        if constexpr (FAST) {
+          if (iso)
+            curv = curvature_iso(lx[3], lx[1], lx[2], lx[4], lx[5], ly[1], ly[2], ly[4], ly[5],
+                                 w[1], w[2], w[4], w[5], xy, xz, yz, f2, curv_thr);
+          else
+            curv = curvature_fast(lx[3], lx[1], lx[2], lx[4], lx[5], ly[1], ly[2], ly[4], ly[5],
+                                  w[1], w[2], w[4], w[5], xy, xz, yz, fx, fy, fz);
+        } else {
+          curv = curvature(lx[3], lx[1], lx[2], lx[4], lx[5], ly[1], ly[2], ly[4], ly[5], w[1],
+                           w[2], w[4], w[5], xy, xz, yz, fx, fy, fz);
+        }
+      }
       real rhs;
       if (VAR) {
         real ax = vxv >= real(0) ? d[1] : d[0];

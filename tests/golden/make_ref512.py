@@ -5,8 +5,8 @@ oracle/Makefile.ref) on one curvature step of a 512^3 drainage case.
     python tests/golden/make_ref512.py c2|c3 [threads]
 
 Cases (seeded Boolean sphere packs, HJ-WENO5 + TVD-RK3, trapping on,
-eps_stop 1e-4 dx, pseudo-time 10 per curvature step, one drainage
-curvature step from a = 0.02 with b = 0.04):
+eps_stop 1e-4 dx, one drainage curvature step from a = 0.02 with b = 0.04,
+pseudo-time 10 (c2) or 4 (c3) in it):
   c2  radius U[6,12],  porosity 0.36, seed 1002, contact angle 30 deg
   c3  radius U[10,20], porosity 0.35, seed 1003, contact angle 39.999 deg
       (overlap 0.3 dx in both precisions; at exactly 40 deg the
@@ -65,7 +65,7 @@ def case_options(name, dtype):
     o.b = 0.04
     o.dc = 0.25
     o.amax = o.a + o.b * o.dc * 0.5
-    o.tmax = 10.0
+    o.tmax = 10.0 if name == "c2" else 4.0
     o.eps_stop = 1e-4
     o.check_connectivity = 1
     return o

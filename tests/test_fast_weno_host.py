@@ -43,3 +43,22 @@ def test_weno_face_adjoint_matches_differences():
     assert f32 < 1e-4
     assert flat < 1e-15
     assert line < 1e-6   # weno5_line_adj, the FP64 adjoint's reference-order WENO5
+
+
+@pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not installed")
+def test_fast_eno_matches_osher_shu_construction():
+    """eno2_fast_raw / eno3_fast_raw, the FP32 library's ENO2 and ENO3
+    one-sided derivatives, against the Osher-Shu ENO construction written
+    from its definition (tests/native/fast_eno_check.cpp): exact to rounding
+    in double, to float accuracy in float away from near-ties of a stencil
+    choice."""
+    exe = os.path.join(tempfile.mkdtemp(prefix="lsm_fe_"), "fast_eno_check")
+    src = os.path.join(ROOT, "tests", "native", "fast_eno_check.cpp")
+    subprocess.run([HIPCC, "-O1", "-std=c++17", "-x", "hip", "--offload-arch=gfx950",
+                    "-I", os.path.join(ROOT, "lsmpqs-1.0_amd", "csrc"), "-I", os.path.join(ROOT, "include"),
+                    "-DQSSLIB_REAL=float", src, "-o", exe], check=True, capture_output=True)
+    out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split()
+    w64, w32, n32 = float(out[0]), float(out[1]), int(out[2])
+    assert w64 < 1e-12
+    assert w32 < 1e-5
+    assert n32 > 1000000
