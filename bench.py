@@ -518,6 +518,12 @@ def precision_leg(n=512, curv_steps=40, tmax=20.0, seed=1003):
                 ((states["f32"]["mask_w"][fb] > 0) != (states["f64"]["mask_w"][fb] > 0))
     dev = np.abs(p32 - p64)
     same_trap = band & ~trap_diff
+    # band voxels at least 4 voxels (26-neighbour steps) from any voxel whose
+    # trapped state differs between the precisions
+    from scipy import ndimage
+    near_trap = ndimage.binary_dilation(trap_diff, structure=np.ones((3, 3, 3), bool),
+                                        iterations=3)
+    away = band & ~near_trap
     out = {"grid": n, "spheres": int(len(radii)), "porosity": round(float(por), 4),
            "curvature_steps": curv_steps, "tmax": tmax, "f32": res["f32"], "f64": res["f64"],
            "fp64_over_fp32_wall": round(res["f64"]["wall_s"] / res["f32"]["wall_s"], 3),
@@ -525,6 +531,8 @@ def precision_leg(n=512, curv_steps=40, tmax=20.0, seed=1003):
            "sat_abs_diff": abs(sat(p32) - sat(p64)),
            "phi_linf_band": float(dev[band].max()),
            "phi_linf_band_same_trapping": float(dev[same_trap].max()) if same_trap.any() else 0.0,
+           "phi_linf_band_away_from_trapping_diff": float(dev[away].max()) if away.any() else 0.0,
+           "band_voxels_near_trapping_diff": int((band & near_trap).sum()),
            "phi_p999_band": float(np.quantile(dev[band], 0.999)),
            "band_voxels": int(band.sum()),
            "phase_mismatch_voxels": flips, "interface_voxels": int(iface.sum()),
