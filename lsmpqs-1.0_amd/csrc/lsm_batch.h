@@ -38,6 +38,12 @@ struct BatchHook {
   // runs one inner step of box s (after the box's own stream is idle) and
   // returns its dt; the order-1 phi/B swap is left to the caller
   virtual real step(Session* s, const StepCfg& c) = 0;
+  // the box has left its inner loop for outer-step work (trapping,
+  // reinitialization, the next curvature step); batched steps of the other
+  // boxes need not wait for it until it asks for its next step
+  virtual void away() {}
+  // progress marks of the box's set-up (host-time tracing only)
+  virtual void mark(int phase) { (void)phase; }
 };
 BatchHook*& batch_hook();  // thread-local, nullptr outside box threads
 

@@ -19,6 +19,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <algorithm>
+#include <chrono>
 #include <condition_variable>
 #include <mutex>
 #include <stdexcept>
@@ -60,10 +61,33 @@ class Batch final : public BatchHook {
   // holding a slot never waits for another box, so every slot comes back.
   void set_slots(int n) { slots_ = n < 1 ? 1 : n; }
   void acquire_slot() {
+    const double t0 = now_s();
     std::unique_lock<std::mutex> lk(slot_mu_);
     cv_slot_.wait(lk, [&] { return slots_ > 0; });
     slots_ -= 1;
+    trace().slot_s += now_s() - t0;
   }
+
+  // Host-time breakdown of one box thread (LSMPQS_BATCH_TRACE=1 prints the
+  // sums and maxima over the boxes to stderr after the run).
+  struct BoxTrace {
+    double start = 0, first_step = -1, last_step = 0, end = 0;
+    double marks[4] = {-1, -1, -1, -1};  // first time each set-up mark was passed
+    double away_since = -1, away_s = 0;  // outer-step work (away() to the next step)
+    long long aways = 0;
+    double slot_s = 0, parked_s = 0;
+    long long steps = 0;
+  };
+  static double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+  }
+  static BoxTrace& trace() {
+    static thread_local BoxTrace t;
+    return t;
+  }
+  std::vector<BoxTrace> traces;  // filled by box_finished
+  double sched_launch_s = 0, sched_wait_s = 0;
   void release_slot() {
     {
       std::lock_guard<std::mutex> lk(slot_mu_);
@@ -79,6 +103,12 @@ class Batch final : public BatchHook {
     release_slot();
     {
       std::unique_lock<std::mutex> lk(mu_);
+      if (box_away()) {
+        box_away() = false;
+        away_ -= 1;
+        BoxTrace& t = trace();
+        if (t.away_since >= 0) { t.away_s += now_s() - t.away_since; t.aways += 1; t.away_since = -1; }
+      }
       if (!failure_.empty()) {
         lk.unlock();
         acquire_slot();
@@ -86,7 +116,13 @@ class Batch final : public BatchHook {
       }
       parked_.push_back(&r);
       cv_sched_.notify_one();
+      BoxTrace& tr = trace();
+      const double t0 = now_s();
+      if (tr.first_step < 0) tr.first_step = t0;
       cv_box_.wait(lk, [&] { return r.done; });
+      tr.last_step = now_s();
+      tr.parked_s += tr.last_step - t0;
+      tr.steps += 1;
     }
     acquire_slot();
     std::lock_guard<std::mutex> lk(mu_);
@@ -94,25 +130,58 @@ class Batch final : public BatchHook {
     return r.dt;
   }
 
+  void mark(int phase) override {
+    BoxTrace& t = trace();
+    if (phase >= 0 && phase < 4 && t.marks[phase] < 0) t.marks[phase] = now_s();
+  }
+
+  // box thread: outer-step work until the next step()
+  void away() override {
+    std::lock_guard<std::mutex> lk(mu_);
+    if (!box_away()) {
+      box_away() = true;
+      away_ += 1;
+      trace().away_since = now_s();
+    }
+    cv_sched_.notify_one();
+  }
+
   void box_finished() {
     release_slot();
     std::lock_guard<std::mutex> lk(mu_);
+    trace().end = now_s();
+    traces.push_back(trace());
+    if (box_away()) {
+      box_away() = false;
+      away_ -= 1;
+    }
     live_ -= 1;
     cv_sched_.notify_one();
   }
 
-  // calling thread: serve rounds until every box has finished
+  // calling thread: serve rounds until every box has finished.  A round
+  // runs the step of every parked box once each live box is parked or away
+  // (in outer-step work, which then overlaps the round on the GPU).  Before
+  // the first step no box is away, so the first round, which also picks the
+  // brick shape, holds every box.  Which round serves a box's step changes
+  // nothing in its arithmetic.
   void run() {
     for (;;) {
       std::vector<Request*> round;
+      const double w0 = now_s();
       {
         std::unique_lock<std::mutex> lk(mu_);
-        cv_sched_.wait(lk, [&] { return (int)parked_.size() == live_; });
+        cv_sched_.wait(lk, [&] {
+          return live_ == 0 || (!parked_.empty() && (int)parked_.size() + away_ == live_);
+        });
         if (live_ == 0) break;
         round.swap(parked_);
       }
+      const double w1 = now_s();
+      sched_wait_s += w1 - w0;
       try {
         launch(round);
+        sched_launch_s += now_s() - w1;
       } catch (const std::exception& e) {
         std::lock_guard<std::mutex> lk(mu_);
         failure_ = std::string("batched step: ") + e.what();
@@ -285,6 +354,11 @@ class Batch final : public BatchHook {
   int slots_ = 1 << 30;
   std::vector<Request*> parked_;
   int live_;
+  int away_ = 0;  // live boxes between leaving an inner loop and their next step
+  static bool& box_away() {
+    static thread_local bool a = false;
+    return a;
+  }
   std::string failure_;
 
   int nmax_;
@@ -344,6 +418,7 @@ extern "C" int lsmpqs_batch_model3d(int n, Options** options, QSS_DataArrays** p
   threads.reserve(n);
   for (int i = 0; i < n; ++i) {
     threads.emplace_back([&, i] {
+      Batch::trace().start = Batch::now_s();
       batch.acquire_slot();
       try {
         if (hipSetDevice(dev) != hipSuccess) throw std::runtime_error("hipSetDevice failed");
@@ -356,8 +431,40 @@ extern "C" int lsmpqs_batch_model3d(int n, Options** options, QSS_DataArrays** p
       batch.box_finished();
     });
   }
+  const double t_run = Batch::now_s();
   batch.run();
   for (auto& t : threads) t.join();
+  if (getenv("LSMPQS_BATCH_TRACE") && atoi(getenv("LSMPQS_BATCH_TRACE")) != 0) {
+    // per box: set-up (start to first step), parked (waiting for batched
+    // steps), own work between steps, tail (last step to end), slot waits
+    double su = 0, pk = 0, own = 0, tl = 0, sl = 0, msu = 0, mtl = 0, mend = 0;
+    double ph[5] = {0, 0, 0, 0, 0};  // session, model set-up, coefficients, first trapping, to step
+    double aw = 0;
+    long long naw = 0;
+    for (const auto& t : batch.traces) {
+      double prev = t.start;
+      for (int q = 0; q < 4; ++q)
+        if (t.marks[q] >= 0) { ph[q] += t.marks[q] - prev; prev = t.marks[q]; }
+      if (t.first_step >= 0) ph[4] += t.first_step - prev;
+      aw += t.away_s;
+      naw += t.aways;
+      const double setup = (t.first_step < 0 ? t.end : t.first_step) - t.start;
+      const double tail = t.end - (t.first_step < 0 ? t.end : t.last_step);
+      su += setup; pk += t.parked_s; tl += tail; sl += t.slot_s;
+      own += (t.end - t.start) - setup - tail - t.parked_s;
+      msu = std::max(msu, setup); mtl = std::max(mtl, tail); mend = std::max(mend, t.end - t_run);
+    }
+    fprintf(stderr,
+            "lsmpqs batch trace: %d boxes, wall %.3f s; scheduler waiting %.3f s, in steps %.3f s; "
+            "per box sums: set-up %.3f (max %.3f), parked %.3f, own work %.3f, tail %.3f (max "
+            "%.3f), slot waits %.3f s\n",
+            n, mend, batch.sched_wait_s, batch.sched_launch_s, su, msu, pk, own, tl, mtl, sl);
+    fprintf(stderr,
+            "lsmpqs batch trace: set-up sums: session %.3f, model set-up %.3f, coefficients "
+            "%.3f, first trapping %.3f, to the first step %.3f s; outer-step work %.3f s in %lld "
+            "stretches\n",
+            ph[0], ph[1], ph[2], ph[3], ph[4], aw, naw);
+  }
   if (stats) {
     *stats = batch.stats;
     stats->boxes = n;

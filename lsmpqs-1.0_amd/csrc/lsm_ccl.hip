@@ -109,8 +109,11 @@ __device__ __forceinline__ void unite_planes(int* p, unsigned need, int tx, int 
 // roots: [count, tile root, tile root, ...] with room for `cap` roots; every
 // tile root is appended (when the list is full, the count still grows and
 // the root-jump passes scan T instead)
+// With all = false only the tile roots' entries of L are written (they are
+// the only entries read before a full labelling's resolve pass).
 __global__ __launch_bounds__(CX * CY) void ccl_local_kernel(Box bx, const uint8_t* bin, int* L,
-                                                          int* T, int conn, int* roots, int cap) {
+                                                          int* T, int conn, int* roots, int cap,
+                                                          bool all) {
   __shared__ int p[CN];
   // bit z of cm[ty + 1][tx + 1] is voxel (tx, ty, z) of the tile; the frame
   // of one column around the tile reads as background
@@ -189,7 +192,7 @@ __global__ __launch_bounds__(CX * CY) void ccl_local_kernel(Box bx, const uint8_
         if (q < cap) roots[1 + q] = g;
       }
     }
-    L[g] = root;
+    if (all || root == g) L[g] = root;
     T[g] = root;
   }
 }
@@ -333,7 +336,7 @@ hipError_t launch_label_components(const Box& bx, const uint8_t* bin, int* L, in
                        : grid_for(n);
   if (roots) hipLaunchKernelGGL(ccl_reset_kernel, dim3(1), dim3(1), 0, st, roots);
   hipLaunchKernelGGL(ccl_local_kernel, tiles, dim3(CX, CY, 1), 0, st, bx, bin, L, T, conn, roots,
-                     cap);
+                     cap, full);
   hipLaunchKernelGGL(ccl_border_kernel<0>, tiles, dim3(256), 0, st, bx, L, (const int*)T, conn);
   hipLaunchKernelGGL(ccl_jump_kernel, dim3(gj), dim3(256), 0, st, L, (const int*)T, (int)n,
                      (const int*)roots, cap);
@@ -362,21 +365,41 @@ __device__ __forceinline__ int conn_value(const int* L, const int* T, int64_t id
   return t >= 0 ? L[t] : 0;
 }
 
+// The trapping passes take UNR voxels per thread and iteration (stride
+// apart), with all their loads issued before any value is used, so that a
+// lane has UNR voxels' loads in flight instead of one (the two wetting-phase
+// passes below read a label through the voxel's tile root, two dependent
+// loads).
+constexpr int UNR = 4;
+
 // phi = merge(phi, mask_nw); bin = phi < 0 (force_bin_kernel then sets the
 // inlet voxels)
 __global__ void trap_nw_bin_kernel(TrapArgs a) {
   const int64_t n = a.bx.size();
-  for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n;
-       l += (int64_t)gridDim.x * blockDim.x) {
-    real p = a.phi[l];
-    if (a.impose_mask) {  // phi = max(phi, mask) first (the var path's imposition)
-      const real m = a.impose_mask[l];
-      if (m > p) p = m;
+  real* __restrict__ phi = a.phi;
+  const real* __restrict__ im = a.impose_mask;
+  const real* __restrict__ mnw = a.mask_nw;
+  uint8_t* __restrict__ bin = a.bin;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t l0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l0 < n; l0 += UNR * stride) {
+    real p[UNR], m[UNR], mn[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      const int64_t l = min(l0 + u * stride, n - 1);
+      p[u] = phi[l];
+      m[u] = im ? im[l] : real(0);
+      mn[u] = mnw[l];
     }
-    real mn = a.mask_nw[l];
-    if (mn > real(0)) p = -mn;
-    if (a.impose_mask || mn > real(0)) a.phi[l] = p;
-    a.bin[l] = p < real(0) ? 1 : 0;
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      const int64_t l = l0 + u * stride;
+      if (l >= n) break;
+      real v = p[u];
+      if (im && m[u] > v) v = m[u];  // phi = max(phi, mask) first (the var path's imposition)
+      if (mn[u] > real(0)) v = -mn[u];
+      if (im || mn[u] > real(0)) phi[l] = v;
+      bin[l] = v < real(0) ? 1 : 0;
+    }
   }
 }
 
@@ -384,20 +407,42 @@ __global__ void trap_nw_bin_kernel(TrapArgs a) {
 // bin = s < 0 (force_bin_kernel then sets the outlet voxels)
 __global__ void trap_w_bin_kernel(TrapArgs a) {
   const int64_t n = a.bx.size();
-  int main_val = conn_value(a.labels, a.troot, a.nw_ind);
-  for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n;
-       l += (int64_t)gridDim.x * blockDim.x) {
-    int cv = conn_value(a.labels, a.troot, l);
-    bool disc = !(cv == main_val || cv == 0);
-    real p = a.phi[l];
-    real mn = (disc && p < real(0)) ? -p : a.fill;
-    a.mask_nw[l] = mn;
-    real s = -p;
-    real m = a.mask[l];
-    s = (m > s) ? m : s;
-    s = (mn > s) ? mn : s;
-    a.s[l] = s;
-    a.bin[l] = s < real(0) ? 1 : 0;
+  const int* __restrict__ T = a.troot;
+  const int* __restrict__ L = a.labels;
+  const real* __restrict__ phi = a.phi;
+  const real* __restrict__ mask = a.mask;
+  real* __restrict__ mnw = a.mask_nw;
+  real* __restrict__ sw = a.s;
+  uint8_t* __restrict__ bin = a.bin;
+  const int main_val = conn_value(L, T, a.nw_ind);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t l0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l0 < n; l0 += UNR * stride) {
+    int t[UNR];
+    real p[UNR], m[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      const int64_t l = l0 + u * stride;
+      const bool in = l < n;
+      t[u] = in ? T[l] : -1;
+      p[u] = in ? phi[l] : real(0);
+      m[u] = in ? mask[l] : real(0);
+    }
+    int cv[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) cv[u] = t[u] >= 0 ? L[t[u]] : 0;
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      const int64_t l = l0 + u * stride;
+      if (l >= n) break;
+      const bool disc = !(cv[u] == main_val || cv[u] == 0);
+      const real mn = (disc && p[u] < real(0)) ? -p[u] : a.fill;
+      mnw[l] = mn;
+      real v = -p[u];
+      v = (m[u] > v) ? m[u] : v;
+      v = (mn > v) ? mn : v;
+      sw[l] = v;
+      bin[l] = v < real(0) ? 1 : 0;
+    }
   }
 }
 
@@ -479,14 +524,33 @@ __global__ void force_bin_kernel(uint8_t* bin, Box bx, ForceSet f) {
 }
 
 __global__ void trap_w_mask_kernel(TrapArgs a) {
-  int64_t n = a.bx.size();
-  int main_val = conn_value(a.labels, a.troot, a.w_ind);
-  for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n;
-       l += (int64_t)gridDim.x * blockDim.x) {
-    int cv = conn_value(a.labels, a.troot, l);
-    bool disc = !(cv == main_val || cv == 0);
-    real s = a.s[l];
-    a.mask_w[l] = (disc && s < real(0)) ? -s : a.fill;
+  const int64_t n = a.bx.size();
+  const int* __restrict__ T = a.troot;
+  const int* __restrict__ L = a.labels;
+  const real* __restrict__ sw = a.s;
+  real* __restrict__ mw = a.mask_w;
+  const int main_val = conn_value(L, T, a.w_ind);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t l0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l0 < n; l0 += UNR * stride) {
+    int t[UNR];
+    real v[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      const int64_t l = l0 + u * stride;
+      const bool in = l < n;
+      t[u] = in ? T[l] : -1;
+      v[u] = in ? sw[l] : real(0);
+    }
+    int cv[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) cv[u] = t[u] >= 0 ? L[t[u]] : 0;
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      const int64_t l = l0 + u * stride;
+      if (l >= n) break;
+      const bool disc = !(cv[u] == main_val || cv[u] == 0);
+      mw[l] = (disc && v[u] < real(0)) ? -v[u] : a.fill;
+    }
   }
 }
 

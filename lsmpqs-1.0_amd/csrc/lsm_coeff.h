@@ -80,6 +80,34 @@ __device__ __forceinline__ void for_rows(const Box& b, F&& f) {
   }
 }
 
+// for_rows with U voxels of its row per lane and pass (64 apart): load(u,
+// idx) for all U first, at indices clamped to the row so that every load is
+// in bounds, then use(u, idx, i, j, k) for those inside the row.  A lane then
+// has U voxels' loads in flight at once instead of one, which latency-bound
+// stencil passes need to approach the HBM rate.
+template <bool FILL, int U, typename Ld, typename F>
+__device__ __forceinline__ void for_rows_u(const Box& b, Ld&& load, F&& use) {
+  int i0, nxr, j0, nyr, k0, nzr;
+  row_extent<FILL>(b, i0, nxr, j0, nyr, k0, nzr);
+  const unsigned nrows = (unsigned)nyr * (unsigned)nzr;
+  const int64_t plane = (int64_t)b.nx * b.ny;
+  const unsigned blk = (unsigned)xcd_tile(blockIdx.x, gridDim.x);
+  for (unsigned r = blk * ROW_TY + threadIdx.y; r < nrows; r += gridDim.x * ROW_TY) {
+    const unsigned kk = r / (unsigned)nyr, jj = r - kk * (unsigned)nyr;
+    const int j = j0 + (int)jj, k = k0 + (int)kk;
+    const int64_t base = (int64_t)k * plane + (int64_t)j * b.nx + i0;
+    for (int x0 = threadIdx.x; x0 < nxr; x0 += ROW_TX * U) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) load(u, base + min(x0 + u * ROW_TX, nxr - 1));
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int x = x0 + u * ROW_TX;
+        if (x < nxr) use(u, base + x, i0 + x, j, k);
+      }
+    }
+  }
+}
+
 // Heaviside weights, |grad mask| and the smoothed sign of the mask of one
 // voxel (reference: qss_spatial_derivatives3d.f:13-87,178-298).
 __device__ __forceinline__ void coeff_common(real m, real mx, real my, real mz, real dx, real& h1,

@@ -66,8 +66,11 @@ struct ReduceReq {
 // result; A, B and S serve as next-iterate, phi0 copy and distance buffers.
 // With `red`, the outer reductions of the result are left in s->red (fused
 // into the single-iteration pass when there is one).
+// fast: the FP32 library's fast arithmetic (the model drivers; off with the
+// exact-arithmetic switch); the exported reinitializeSubcellFix3d is exact.
+static bool fast_fp32() { return sizeof(real) == 4 && !(quirks() & EXACT_FP32); }
 static void reinit_dev(Session* s, real*& f, const Grid* g, const Options* o,
-                       const ReduceReq* red = nullptr) {
+                       const ReduceReq* red = nullptr, bool fast = false) {
   const Box& bx = s->bx;
   real t_r = 0, dt_r = real(0.5) * g->dx[0];
   auto reduce_after = [&]() {
@@ -84,9 +87,10 @@ static void reinit_dev(Session* s, real*& f, const Grid* g, const Options* o,
     if (red)
       DCHECK(launch_reinit_first_step_reduce(bx, s->A, f, dt_r, g->dx[0], g->dx[1], g->dx[2],
                                              red->prev, red->zone, red->eps, red->dV, s->part,
-                                             s->red, s->st));
+                                             s->red, s->st, fast));
     else
-      DCHECK(launch_reinit_first_step(bx, s->A, f, dt_r, g->dx[0], g->dx[1], g->dx[2], s->st));
+      DCHECK(launch_reinit_first_step(bx, s->A, f, dt_r, g->dx[0], g->dx[1], g->dx[2], s->st,
+                                      fast));
     BcArgs ba{};
     ba.bx = bx; ba.src = s->A; ba.dst = s->A;
     DCHECK(launch_bc(ba, false, s->st));
@@ -96,9 +100,10 @@ static void reinit_dev(Session* s, real*& f, const Grid* g, const Options* o,
   real* phi0 = s->B;
   real* dist0 = s->S;
   DCHECK(hipMemcpyAsync(phi0, f, bytes(s), hipMemcpyDeviceToDevice, s->st));
-  DCHECK(launch_reinit_distance(bx, dist0, phi0, g->dx[0], g->dx[1], g->dx[2], s->st));
+  DCHECK(launch_reinit_distance(bx, dist0, phi0, g->dx[0], g->dx[1], g->dx[2], s->st, fast));
   while (t_r < o->tmax_r) {
-    DCHECK(launch_reinit_step(bx, s->A, f, phi0, dist0, dt_r, g->dx[0], g->dx[1], g->dx[2], s->st));
+    DCHECK(launch_reinit_step(bx, s->A, f, phi0, dist0, dt_r, g->dx[0], g->dx[1], g->dx[2], s->st,
+                              fast));
     BcArgs ba{};
     ba.bx = bx; ba.src = s->A; ba.dst = s->A;
     DCHECK(launch_bc(ba, false, s->st));
@@ -451,6 +456,7 @@ static real inner_loop(Session* s, const StepCfg& c, real tplot, real* last_dt) 
     dt_sub += dt;
     s->tm.inner_steps++;
   }
+  if (hook) hook->away();
   if (last_dt) *last_dt = dt;
   return dt_sub;
 }
@@ -513,8 +519,8 @@ static real ccm_dev(Session* s, Options* o, const Grid* g, FILE* fp, bool var, r
       DCHECK(launch_impose(s->phi, s->mask, s->phi, 0, s->n, s->st));
     if (o->check_connectivity) {
       trap_dev(s, g, o, false, var ? s->mask : nullptr);
-      reinit_dev(s, s->mask_w, g, o);
-      reinit_dev(s, s->mask_nw, g, o);
+      reinit_dev(s, s->mask_w, g, o, nullptr, fast_fp32());
+      reinit_dev(s, s->mask_nw, g, o, nullptr, fast_fp32());
       if (var && o->conserve_imbibe) {
         DCHECK(launch_trap_velocity(s->bx, s->mask_nw, s->vn, s->b, s->vx, s->vy, s->vz,
                                     g->dx[0], s->st));
@@ -525,7 +531,7 @@ static real ccm_dev(Session* s, Options* o, const Grid* g, FILE* fp, bool var, r
     }
     if (fp) fprintf(fp, "Reinitializing....");
     const ReduceReq rq{s->phi_prev, o->err_check_zone, eps, dV};
-    reinit_dev(s, s->phi, g, o, &rq);
+    reinit_dev(s, s->phi, g, o, &rq, fast_fp32());
     if (fp) fprintf(fp, "Reinitialized\n");
     read_reduce(s);
     vol_phi = s->h_red[0];
@@ -624,7 +630,7 @@ static void model_setup(Session* s, Options* o, QSS_DataArrays* p, Grid* g, bool
   disconnected_init(s, s->mask_w, g);
   disconnected_init(s, s->mask_nw, g);
   reservoir_box(s, g);
-  if (drain && o->mask_reinit) reinit_dev(s, s->mask, g, o);
+  if (drain && o->mask_reinit) reinit_dev(s, s->mask, g, o, nullptr, fast_fp32());
   DCHECK(launch_central_grad(s->bx, s->gx, s->gy, s->gz, s->mask, g->dx[0], g->dx[1], g->dx[2],
                              s->st));
   set_theta_overlap(o, g);
@@ -664,10 +670,13 @@ static void model_coeffs(Session* s, Options* o, const Grid* g, bool var, bool d
 }
 
 void model_run(Options* o, QSS_DataArrays* p, Grid* g, FILE* fp, bool drain) {
+  BatchHook* hook = batch_hook();
   Session* s = session_for(p, g);
+  if (hook) hook->mark(0);
   s->tm = Timers();
   bool var = (o->theta > 0) || o->use_var_theta;
   model_setup(s, o, p, g, drain, var);
+  if (hook) hook->mark(1);
   real a0 = o->a, da = o->b * o->dc;
   int step = o->init_step + 1;
   std::unique_ptr<StepWriter> writer;
@@ -675,7 +684,9 @@ void model_run(Options* o, QSS_DataArrays* p, Grid* g, FILE* fp, bool drain) {
     printf("\nStep %d: a = %f\n", step, (double)a0);
     if (fp) fprintf(fp, "\nStep %d: a = %f\n", step, (double)a0);
     model_coeffs(s, o, g, var, drain, a0);
+    if (hook) hook->mark(2);
     if (o->check_connectivity) trap_dev(s, g, o);
+    if (hook) hook->mark(3);
     real t_step = ccm_dev(s, o, g, fp, var, a0);
     if (t_step == -1) break;
     if (drain) a0 += da; else a0 -= da;

@@ -165,24 +165,26 @@ struct ReinitArgs {
 hipError_t launch_reinit_smooth(const ReinitArgs& a, int stage, hipStream_t st);
 
 // Subcell-fix reinitialization pieces.
+// fast: single-precision fast arithmetic (hardware reciprocal and square
+// root), which the model drivers use in the FP32 library
 hipError_t launch_reinit_distance(const Box& bx, real* dist0, const real* phi0, real dx, real dy,
-                                  real dz, hipStream_t st);
+                                  real dz, hipStream_t st, bool fast = false);
 hipError_t launch_reinit_step(const Box& bx, real* next, const real* data, const real* phi0,
                               const real* dist0, real dt_r, real dx, real dy, real dz,
-                              hipStream_t st);
+                              hipStream_t st, bool fast = false);
 // the subcell-fix right-hand side alone, fill box
 hipError_t launch_reinit_rhs(const Box& bx, real* rhs, const real* data, const real* phi0,
                              const real* dist0, real dx, real dy, real dz, hipStream_t st);
 // the first step with data = phi0, D0 formed on the fly (no phi0 copy, no D0 field)
 hipError_t launch_reinit_first_step(const Box& bx, real* next, const real* phi0, real dt_r,
-                                    real dx, real dy, real dz, hipStream_t st);
+                                    real dx, real dy, real dz, hipStream_t st, bool fast = false);
 
 // launch_reinit_first_step plus the outer reductions of its result (as
 // launch_outer_reduce(bx, next, prev, zone, eps, dV, ...), same sums)
 hipError_t launch_reinit_first_step_reduce(const Box& bx, real* next, const real* phi0, real dt_r,
                                            real dx, real dy, real dz, const real* prev, real zone,
                                            real eps, real dV, double* partials, double* out,
-                                           hipStream_t st);
+                                           hipStream_t st, bool fast = false);
 // phi = -mask_nw where mask_nw > 0, and the imposition floor with the
 // trapped-phase masks (launch_merge then launch_floor with conn = 1), one pass
 hipError_t launch_merge_floor(real* phi, real* floor, const real* m, const real* mw,

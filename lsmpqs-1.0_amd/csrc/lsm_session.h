@@ -65,6 +65,9 @@ struct Session {
   bool redirect = false;
   GraphCache* graphs = nullptr;
   Timers tm;
+  // device memory: one allocation per group of fields (the fields above
+  // point into these), freed with the session
+  std::vector<void*> slabs;
 };
 
 // error state of the calling thread

@@ -4,6 +4,7 @@
 #include <string.h>
 #include <unistd.h>
 #include <map>
+#include <initializer_list>
 #include <mutex>
 #include <stdexcept>
 #include "lsm_session.h"
@@ -61,9 +62,29 @@ Box box_of(const Grid* g) {
       throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// One device allocation for a group of fields, each 256-byte aligned: a
+// session makes four such allocations (core, variable coefficients, mask
+// gradient, labelling) instead of one per field, so that many boxes set up
+// at once (batched runs) do not queue on the allocator.
+struct SlabPart {
+  void** p;
+  size_t bytes;
+};
+static void slab_alloc(Session* s, std::initializer_list<SlabPart> parts) {
+  size_t tot = 0;
+  for (const SlabPart& q : parts) tot += (q.bytes + 255) & ~(size_t)255;
+  char* base = nullptr;
+  SCHECK(hipMalloc((void**)&base, tot));
+  s->slabs.push_back(base);
+  size_t off = 0;
+  for (const SlabPart& q : parts) {
+    *q.p = base + off;
+    off += (q.bytes + 255) & ~(size_t)255;
+  }
+}
 template <typename P>
-static void dalloc(P*& p, int64_t count) {
-  if (!p) SCHECK(hipMalloc((void**)&p, (size_t)count * sizeof(P)));
+static SlabPart part(P*& p, int64_t count) {
+  return SlabPart{(void**)&p, (size_t)count * sizeof(P)};
 }
 
 static void free_all(Session* s) {
@@ -71,15 +92,12 @@ static void free_all(Session* s) {
     graph_cache_free(s->graphs);
     s->graphs = nullptr;
   }
-  void* bufs[] = {s->phi, s->A, s->B, s->S, s->D, s->phi_prev, s->mask, s->mask_w, s->mask_nw,
-                  s->floor, s->phi_extra, s->vn, s->b, s->vx, s->vy, s->vz, s->gx, s->gy, s->gz,
-                  s->labels, s->troot, s->roots, s->bin, s->part, s->red, s->dt, s->count,
-                  s->stale_idx, s->stale_src, s->stale_list};
+  void* bufs[] = {s->stale_idx, s->stale_src, s->stale_list};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
+  for (void* p : s->slabs) (void)hipFree(p);
+  s->slabs.clear();
   if (s->h_dt) (void)hipHostFree(s->h_dt);
-  if (s->h_red) (void)hipHostFree(s->h_red);
-  if (s->h_count) (void)hipHostFree(s->h_count);
   if (s->ev_dt) (void)hipEventDestroy(s->ev_dt);
   if (s->ev_a) (void)hipEventDestroy(s->ev_a);
   if (s->ev_b) (void)hipEventDestroy(s->ev_b);
@@ -87,20 +105,9 @@ static void free_all(Session* s) {
   if (s->st) (void)hipStreamDestroy(s->st);
 }
 
-Session* session_for(const QSS_DataArrays* p, const Grid* g) {
-  std::lock_guard<std::mutex> lk(g_mu);
-  Box bx = box_of(g);
-  auto it = g_sessions.find(p);
-  if (it != g_sessions.end()) {
-    Session* s = it->second;
-    if (s->bx.nx == bx.nx && s->bx.ny == bx.ny && s->bx.nz == bx.nz) {
-      s->bx = bx;
-      return s;
-    }
-    free_all(s);
-    delete s;
-    g_sessions.erase(it);
-  }
+// A new session's buffers, allocated and zeroed outside the session-map
+// lock (other threads keep creating and finding their own sessions).
+static Session* session_new(const QSS_DataArrays* p, const Box& bx) {
   if (bx.ilo < 3 || bx.jlo < 3 || bx.klo < 3 || bx.ihi > bx.nx - 4 || bx.jhi > bx.ny - 4 ||
       bx.khi > bx.nz - 4 || bx.ihi - bx.ilo < 3 || bx.jhi - bx.jlo < 3 || bx.khi - bx.klo < 3)
     throw std::runtime_error("grid needs at least 3 ghost layers and 4 fill voxels per axis");
@@ -113,13 +120,16 @@ Session* session_for(const QSS_DataArrays* p, const Grid* g) {
     s->n = bx.size();
     SCHECK(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
     int64_t n = s->n;
-    dalloc(s->phi, n); dalloc(s->A, n); dalloc(s->B, n); dalloc(s->S, n); dalloc(s->D, n);
-    dalloc(s->phi_prev, n); dalloc(s->mask, n); dalloc(s->mask_w, n); dalloc(s->mask_nw, n);
-    dalloc(s->floor, n);
-    dalloc(s->part, reduce_partials()); dalloc(s->red, 4); dalloc(s->dt, 4); dalloc(s->count, 4);
-    SCHECK(hipHostMalloc((void**)&s->h_dt, 4 * sizeof(real), hipHostMallocDefault));
-    SCHECK(hipHostMalloc((void**)&s->h_red, 4 * sizeof(double), hipHostMallocDefault));
-    SCHECK(hipHostMalloc((void**)&s->h_count, 4 * sizeof(int), hipHostMallocDefault));
+    slab_alloc(s, {part(s->phi, n), part(s->A, n), part(s->B, n), part(s->S, n), part(s->D, n),
+                   part(s->phi_prev, n), part(s->mask, n), part(s->mask_w, n),
+                   part(s->mask_nw, n), part(s->floor, n), part(s->part, reduce_partials()),
+                   part(s->red, 4), part(s->dt, 4), part(s->count, 4)});
+    // pinned read-back slots: dt, the two reductions, a counter
+    char* h = nullptr;
+    SCHECK(hipHostMalloc((void**)&h, 256, hipHostMallocDefault));
+    s->h_dt = (real*)h;
+    s->h_red = (double*)(h + 64);
+    s->h_count = (int*)(h + 128);
     SCHECK(hipEventCreate(&s->ev_dt));
     SCHECK(hipEventCreate(&s->ev_a));
     SCHECK(hipEventCreate(&s->ev_b));
@@ -133,37 +143,64 @@ Session* session_for(const QSS_DataArrays* p, const Grid* g) {
     delete s;
     throw;
   }
+  return s;
+}
+
+Session* session_for(const QSS_DataArrays* p, const Grid* g) {
+  Box bx = box_of(g);
+  Session* stale = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_sessions.find(p);
+    if (it != g_sessions.end()) {
+      Session* s = it->second;
+      if (s->bx.nx == bx.nx && s->bx.ny == bx.ny && s->bx.nz == bx.nz) {
+        s->bx = bx;
+        return s;
+      }
+      stale = s;
+      g_sessions.erase(it);
+    }
+  }
+  if (stale) {
+    (void)hipStreamSynchronize(stale->st);
+    free_all(stale);
+    delete stale;
+  }
+  Session* s = session_new(p, bx);
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_sessions.find(p);
+  if (it != g_sessions.end()) {  // another thread made one for the same record meanwhile
+    free_all(s);
+    delete s;
+    return it->second;
+  }
   g_sessions[p] = s;
   return s;
 }
 
 // lazily allocated fields of the variable-coefficient path and labelling
 void ensure_var(Session* s) {
+  if (s->vn) return;
   int64_t n = s->n;
-  bool fresh = !s->vn;
-  dalloc(s->vn, n); dalloc(s->b, n); dalloc(s->vx, n); dalloc(s->vy, n); dalloc(s->vz, n);
-  if (fresh) {
-    real* f[] = {s->vn, s->b, s->vx, s->vy, s->vz};
-    for (real* q : f) SCHECK(launch_fill(q, real(0), n, s->st));
-  }
+  slab_alloc(s, {part(s->vn, n), part(s->b, n), part(s->vx, n), part(s->vy, n), part(s->vz, n)});
+  real* f[] = {s->vn, s->b, s->vx, s->vy, s->vz};
+  for (real* q : f) SCHECK(launch_fill(q, real(0), n, s->st));
 }
 void ensure_grad(Session* s) {
+  if (s->gx) return;
   int64_t n = s->n;
-  bool fresh = !s->gx;
-  dalloc(s->gx, n); dalloc(s->gy, n); dalloc(s->gz, n); dalloc(s->phi_extra, n);
-  if (fresh) {
-    real* f[] = {s->gx, s->gy, s->gz, s->phi_extra};
-    for (real* q : f) SCHECK(launch_fill(q, real(0), n, s->st));
-  }
+  slab_alloc(s, {part(s->gx, n), part(s->gy, n), part(s->gz, n), part(s->phi_extra, n)});
+  real* f[] = {s->gx, s->gy, s->gz, s->phi_extra};
+  for (real* q : f) SCHECK(launch_fill(q, real(0), n, s->st));
 }
 void ensure_labels(Session* s) {
-  dalloc(s->labels, s->n);
-  dalloc(s->troot, s->n);
-  dalloc(s->bin, s->n);
+  if (s->labels) return;
   // room for n/16 tile roots (a 64x16x16 tile holds 16384 voxels); a labelling
   // with more falls back to scanning troot
   s->roots_cap = s->n / 16 + 1;
-  dalloc(s->roots, s->roots_cap + 1);
+  slab_alloc(s, {part(s->labels, s->n), part(s->troot, s->n), part(s->bin, s->n),
+                 part(s->roots, s->roots_cap + 1)});
 }
 
 void session_release(const QSS_DataArrays* p) {

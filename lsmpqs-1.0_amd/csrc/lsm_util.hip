@@ -209,30 +209,49 @@ hipError_t launch_coeff_reduce(const Box& bx, const real* b, const real* vx, con
 // Subcell-fix reinitialization (reference:
 // src/reinitialization/qss_reinitialization3d.f:476-746)
 
+// Bitwise, not short-circuit, operators: the test of all six neighbours is
+// straight-line code (with && / || each neighbour became a branch that
+// waited for that neighbour's load before the next one was examined).
 __device__ __forceinline__ bool sign_change(real a, real c) {
-  return (a > real(0) && c <= real(0)) || (a <= real(0) && c > real(0));
+  return ((a > real(0)) & (c <= real(0))) | ((a <= real(0)) & (c > real(0)));
+}
+
+// Quotient a / b: IEEE division, or in the fast single-precision mode a
+// times the hardware reciprocal (1 ulp; b is never denormal here: delta >= 1e-5
+// and max dx are the divisors)
+template <bool FAST>
+__device__ __forceinline__ real rdiv(real a, real b) {
+  if constexpr (FAST && sizeof(real) == 4) return a * fast_rcp(b);
+  else return a / b;
+}
+template <bool FAST>
+__device__ __forceinline__ real rsqrt_(real x) {
+  if constexpr (FAST && sizeof(real) == 4) return __builtin_amdgcn_sqrtf(x);
+  else return sqrt(x);
 }
 
 // D0 of one voxel from phi0 and its six neighbours (:518-570): phi0 / delta
 // next to the interface (a sign change to a face neighbour), else the flag
+template <bool FAST>
 __device__ __forceinline__ real reinit_d0(real c, real xp, real xm, real yp, real ym, real zp,
                                           real zm, real flag) {
-  bool near = sign_change(xp, c) || sign_change(xm, c) || sign_change(yp, c) ||
-              sign_change(ym, c) || sign_change(zp, c) || sign_change(zm, c);
+  const bool near = sign_change(xp, c) | sign_change(xm, c) | sign_change(yp, c) |
+                    sign_change(ym, c) | sign_change(zp, c) | sign_change(zm, c);
   if (!near) return flag;
   real delta = tmax(tmax(tmax(fabs(xp - xm) / real(2), fabs(xp - c)), fabs(c - xm)), real(1.e-5));
   delta = tmax(tmax(tmax(fabs(yp - ym) / real(2), fabs(yp - c)), fabs(c - ym)), delta);
   delta = tmax(tmax(tmax(fabs(zp - zm) / real(2), fabs(zp - c)), fabs(c - zm)), delta);
-  return c / delta;
+  return rdiv<FAST>(c, delta);
 }
 
 // First-order subcell-fix right-hand side (:665-737): sgn(phi0)(1 - |grad phi|_G)
 // with one-sided differences gp (forward), gm (backward), or, next to the
 // interface (d0 != flag), d0 - sgn(phi0) |phi| / max dx
+template <bool FAST>
 __device__ __forceinline__ real reinit_rhs(real c, const real* gp, const real* gm, real p0,
                                            real d0, real flag, real max_dx) {
   real sgn0 = fabs(p0) > real(1.e-5) ? copysign(real(1), p0) : real(0);
-  if (d0 != flag) return d0 - sgn0 * fabs(c) / max_dx;
+  if (d0 != flag) return d0 - rdiv<FAST>(sgn0 * fabs(c), max_dx);
   real st[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -243,28 +262,34 @@ __device__ __forceinline__ real reinit_rhs(real c, const real* gp, const real* g
   }
   if (!(fabs(c) > real(1.e-5))) return real(0);
   real n2 = st[0] * st[0] + st[1] * st[1] + st[2] * st[2];
-  return sgn0 - sgn0 * sqrt(n2);
+  return sgn0 - sgn0 * rsqrt_<FAST>(n2);
 }
 
+// The kernels below take FAST (single precision with fast_rcp and the
+// hardware square root, the model drivers' FP32 arithmetic) as a template
+// flag; the exported kernel-level routines run them with FAST = false.
+template <bool FAST>
 __global__ void reinit_distance_kernel(Box bx, real* dist0, const real* phi0, real dx, real dy,
                                        real dz) {
   real flag = real(-1000.0) * tmax(tmax(dx, dy), dz);
   for_rows<true>(bx, [&](int64_t idx, int i, int j, int k) {
     (void)i; (void)j; (void)k;
     int64_t sy = bx.nx, sz = (int64_t)bx.nx * bx.ny;
-    dist0[idx] = reinit_d0(phi0[idx], phi0[idx + 1], phi0[idx - 1], phi0[idx + sy],
-                           phi0[idx - sy], phi0[idx + sz], phi0[idx - sz], flag);
+    dist0[idx] = reinit_d0<FAST>(phi0[idx], phi0[idx + 1], phi0[idx - 1], phi0[idx + sy],
+                                 phi0[idx - sy], phi0[idx + sz], phi0[idx - sz], flag);
   });
 }
 hipError_t launch_reinit_distance(const Box& bx, real* dist0, const real* phi0, real dx, real dy,
-                                  real dz, hipStream_t st) {
-  hipLaunchKernelGGL(reinit_distance_kernel, dim3(row_blocks<true>(bx, 16384)), dim3(ROW_TX, ROW_TY), 0, st, bx,
-                     dist0, phi0, dx, dy, dz);
+                                  real dz, hipStream_t st, bool fast) {
+  const dim3 g(row_blocks<true>(bx, 16384)), b(ROW_TX, ROW_TY);
+  if (fast) hipLaunchKernelGGL(reinit_distance_kernel<true>, g, b, 0, st, bx, dist0, phi0, dx, dy, dz);
+  else hipLaunchKernelGGL(reinit_distance_kernel<false>, g, b, 0, st, bx, dist0, phi0, dx, dy, dz);
   return hipGetLastError();
 }
 
 // one forward-Euler step of the reinitialization equation (RK1 fused with
 // the first-order subcell-fix right-hand side)
+template <bool FAST>
 __global__ void reinit_step_kernel(Box bx, real* next, const real* data, const real* phi0,
                                    const real* dist0, real dt_r, real dx, real dy, real dz) {
   real max_dx = tmax(tmax(dx, dy), dz);
@@ -281,14 +306,19 @@ __global__ void reinit_step_kernel(Box bx, real* next, const real* data, const r
     gm[1] = (c - data[idx - sy]) * inv_dy;
     gp[2] = (data[idx + sz] - c) * inv_dz;
     gm[2] = (c - data[idx - sz]) * inv_dz;
-    next[idx] = c + dt_r * reinit_rhs(c, gp, gm, phi0[idx], dist0[idx], flag, max_dx);
+    next[idx] = c + dt_r * reinit_rhs<FAST>(c, gp, gm, phi0[idx], dist0[idx], flag, max_dx);
   });
 }
 hipError_t launch_reinit_step(const Box& bx, real* next, const real* data, const real* phi0,
                               const real* dist0, real dt_r, real dx, real dy, real dz,
-                              hipStream_t st) {
-  hipLaunchKernelGGL(reinit_step_kernel, dim3(row_blocks<true>(bx, 16384)), dim3(ROW_TX, ROW_TY), 0, st, bx, next,
-                     data, phi0, dist0, dt_r, dx, dy, dz);
+                              hipStream_t st, bool fast) {
+  const dim3 g(row_blocks<true>(bx, 16384)), b(ROW_TX, ROW_TY);
+  if (fast)
+    hipLaunchKernelGGL(reinit_step_kernel<true>, g, b, 0, st, bx, next, data, phi0, dist0, dt_r,
+                       dx, dy, dz);
+  else
+    hipLaunchKernelGGL(reinit_step_kernel<false>, g, b, 0, st, bx, next, data, phi0, dist0, dt_r,
+                       dx, dy, dz);
   return hipGetLastError();
 }
 
@@ -309,7 +339,7 @@ __global__ void reinit_rhs_kernel(Box bx, real* rhs, const real* data, const rea
     gm[1] = (c - data[idx - sy]) * inv_dy;
     gp[2] = (data[idx + sz] - c) * inv_dz;
     gm[2] = (c - data[idx - sz]) * inv_dz;
-    rhs[idx] = reinit_rhs(c, gp, gm, phi0[idx], dist0[idx], flag, max_dx);
+    rhs[idx] = reinit_rhs<false>(c, gp, gm, phi0[idx], dist0[idx], flag, max_dx);
   });
 }
 hipError_t launch_reinit_rhs(const Box& bx, real* rhs, const real* data, const real* phi0,
@@ -322,31 +352,51 @@ hipError_t launch_reinit_rhs(const Box& bx, real* rhs, const real* data, const r
 // The first step from phi0 itself (data = phi0): D0 and the step from the
 // same seven values, with no copy of phi0 and no D0 field.  Bit for bit the
 // distance kernel followed by the step kernel on that first iteration.
-__global__ void reinit_first_step_kernel(Box bx, real* next, const real* phi0, real dt_r, real dx,
+// the seven values of a voxel's stencil
+struct Stencil7 {
+  real c, xp, xm, yp, ym, zp, zm;
+};
+__device__ __forceinline__ Stencil7 stencil7(const real* __restrict__ f, int64_t idx, int64_t sy,
+                                             int64_t sz) {
+  return Stencil7{f[idx], f[idx + 1], f[idx - 1], f[idx + sy], f[idx - sy], f[idx + sz],
+                  f[idx - sz]};
+}
+// one subcell-fix step of phi0 itself at one voxel (the first iteration)
+template <bool FAST>
+__device__ __forceinline__ real reinit_first_value(const Stencil7& v, real dt_r, real inv_dx,
+                                                   real inv_dy, real inv_dz, real flag,
+                                                   real max_dx) {
+  const real c = v.c;
+  real d0 = reinit_d0<FAST>(c, v.xp, v.xm, v.yp, v.ym, v.zp, v.zm, flag);
+  real gp[3] = {(v.xp - c) * inv_dx, (v.yp - c) * inv_dy, (v.zp - c) * inv_dz};
+  real gm[3] = {(c - v.xm) * inv_dx, (c - v.ym) * inv_dy, (c - v.zm) * inv_dz};
+  return c + dt_r * reinit_rhs<FAST>(c, gp, gm, c, d0, flag, max_dx);
+}
+constexpr int REINIT_U = 4;  // voxels per lane and pass of the first-step kernels
+
+template <bool FAST>
+__global__ void reinit_first_step_kernel(Box bx, real* __restrict__ next,
+                                         const real* __restrict__ phi0, real dt_r, real dx,
                                          real dy, real dz) {
   real max_dx = tmax(tmax(dx, dy), dz);
   real flag = real(-1000.0) * max_dx;
   real inv_dx = real(1.0) / dx, inv_dy = real(1.0) / dy, inv_dz = real(1.0) / dz;
   int64_t sy = bx.nx, sz = (int64_t)bx.nx * bx.ny;
-  for_rows<true>(bx, [&](int64_t idx, int i, int j, int k) {
-    (void)i; (void)j; (void)k;
-    real c = phi0[idx];
-    real xp = phi0[idx + 1], xm = phi0[idx - 1];
-    real yp = phi0[idx + sy], ym = phi0[idx - sy];
-    real zp = phi0[idx + sz], zm = phi0[idx - sz];
-    real d0 = reinit_d0(c, xp, xm, yp, ym, zp, zm, flag);
-    real gp[3] = {(xp - c) * inv_dx, (yp - c) * inv_dy, (zp - c) * inv_dz};
-    real gm[3] = {(c - xm) * inv_dx, (c - ym) * inv_dy, (c - zm) * inv_dz};
-    next[idx] = c + dt_r * reinit_rhs(c, gp, gm, c, d0, flag, max_dx);
-  });
+  Stencil7 v[REINIT_U];
+  for_rows_u<true, REINIT_U>(
+      bx, [&](int u, int64_t idx) { v[u] = stencil7(phi0, idx, sy, sz); },
+      [&](int u, int64_t idx, int, int, int) {
+        next[idx] = reinit_first_value<FAST>(v[u], dt_r, inv_dx, inv_dy, inv_dz, flag, max_dx);
+      });
 }
 // reinit_first_step_kernel fused with the outer reductions of its result:
 // per-block FP64 partials of the smoothed volume of {next < 0} and of the
 // local max-norm |next - prev| over |next| < zone, in the same order as
 // outer_reduce_kernel (same launch shape), so the sums are the same bits.
+template <bool FAST>
 __global__ __launch_bounds__(256) void reinit_first_step_reduce_kernel(
-    Box bx, real* next, const real* phi0, real dt_r, real dx, real dy, real dz, const real* prev,
-    real zone, real eps, real dV, double* part) {
+    Box bx, real* __restrict__ next, const real* __restrict__ phi0, real dt_r, real dx, real dy,
+    real dz, const real* __restrict__ prev, real zone, real eps, real dV, double* part) {
   const real pi = real(3.14159265358979323846), one_over_pi = real(0.31830988618379);
   real max_dx = tmax(tmax(dx, dy), dz);
   real flag = real(-1000.0) * max_dx;
@@ -354,25 +404,28 @@ __global__ __launch_bounds__(256) void reinit_first_step_reduce_kernel(
   int64_t sy = bx.nx, sz = (int64_t)bx.nx * bx.ny;
   double vol = 0.0;
   real mx = real(-1);
-  for_rows<true>(bx, [&](int64_t idx, int i, int j, int k) {
-    (void)i; (void)j; (void)k;
-    real c = phi0[idx];
-    real xp = phi0[idx + 1], xm = phi0[idx - 1];
-    real yp = phi0[idx + sy], ym = phi0[idx - sy];
-    real zp = phi0[idx + sz], zm = phi0[idx - sz];
-    real d0 = reinit_d0(c, xp, xm, yp, ym, zp, zm, flag);
-    real gp[3] = {(xp - c) * inv_dx, (yp - c) * inv_dy, (zp - c) * inv_dz};
-    real gm[3] = {(c - xm) * inv_dx, (c - ym) * inv_dy, (c - zm) * inv_dz};
-    const real p = c + dt_r * reinit_rhs(c, gp, gm, c, d0, flag, max_dx);
-    next[idx] = p;
-    real q = p / eps;
-    if (p < -eps) vol += (double)dV;
-    else if (p < eps) vol += (double)(real(0.5) * (real(1) - q - one_over_pi * sin(pi * q)) * dV);
-    if (fabs(p) < zone) {
-      real d = fabs(p - prev[idx]);
-      if (d > mx) mx = d;
-    }
-  });
+  Stencil7 v[REINIT_U];
+  real pv[REINIT_U];
+  // the voxels of a lane are visited in the order of for_rows (u runs along
+  // the row), so the per-lane sums are added in the same order
+  for_rows_u<true, REINIT_U>(
+      bx,
+      [&](int u, int64_t idx) {
+        v[u] = stencil7(phi0, idx, sy, sz);
+        pv[u] = prev[idx];
+      },
+      [&](int u, int64_t idx, int, int, int) {
+        const real p = reinit_first_value<FAST>(v[u], dt_r, inv_dx, inv_dy, inv_dz, flag, max_dx);
+        next[idx] = p;
+        real q = p / eps;
+        if (p < -eps) vol += (double)dV;
+        else if (p < eps)
+          vol += (double)(real(0.5) * (real(1) - q - one_over_pi * sin(pi * q)) * dV);
+        if (fabs(p) < zone) {
+          real d = fabs(p - pv[u]);
+          if (d > mx) mx = d;
+        }
+      });
   __shared__ double s_sum[ROW_TY], s_max[ROW_TY];
   double ws = wave_sum(vol), wm = wave_max((double)mx);
   const int lane = threadIdx.x, wid = threadIdx.y;
@@ -388,10 +441,14 @@ __global__ __launch_bounds__(256) void reinit_first_step_reduce_kernel(
 hipError_t launch_reinit_first_step_reduce(const Box& bx, real* next, const real* phi0, real dt_r,
                                            real dx, real dy, real dz, const real* prev, real zone,
                                            real eps, real dV, double* part, double* out,
-                                           hipStream_t st) {
+                                           hipStream_t st, bool fast) {
   const int nb = row_blocks<true>(bx, RED_BLOCKS);
-  hipLaunchKernelGGL(reinit_first_step_reduce_kernel, dim3(nb), dim3(ROW_TX, ROW_TY), 0, st, bx,
-                     next, phi0, dt_r, dx, dy, dz, prev, zone, eps, dV, part);
+  if (fast)
+    hipLaunchKernelGGL(reinit_first_step_reduce_kernel<true>, dim3(nb), dim3(ROW_TX, ROW_TY), 0,
+                       st, bx, next, phi0, dt_r, dx, dy, dz, prev, zone, eps, dV, part);
+  else
+    hipLaunchKernelGGL(reinit_first_step_reduce_kernel<false>, dim3(nb), dim3(ROW_TX, ROW_TY), 0,
+                       st, bx, next, phi0, dt_r, dx, dy, dz, prev, zone, eps, dV, part);
   hipLaunchKernelGGL(finish_reduce_kernel, dim3(1), dim3(256), 0, st, part, nb, out, 0);
   return hipGetLastError();
 }
@@ -419,9 +476,13 @@ hipError_t launch_merge_floor(real* phi, real* floor, const real* m, const real*
 }
 
 hipError_t launch_reinit_first_step(const Box& bx, real* next, const real* phi0, real dt_r,
-                                    real dx, real dy, real dz, hipStream_t st) {
-  hipLaunchKernelGGL(reinit_first_step_kernel, dim3(row_blocks<true>(bx, 16384)), dim3(ROW_TX, ROW_TY), 0, st, bx,
-                     next, phi0, dt_r, dx, dy, dz);
+                                    real dx, real dy, real dz, hipStream_t st, bool fast) {
+  const dim3 g(row_blocks<true>(bx, 16384)), b(ROW_TX, ROW_TY);
+  if (fast)
+    hipLaunchKernelGGL(reinit_first_step_kernel<true>, g, b, 0, st, bx, next, phi0, dt_r, dx, dy, dz);
+  else
+    hipLaunchKernelGGL(reinit_first_step_kernel<false>, g, b, 0, st, bx, next, phi0, dt_r, dx, dy,
+                       dz);
   return hipGetLastError();
 }
 
