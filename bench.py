@@ -435,7 +435,9 @@ def batch_leg(tmax=20.0, seed=1004, sequential=True):
                "batched_steps": int(st.batched_steps), "box_steps": int(st.box_steps),
                "brick_depth": int(st.brick_depth), "workgroups": int(st.workgroups),
                "voxel_updates": vu,
-               "voxel_updates_per_s_batched_gpu": round(vu / (st.batched_ms * 1e-3), 1)}
+               "voxel_updates_per_s_batched_gpu": round(vu / (st.batched_ms * 1e-3), 1),
+               "batched_outer_steps": int(st.batched_outer), "box_outer_steps": int(st.box_outer),
+               "batched_outer_gpu_s": round(st.batched_outer_ms * 1e-3, 3)}
         if sequential:
             items2 = records()
             w0 = time.perf_counter()

@@ -271,8 +271,11 @@ unsigned lsmpqs_set_quirks(unsigned mask);
    persistent stage kernel fed by a work queue of 64 x 8 x zc voxel bricks of
    all boxes, and a batched ghost extrapolation, captured once as a HIP graph
    and replayed.  zc and the number of persistent workgroups are picked on
-   the first step by timing.  Results equal the boxes' individual runs bit
-   for bit.  All boxes must share the spatial scheme, the time order and the
+   the first step by timing.  The device work of the outer steps (trapping,
+   reinitialization, merge and floor, reductions) of the boxes waiting for
+   one likewise runs as one launch per pass over all of them, when trapping
+   is on and the reinitialization is a single iteration.  Results equal the
+   boxes' individual runs bit for bit.  All boxes must share the spatial scheme, the time order and the
    contact-angle path (theta > 0 or not).  fp_out: one log per box or NULL;
    no data_step_N files are written.  Returns 0, or -2 with the message in
    lsmpqs_hip_last_error(). */
@@ -284,6 +287,9 @@ typedef struct {
   int workgroups;             /* persistent workgroups chosen by the tuner */
   double batched_ms;          /* GPU time of the batched steps */
   double autotune_best_ms;    /* fastest first-stage launch seen while tuning */
+  long long batched_outer;    /* batched outer steps (launch sequences) */
+  long long box_outer;        /* outer steps they served, summed over the boxes */
+  double batched_outer_ms;    /* GPU time of the batched outer steps */
 } lsmpqs_batch_stats;
 int lsmpqs_batch_model3d(int n, Options **options, QSS_DataArrays **p, Grid **g, FILE **fp_out,
                          int drain, lsmpqs_batch_stats *stats);

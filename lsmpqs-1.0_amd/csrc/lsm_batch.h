@@ -42,6 +42,13 @@ struct BatchHook {
   // reinitialization, the next curvature step); batched steps of the other
   // boxes need not wait for it until it asks for its next step
   virtual void away() {}
+  // runs the device work of one outer step of box s (after the box's own
+  // stream is idle) with the other boxes' and leaves its reductions in
+  // s->h_red; false: not taken, the caller runs it itself
+  virtual bool outer(Session* s, const OuterJob& j, bool fast, int conn) {
+    (void)s; (void)j; (void)fast; (void)conn;
+    return false;
+  }
   // progress marks of the box's set-up (host-time tracing only)
   virtual void mark(int phase) { (void)phase; }
 };

@@ -46,7 +46,12 @@ struct Request {
   StepCfg c;
   real dt = 0;
   bool done = false;
+  const OuterJob* job = nullptr;  // an outer step instead of an inner step
+  bool fast = false;
+  int conn = 26;
 };
+
+constexpr int kTables = 9;  // block-prefix tables of a batched outer step
 
 class Batch final : public BatchHook {
  public:
@@ -98,8 +103,23 @@ class Batch final : public BatchHook {
 
   // box thread: one inner step of box s
   real step(Session* s, const StepCfg& c) override {
-    BCHECK(hipStreamSynchronize(s->st));
     Request r{s, c};
+    park(r);
+    return r.dt;
+  }
+  // box thread: the device work of one outer step of box s
+  bool outer(Session* s, const OuterJob& j, bool fast, int conn) override {
+    Request r{s, StepCfg{}};
+    r.job = &j;
+    r.fast = fast;
+    r.conn = conn;
+    park(r);
+    return true;
+  }
+
+  // box thread: wait until a round has served r
+  void park(Request& r) {
+    BCHECK(hipStreamSynchronize(r.s->st));
     release_slot();
     {
       std::unique_lock<std::mutex> lk(mu_);
@@ -127,7 +147,6 @@ class Batch final : public BatchHook {
     acquire_slot();
     std::lock_guard<std::mutex> lk(mu_);
     if (!failure_.empty()) throw std::runtime_error(failure_);
-    return r.dt;
   }
 
   void mark(int phase) override {
@@ -195,13 +214,20 @@ class Batch final : public BatchHook {
   lsmpqs_batch_stats stats{};
 
  private:
+  void ensure_stream() {
+    if (st_) return;
+    BCHECK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+    BCHECK(hipEventCreate(&ev0_));
+    BCHECK(hipEventCreate(&ev1_));
+  }
+
   void configure(const Request& first) {
     scheme_ = first.c.scheme;
     var_ = first.c.var;
     order_ = first.c.order;
     fast_ = sizeof(real) == 4 && !(quirks() & EXACT_FP32);
     nstage_ = order_ <= 1 ? 1 : (order_ == 2 ? 2 : 3);
-    BCHECK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+    ensure_stream();
     BCHECK(hipMalloc((void**)&d_recs_, sizeof(BatchRec) * nmax_));
     BCHECK(hipMalloc((void**)&d_count_, sizeof(int) * 4));
     BCHECK(hipMalloc((void**)&d_brick_, sizeof(int) * (nmax_ + 1)));
@@ -213,8 +239,6 @@ class Batch final : public BatchHook {
     BCHECK(hipHostMalloc((void**)&h_brick_, sizeof(int) * (nmax_ + 1), hipHostMallocDefault));
     BCHECK(hipHostMalloc((void**)&h_ghost_, sizeof(int64_t) * (nmax_ + 1), hipHostMallocDefault));
     BCHECK(hipHostMalloc((void**)&h_dts_, sizeof(real) * nmax_, hipHostMallocDefault));
-    BCHECK(hipEventCreate(&ev0_));
-    BCHECK(hipEventCreate(&ev1_));
     view_.recs = d_recs_;
     view_.count = d_count_;
     view_.brick_prefix = d_brick_;
@@ -232,12 +256,15 @@ class Batch final : public BatchHook {
   void release() {
     if (exec_) (void)hipGraphExecDestroy(exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
-    void* d[] = {d_recs_, d_count_, d_brick_, d_ghost_, d_counters_, d_dts_};
+    void* d[] = {d_recs_, d_count_, d_brick_, d_ghost_, d_counters_, d_dts_, d_jobs_, d_tab_,
+                 d_redout_};
     for (void* p : d)
       if (p) (void)hipFree(p);
-    void* h[] = {h_recs_, h_count_, h_brick_, h_ghost_, h_dts_};
+    void* h[] = {h_recs_, h_count_, h_brick_, h_ghost_, h_dts_, h_jobs_, h_tab_, h_redout_};
     for (void* p : h)
       if (p) (void)hipHostFree(p);
+    for (hipEvent_t e : pass_ev_)
+      if (e) (void)hipEventDestroy(e);
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
     if (st_) (void)hipStreamDestroy(st_);
@@ -312,6 +339,28 @@ class Batch final : public BatchHook {
     stats.brick_depth = zc_;
     stats.workgroups = grid_;
     stats.autotune_best_ms = best;
+    ratio_ = (double)bricks_of(round, zc_) / grid_;
+  }
+
+  // bricks of the round's boxes at depth zc (one stage)
+  static int bricks_of(const std::vector<Request*>& round, int zc) {
+    int tot = 0;
+    for (const Request* r : round) {
+      StageArgs a{};
+      a.bx = r->s->bx;
+      tot += stage_bricks(a, zc);
+    }
+    return tot;
+  }
+  // Brick depth of a later round: the deepest whose bricks per persistent
+  // workgroup are at least the tuned round's, so that a round of fewer or
+  // smaller boxes keeps the queue as well balanced over the workgroups (the
+  // graph's workgroup count stays fixed; the depth only changes the brick
+  // table, and no voxel's arithmetic).
+  int round_depth(const std::vector<Request*>& round) const {
+    for (int zc : {64, 32, 16})
+      if (zc <= zc_ && bricks_of(round, zc) >= ratio_ * grid_) return zc;
+    return 8;
   }
 
   void capture() {
@@ -329,13 +378,124 @@ class Batch final : public BatchHook {
   }
 
   void launch(const std::vector<Request*>& round) {
-    if (!st_) configure(*round[0]);
+    std::vector<Request*> inner, outer;
+    for (Request* r : round) (r->job ? outer : inner).push_back(r);
+    if (!outer.empty()) launch_outer(outer);
+    if (!inner.empty()) launch_inner(inner);
+  }
+
+  // One outer step of each box of `reqs`: every pass one launch over all of
+  // them, in the single-box order (lsm_driver.hip: trap_dev, reinit_dev of
+  // mask_w and mask_nw, trapped-phase velocities, merge and floor, reinit_dev
+  // of phi with the outer reductions), then the reductions read back.
+  void launch_outer(const std::vector<Request*>& reqs) {
+    ensure_stream();
+    const int n = (int)reqs.size();
+    const int conn = reqs[0]->conn;
+    const bool fast = reqs[0]->fast;
+    for (Request* r : reqs)
+      if (r->conn != conn || r->fast != fast)
+        throw std::runtime_error("batched outer steps need one connectivity and arithmetic mode");
+    if (!d_jobs_) {
+      BCHECK(hipMalloc((void**)&d_jobs_, sizeof(OuterJob) * nmax_));
+      BCHECK(hipMalloc((void**)&d_tab_, sizeof(int) * kTables * (nmax_ + 1)));
+      BCHECK(hipMalloc((void**)&d_redout_, sizeof(double) * 2 * nmax_));
+      BCHECK(hipHostMalloc((void**)&h_jobs_, sizeof(OuterJob) * nmax_, hipHostMallocDefault));
+      BCHECK(hipHostMalloc((void**)&h_tab_, sizeof(int) * kTables * (nmax_ + 1),
+                           hipHostMallocDefault));
+      BCHECK(hipHostMalloc((void**)&h_redout_, sizeof(double) * 2 * nmax_, hipHostMallocDefault));
+    }
+    if (n > nmax_) throw std::runtime_error("more outer-step requests than boxes");
+    bool velocities = false;
+    for (int i = 0; i < n; ++i) {
+      h_jobs_[i] = *reqs[i]->job;
+      velocities = velocities || h_jobs_[i].vn != nullptr;
+    }
+    // per launch kind: a prefix table of the boxes' single-box grids
+    auto table = [&](int k, auto count) {
+      int* p = h_tab_ + k * (nmax_ + 1);
+      p[0] = 0;
+      for (int i = 0; i < n; ++i) p[i + 1] = p[i] + count(h_jobs_[i]);
+      return Multi{d_tab_ + k * (nmax_ + 1), n, p[n]};
+    };
+    const Multi trap = table(0, [](const OuterJob& j) { return trap_pass_blocks(j.ta.bx); });
+    const Multi force = table(1, [](const OuterJob&) { return force_pass_blocks(); });
+    const Multi one = table(2, [](const OuterJob&) { return 1; });
+    const Multi tiles = table(3, [](const OuterJob& j) { return label_tile_blocks(j.ta.bx); });
+    const Multi jump = table(4, [](const OuterJob& j) { return label_jump_blocks(j.ta.roots_cap); });
+    const Multi rows = table(5, [](const OuterJob& j) { return reinit_pass_blocks(j.ta.bx); });
+    const Multi red = table(6, [](const OuterJob& j) { return j.nred; });
+    const Multi stream = table(7, [](const OuterJob& j) { return stream_pass_blocks(j.ta.bx.size()); });
+    const Multi ghosts = table(8, [](const OuterJob& j) { return bc_pass_blocks(j.ta.bx); });
+    BCHECK(hipMemcpyAsync(d_jobs_, h_jobs_, sizeof(OuterJob) * n, hipMemcpyHostToDevice, st_));
+    BCHECK(hipMemcpyAsync(d_tab_, h_tab_, sizeof(int) * kTables * (nmax_ + 1),
+                          hipMemcpyHostToDevice, st_));
+    BCHECK(hipEventRecord(ev0_, st_));
+    const bool tr = trace_on();
+    if (tr) mark_pass(0);
+    BCHECK(launch_trap_multi(d_jobs_, trap, force, one, tiles, jump, conn, st_));
+    if (tr) mark_pass(1);
+    for (int pass = 0; pass < 2; ++pass) {
+      BCHECK(launch_reinit_multi(d_jobs_, pass, rows, fast, st_));
+      BCHECK(launch_bc_multi(d_jobs_, pass, ghosts, st_));
+    }
+    if (tr) mark_pass(2);
+    if (velocities) BCHECK(launch_trap_velocity_multi(d_jobs_, rows, st_));
+    BCHECK(launch_merge_floor_multi(d_jobs_, stream, st_));
+    if (tr) mark_pass(3);
+    BCHECK(launch_reinit_reduce_multi(d_jobs_, red, one, fast, st_));
+    BCHECK(launch_bc_multi(d_jobs_, 2, ghosts, st_));
+    BCHECK(launch_gather_red(d_jobs_, n, d_redout_, st_));
+    if (tr) mark_pass(4);
+    BCHECK(hipEventRecord(ev1_, st_));
+    BCHECK(hipMemcpyAsync(h_redout_, d_redout_, sizeof(double) * 2 * n, hipMemcpyDeviceToHost,
+                          st_));
+    BCHECK(hipStreamSynchronize(st_));
+    float ms = 0;
+    BCHECK(hipEventElapsedTime(&ms, ev0_, ev1_));
+    stats.batched_outer += 1;
+    stats.box_outer += n;
+    stats.batched_outer_ms += ms;
+    if (tr)
+      for (int q = 0; q < 4; ++q) {
+        float m = 0;
+        BCHECK(hipEventElapsedTime(&m, pass_ev_[q], pass_ev_[q + 1]));
+        pass_ms[q] += m;
+      }
+    for (int i = 0; i < n; ++i) {
+      reqs[i]->s->h_red[0] = h_redout_[2 * i];
+      reqs[i]->s->h_red[1] = h_redout_[2 * i + 1];
+    }
+  }
+
+  // per-pass timing of the batched outer steps (LSMPQS_BATCH_TRACE=1)
+  static bool trace_on() {
+    const char* e = getenv("LSMPQS_BATCH_TRACE");
+    return e && *e && atoi(e) != 0;
+  }
+  void mark_pass(int q) {
+    if (!pass_ev_[q]) BCHECK(hipEventCreate(&pass_ev_[q]));
+    BCHECK(hipEventRecord(pass_ev_[q], st_));
+  }
+
+ public:
+  double pass_ms[4] = {0, 0, 0, 0};  // trapping, mask reinitializations, merge, phi reinitialization
+
+ private:
+  hipEvent_t pass_ev_[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+
+  void launch_inner(const std::vector<Request*>& round) {
+    if (!configured_) {
+      configure(*round[0]);
+      configured_ = true;
+    }
     for (Request* r : round)
       if (r->c.scheme != scheme_ || r->c.var != var_ || r->c.order != order_)
         throw std::runtime_error(
             "all boxes of a batch need the same spatial scheme, time order and contact-angle path");
     if (zc_ == 0) autotune(round);
-    upload(round, zc_);
+    const int zc = round_depth(round);
+    upload(round, zc);
     if (!exec_) capture();
     BCHECK(hipEventRecord(ev0_, st_));
     BCHECK(hipGraphLaunch(exec_, st_));
@@ -365,8 +525,16 @@ class Batch final : public BatchHook {
   int scheme_ = 0, order_ = 0, nstage_ = 0;
   bool var_ = false, fast_ = false;
   int cus_ = 256, zc_ = 0, grid_ = 0, bc_grid_ = 1024;
+  double ratio_ = 0;  // bricks per persistent workgroup of the tuned round
   hipStream_t st_ = nullptr;
+  bool configured_ = false;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+  OuterJob* d_jobs_ = nullptr;
+  OuterJob* h_jobs_ = nullptr;
+  int* d_tab_ = nullptr;
+  int* h_tab_ = nullptr;
+  double* d_redout_ = nullptr;
+  double* h_redout_ = nullptr;
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t exec_ = nullptr;
   BatchView view_{};
@@ -464,6 +632,11 @@ extern "C" int lsmpqs_batch_model3d(int n, Options** options, QSS_DataArrays** p
             "%.3f, first trapping %.3f, to the first step %.3f s; outer-step work %.3f s in %lld "
             "stretches\n",
             ph[0], ph[1], ph[2], ph[3], ph[4], aw, naw);
+    fprintf(stderr,
+            "lsmpqs batch trace: batched outer steps: trapping %.3f, mask reinitializations "
+            "%.3f, merge and floor %.3f, phi reinitialization and reductions %.3f s\n",
+            batch.pass_ms[0] * 1e-3, batch.pass_ms[1] * 1e-3, batch.pass_ms[2] * 1e-3,
+            batch.pass_ms[3] * 1e-3);
   }
   if (stats) {
     *stats = batch.stats;

@@ -111,15 +111,16 @@ __device__ __forceinline__ void unite_planes(int* p, unsigned need, int tx, int 
 // the root-jump passes scan T instead)
 // With all = false only the tile roots' entries of L are written (they are
 // the only entries read before a full labelling's resolve pass).
-__global__ __launch_bounds__(CX * CY) void ccl_local_kernel(Box bx, const uint8_t* bin, int* L,
-                                                          int* T, int conn, int* roots, int cap,
-                                                          bool all) {
+// (cx, cy, cz): the tile's position in the tile grid
+__device__ __forceinline__ void ccl_local_tile(const Box& bx, const uint8_t* bin, int* L, int* T,
+                                               int conn, int* roots, int cap, bool all, int cx,
+                                               int cy, int cz) {
   __shared__ int p[CN];
   // bit z of cm[ty + 1][tx + 1] is voxel (tx, ty, z) of the tile; the frame
   // of one column around the tile reads as background
   __shared__ unsigned short cm[CY + 2][CX + 2];
   const int tx = threadIdx.x, ty = threadIdx.y;
-  const int i = blockIdx.x * CX + tx, j = blockIdx.y * CY + ty, kb = blockIdx.z * CZ;
+  const int i = cx * CX + tx, j = cy * CY + ty, kb = cz * CZ;
   const bool col = i < bx.nx && j < bx.ny;
   const int plane = bx.nx * bx.ny;
   const int g0 = i + bx.nx * j;
@@ -186,7 +187,7 @@ __global__ __launch_bounds__(CX * CY) void ccl_local_kernel(Box bx, const uint8_
     if (fgm >> z & 1u) {
       int r = lds_find(p, l);
       int rx = r % CX, ry = (r / CX) % CY, rz = r / (CX * CY);
-      root = (blockIdx.x * CX + rx) + bx.nx * ((blockIdx.y * CY + ry) + bx.ny * (kb + rz));
+      root = (cx * CX + rx) + bx.nx * ((cy * CY + ry) + bx.ny * (kb + rz));
       if (r == l && roots) {
         const int q = atomicAdd(&roots[0], 1);
         if (q < cap) roots[1 + q] = g;
@@ -195,6 +196,11 @@ __global__ __launch_bounds__(CX * CY) void ccl_local_kernel(Box bx, const uint8_
     if (all || root == g) L[g] = root;
     T[g] = root;
   }
+}
+__global__ __launch_bounds__(CX * CY) void ccl_local_kernel(Box bx, const uint8_t* bin, int* L,
+                                                          int* T, int conn, int* roots, int cap,
+                                                          bool all) {
+  ccl_local_tile(bx, bin, L, T, conn, roots, cap, all, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 // Pass 2: unions across tile boundaries.  A pair of neighbouring foreground
@@ -239,12 +245,13 @@ constexpr int face_count() {
 // face job e of tile blockIdx, offset class AX
 template <int AX>
 __device__ __forceinline__ void border_job(const Box& bx, int* L, const int* T, int conn,
-                                           unsigned long long* set, int plane, int e) {
+                                           unsigned long long* set, int plane, int e, int cx,
+                                           int cy, int cz) {
   int lx, ly, lz;
   if (AX == 0) { lx = CX - 1; ly = e % CY; lz = e / CY; }
   else if (AX == 1) { lx = e % CX; ly = CY - 1; lz = e / CX; }
   else { lx = e % CX; ly = e / CX; lz = CZ - 1; }
-  const int i = blockIdx.x * CX + lx, j = blockIdx.y * CY + ly, k = blockIdx.z * CZ + lz;
+  const int i = cx * CX + lx, j = cy * CY + ly, k = cz * CZ + lz;
   if (i >= bx.nx || j >= bx.ny || k >= bx.nz) return;
   const int l = i + bx.nx * j + plane * k;
   const int a0 = T[l];
@@ -277,13 +284,18 @@ __device__ __forceinline__ void border_job(const Box& bx, int* L, const int* T, 
 }
 
 template <int AX>
-__global__ __launch_bounds__(1024) void ccl_border_kernel(Box bx, int* L, const int* T, int conn) {
+__device__ __forceinline__ void ccl_border_tile(const Box& bx, int* L, const int* T, int conn,
+                                                int cx, int cy, int cz) {
   __shared__ unsigned long long set[PAIR_SET];
   for (int e = threadIdx.x; e < PAIR_SET; e += blockDim.x) set[e] = NO_PAIR;
   __syncthreads();
   const int plane = bx.nx * bx.ny;
   for (int e = threadIdx.x; e < face_count<AX>(); e += blockDim.x)
-    border_job<AX>(bx, L, T, conn, set, plane, e);
+    border_job<AX>(bx, L, T, conn, set, plane, e, cx, cy, cz);
+}
+template <int AX>
+__global__ __launch_bounds__(1024) void ccl_border_kernel(Box bx, int* L, const int* T, int conn) {
+  ccl_border_tile<AX>(bx, L, T, conn, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 // After each border launch: every tile root (T[v] == v) is pointed straight
@@ -296,8 +308,9 @@ __device__ __forceinline__ void jump_root(int* L, int v) {
   for (int q = ld_dev(&L[r]); q != r; q = ld_dev(&L[r])) r = q;
   st_dev(&L[v], r);
 }
-__global__ void ccl_jump_kernel(int* L, const int* T, int n, const int* roots, int cap) {
-  const int stride = gridDim.x * blockDim.x, first = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void ccl_jump_body(int* L, const int* T, int n, const int* roots,
+                                              int cap, int lb, int lg) {
+  const int stride = lg * blockDim.x, first = lb * blockDim.x + threadIdx.x;
   const int count = roots ? roots[0] : 0;
   if (roots && count <= cap) {
     for (int q = first; q < count; q += stride) jump_root(L, roots[1 + q]);
@@ -305,6 +318,9 @@ __global__ void ccl_jump_kernel(int* L, const int* T, int n, const int* roots, i
     for (int v = first; v < n; v += stride)
       if (T[v] == v) jump_root(L, v);
   }
+}
+__global__ void ccl_jump_kernel(int* L, const int* T, int n, const int* roots, int cap) {
+  ccl_jump_body(L, T, n, roots, cap, blockIdx.x, gridDim.x);
 }
 
 // Full labels once every tile root points at its component's root: each
@@ -318,8 +334,15 @@ __global__ void ccl_resolve_kernel(int* L, const int* T, int n) {
 
 __global__ void ccl_reset_kernel(int* roots) { roots[0] = 0; }
 
+constexpr int UNR = 4;  // voxels per thread and iteration of the trapping passes
+
 static int grid_for(int64_t n) {
   int64_t b = (n + 255) / 256;
+  return (int)(b > 16384 ? 16384 : (b < 1 ? 1 : b));
+}
+// the trapping passes take UNR voxels per thread and iteration
+static int trap_grid(int64_t n) {
+  int64_t b = (n + 256 * UNR - 1) / (256 * UNR);
   return (int)(b > 16384 ? 16384 : (b < 1 ? 1 : b));
 }
 
@@ -370,18 +393,17 @@ __device__ __forceinline__ int conn_value(const int* L, const int* T, int64_t id
 // lane has UNR voxels' loads in flight instead of one (the two wetting-phase
 // passes below read a label through the voxel's tile root, two dependent
 // loads).
-constexpr int UNR = 4;
 
 // phi = merge(phi, mask_nw); bin = phi < 0 (force_bin_kernel then sets the
 // inlet voxels)
-__global__ void trap_nw_bin_kernel(TrapArgs a) {
+__device__ __forceinline__ void trap_nw_bin_body(const TrapArgs& a, int lb, int lg) {
   const int64_t n = a.bx.size();
   real* __restrict__ phi = a.phi;
   const real* __restrict__ im = a.impose_mask;
   const real* __restrict__ mnw = a.mask_nw;
   uint8_t* __restrict__ bin = a.bin;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t l0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l0 < n; l0 += UNR * stride) {
+  const int64_t stride = (int64_t)lg * blockDim.x;
+  for (int64_t l0 = (int64_t)lb * blockDim.x + threadIdx.x; l0 < n; l0 += UNR * stride) {
     real p[UNR], m[UNR], mn[UNR];
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
@@ -402,10 +424,11 @@ __global__ void trap_nw_bin_kernel(TrapArgs a) {
     }
   }
 }
+__global__ void trap_nw_bin_kernel(TrapArgs a) { trap_nw_bin_body(a, blockIdx.x, gridDim.x); }
 
 // mask_nw from the non-wetting labels; s = max(-phi, mask, mask_nw);
 // bin = s < 0 (force_bin_kernel then sets the outlet voxels)
-__global__ void trap_w_bin_kernel(TrapArgs a) {
+__device__ __forceinline__ void trap_w_bin_body(const TrapArgs& a, int lb, int lg) {
   const int64_t n = a.bx.size();
   const int* __restrict__ T = a.troot;
   const int* __restrict__ L = a.labels;
@@ -415,8 +438,8 @@ __global__ void trap_w_bin_kernel(TrapArgs a) {
   real* __restrict__ sw = a.s;
   uint8_t* __restrict__ bin = a.bin;
   const int main_val = conn_value(L, T, a.nw_ind);
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t l0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l0 < n; l0 += UNR * stride) {
+  const int64_t stride = (int64_t)lg * blockDim.x;
+  for (int64_t l0 = (int64_t)lb * blockDim.x + threadIdx.x; l0 < n; l0 += UNR * stride) {
     int t[UNR];
     real p[UNR], m[UNR];
 #pragma unroll
@@ -445,27 +468,21 @@ __global__ void trap_w_bin_kernel(TrapArgs a) {
     }
   }
 }
+__global__ void trap_w_bin_kernel(TrapArgs a) { trap_w_bin_body(a, blockIdx.x, gridDim.x); }
 
-// The voxels the reference forces into the binary field before labelling
-// (trapComponents.c:185-194 inlet, :290-299 outlet, :196-211 and :297-337 in
-// 2D): up to four boxes of the ghost box, inclusive bounds.
-struct ForceSet {
-  int n;
-  int lo[4][3], hi[4][3];
-  void add(const Box& bx, int i0, int i1, int j0, int j1, int k0, int k1) {
-    i0 = std::max(i0, 0); j0 = std::max(j0, 0); k0 = std::max(k0, 0);
-    i1 = std::min(i1, bx.nx - 1); j1 = std::min(j1, bx.ny - 1); k1 = std::min(k1, bx.nz - 1);
-    if (i0 > i1 || j0 > j1 || k0 > k1) return;
-    lo[n][0] = i0; lo[n][1] = j0; lo[n][2] = k0;
-    hi[n][0] = i1; hi[n][1] = j1; hi[n][2] = k1;
-    ++n;
-  }
-};
+void ForceSet::add(const Box& bx, int i0, int i1, int j0, int j1, int k0, int k1) {
+  i0 = std::max(i0, 0); j0 = std::max(j0, 0); k0 = std::max(k0, 0);
+  i1 = std::min(i1, bx.nx - 1); j1 = std::min(j1, bx.ny - 1); k1 = std::min(k1, bx.nz - 1);
+  if (i0 > i1 || j0 > j1 || k0 > k1) return;
+  lo[n][0] = i0; lo[n][1] = j0; lo[n][2] = k0;
+  hi[n][0] = i1; hi[n][1] = j1; hi[n][2] = k1;
+  ++n;
+}
 
 // non-wetting phase: the inlet (the ghost plane i = ilo - 1 over the fill
 // box's j, k; 2D the whole ghost column; with a centre inlet the line
 // i = ci, k = ck, in 2D the point i = ci, j = cj)
-static ForceSet nw_force(const TrapArgs& a) {
+ForceSet trap_nw_force(const TrapArgs& a) {
   const Box& b = a.bx;
   const int ci = (int)(0.5 * (b.ilo + b.ihi)), cj = (int)(0.5 * (b.jlo + b.jhi));
   const int ck = (int)(0.5 * (b.klo + b.khi));
@@ -486,7 +503,7 @@ static ForceSet nw_force(const TrapArgs& a) {
 // j, k; with a centre inlet the ghost planes k = klo - 1, k = khi + 1,
 // i = ilo - 1, i = ihi + 1; 2D the fill column i = ilo + 1, or with a centre
 // inlet the four ghost lines)
-static ForceSet w_force(const TrapArgs& a) {
+ForceSet trap_w_force(const TrapArgs& a) {
   const Box& b = a.bx;
   const int all = 1 << 30;
   ForceSet f{};
@@ -510,28 +527,32 @@ static ForceSet w_force(const TrapArgs& a) {
   return f;
 }
 
-__global__ void force_bin_kernel(uint8_t* bin, Box bx, ForceSet f) {
+__device__ __forceinline__ void force_bin_body(uint8_t* bin, const Box& bx, const ForceSet& f,
+                                               int lb, int lg) {
   for (int q = 0; q < f.n; ++q) {
     const unsigned nx = (unsigned)(f.hi[q][0] - f.lo[q][0] + 1);
     const unsigned ny = (unsigned)(f.hi[q][1] - f.lo[q][1] + 1);
     const unsigned cnt = nx * ny * (unsigned)(f.hi[q][2] - f.lo[q][2] + 1);
-    for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < cnt; t += gridDim.x * blockDim.x) {
+    for (unsigned t = lb * blockDim.x + threadIdx.x; t < cnt; t += lg * blockDim.x) {
       const unsigned r = t / nx, kk = r / ny;
       bin[bx.at(f.lo[q][0] + (int)(t - r * nx), f.lo[q][1] + (int)(r - kk * ny),
                 f.lo[q][2] + (int)kk)] = 1;
     }
   }
 }
+__global__ void force_bin_kernel(uint8_t* bin, Box bx, ForceSet f) {
+  force_bin_body(bin, bx, f, blockIdx.x, gridDim.x);
+}
 
-__global__ void trap_w_mask_kernel(TrapArgs a) {
+__device__ __forceinline__ void trap_w_mask_body(const TrapArgs& a, int lb, int lg) {
   const int64_t n = a.bx.size();
   const int* __restrict__ T = a.troot;
   const int* __restrict__ L = a.labels;
   const real* __restrict__ sw = a.s;
   real* __restrict__ mw = a.mask_w;
   const int main_val = conn_value(L, T, a.w_ind);
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t l0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l0 < n; l0 += UNR * stride) {
+  const int64_t stride = (int64_t)lg * blockDim.x;
+  for (int64_t l0 = (int64_t)lb * blockDim.x + threadIdx.x; l0 < n; l0 += UNR * stride) {
     int t[UNR];
     real v[UNR];
 #pragma unroll
@@ -553,20 +574,119 @@ __global__ void trap_w_mask_kernel(TrapArgs a) {
     }
   }
 }
+__global__ void trap_w_mask_kernel(TrapArgs a) { trap_w_mask_body(a, blockIdx.x, gridDim.x); }
 
 hipError_t launch_trap_components(const TrapArgs& a, int conn, hipStream_t st) {
   int64_t n = a.bx.size();
-  hipLaunchKernelGGL(trap_nw_bin_kernel, dim3(grid_for(n)), dim3(256), 0, st, a);
-  hipLaunchKernelGGL(force_bin_kernel, dim3(256), dim3(256), 0, st, a.bin, a.bx, nw_force(a));
+  hipLaunchKernelGGL(trap_nw_bin_kernel, dim3(trap_grid(n)), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(force_bin_kernel, dim3(256), dim3(256), 0, st, a.bin, a.bx, trap_nw_force(a));
   hipError_t e = launch_label_components(a.bx, a.bin, a.labels, a.troot, conn, st, a.roots,
                                          a.roots_cap, false);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(trap_w_bin_kernel, dim3(grid_for(n)), dim3(256), 0, st, a);
-  hipLaunchKernelGGL(force_bin_kernel, dim3(256), dim3(256), 0, st, a.bin, a.bx, w_force(a));
+  hipLaunchKernelGGL(trap_w_bin_kernel, dim3(trap_grid(n)), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(force_bin_kernel, dim3(256), dim3(256), 0, st, a.bin, a.bx, trap_w_force(a));
   e = launch_label_components(a.bx, a.bin, a.labels, a.troot, conn, st, a.roots, a.roots_cap,
                               a.full_labels);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(trap_w_mask_kernel, dim3(grid_for(n)), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(trap_w_mask_kernel, dim3(trap_grid(n)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Trapping of many boxes at once (batched runs): the same passes, each one
+// launch over every box (Multi: block -> box and its block there).
+
+int trap_pass_blocks(const Box& bx) { return trap_grid(bx.size()); }
+int force_pass_blocks() { return 64; }
+int label_tile_blocks(const Box& bx) {
+  return ((bx.nx + CX - 1) / CX) * ((bx.ny + CY - 1) / CY) * ((bx.nz + CZ - 1) / CZ);
+}
+int label_jump_blocks(int64_t roots_cap) {
+  const int cap = (int)std::min<int64_t>(roots_cap, INT32_MAX - 1);
+  return (int)std::min<int64_t>(std::max<int64_t>((cap + 255) / 256, 1), 16384);
+}
+
+// tile t of box bx -> its position in the tile grid
+__device__ __forceinline__ void tile_pos(const Box& bx, int t, int& cx, int& cy, int& cz) {
+  const int tx = (bx.nx + CX - 1) / CX, ty = (bx.ny + CY - 1) / CY;
+  cx = t % tx;
+  cy = (t / tx) % ty;
+  cz = t / (tx * ty);
+}
+
+__global__ void trap_nw_bin_multi(const OuterJob* J, Multi m) {
+  int lb, lg;
+  const int r = multi_box(m, blockIdx.x, lb, lg);
+  trap_nw_bin_body(J[r].ta, lb, lg);
+}
+__global__ void trap_w_bin_multi(const OuterJob* J, Multi m) {
+  int lb, lg;
+  const int r = multi_box(m, blockIdx.x, lb, lg);
+  trap_w_bin_body(J[r].ta, lb, lg);
+}
+__global__ void trap_w_mask_multi(const OuterJob* J, Multi m) {
+  int lb, lg;
+  const int r = multi_box(m, blockIdx.x, lb, lg);
+  trap_w_mask_body(J[r].ta, lb, lg);
+}
+__global__ void force_bin_multi(const OuterJob* J, Multi m, int wet) {
+  int lb, lg;
+  const int r = multi_box(m, blockIdx.x, lb, lg);
+  force_bin_body(J[r].ta.bin, J[r].ta.bx, wet ? J[r].fw : J[r].fnw, lb, lg);
+}
+// one block per box
+__global__ void ccl_reset_multi(const OuterJob* J, Multi m) {
+  int lb, lg;
+  const int r = multi_box(m, blockIdx.x, lb, lg);
+  if (threadIdx.x == 0 && J[r].ta.roots) J[r].ta.roots[0] = 0;
+}
+__global__ __launch_bounds__(CX * CY) void ccl_local_multi(const OuterJob* J, Multi m, int conn) {
+  int lb, lg, cx, cy, cz;
+  const int r = multi_box(m, blockIdx.x, lb, lg);
+  const TrapArgs& a = J[r].ta;
+  tile_pos(a.bx, lb, cx, cy, cz);
+  const int cap = a.roots ? (int)min(a.roots_cap, (int64_t)(INT32_MAX - 1)) : 0;
+  ccl_local_tile(a.bx, a.bin, a.labels, a.troot, conn, a.roots, cap, false, cx, cy, cz);
+}
+template <int AX>
+__global__ __launch_bounds__(1024) void ccl_border_multi(const OuterJob* J, Multi m, int conn) {
+  int lb, lg, cx, cy, cz;
+  const int r = multi_box(m, blockIdx.x, lb, lg);
+  const TrapArgs& a = J[r].ta;
+  tile_pos(a.bx, lb, cx, cy, cz);
+  ccl_border_tile<AX>(a.bx, a.labels, a.troot, conn, cx, cy, cz);
+}
+__global__ void ccl_jump_multi(const OuterJob* J, Multi m) {
+  int lb, lg;
+  const int r = multi_box(m, blockIdx.x, lb, lg);
+  const TrapArgs& a = J[r].ta;
+  const int cap = a.roots ? (int)min(a.roots_cap, (int64_t)(INT32_MAX - 1)) : 0;
+  ccl_jump_body(a.labels, a.troot, (int)a.bx.size(), a.roots, cap, lb, lg);
+}
+
+// the labelling passes of launch_label_components (full = false)
+static void label_multi(const OuterJob* J, const Multi& one, const Multi& tiles, const Multi& jump,
+                        int conn, hipStream_t st) {
+  hipLaunchKernelGGL(ccl_reset_multi, dim3(one.total), dim3(64), 0, st, J, one);
+  hipLaunchKernelGGL(ccl_local_multi, dim3(tiles.total), dim3(CX, CY, 1), 0, st, J, tiles, conn);
+  hipLaunchKernelGGL(ccl_border_multi<0>, dim3(tiles.total), dim3(256), 0, st, J, tiles, conn);
+  hipLaunchKernelGGL(ccl_jump_multi, dim3(jump.total), dim3(256), 0, st, J, jump);
+  hipLaunchKernelGGL(ccl_border_multi<1>, dim3(tiles.total), dim3(1024), 0, st, J, tiles, conn);
+  hipLaunchKernelGGL(ccl_jump_multi, dim3(jump.total), dim3(256), 0, st, J, jump);
+  hipLaunchKernelGGL(ccl_border_multi<2>, dim3(tiles.total), dim3(1024), 0, st, J, tiles, conn);
+  hipLaunchKernelGGL(ccl_jump_multi, dim3(jump.total), dim3(256), 0, st, J, jump);
+}
+
+hipError_t launch_trap_multi(const OuterJob* J, const Multi& trap, const Multi& force,
+                             const Multi& one, const Multi& tiles, const Multi& jump, int conn,
+                             hipStream_t st) {
+  hipLaunchKernelGGL(trap_nw_bin_multi, dim3(trap.total), dim3(256), 0, st, J, trap);
+  hipLaunchKernelGGL(force_bin_multi, dim3(force.total), dim3(256), 0, st, J, force, 0);
+  label_multi(J, one, tiles, jump, conn, st);
+  hipLaunchKernelGGL(trap_w_bin_multi, dim3(trap.total), dim3(256), 0, st, J, trap);
+  hipLaunchKernelGGL(force_bin_multi, dim3(force.total), dim3(256), 0, st, J, force, 1);
+  label_multi(J, one, tiles, jump, conn, st);
+  hipLaunchKernelGGL(trap_w_mask_multi, dim3(trap.total), dim3(256), 0, st, J, trap);
   return hipGetLastError();
 }
 

@@ -62,14 +62,16 @@ inline int row_blocks(const Box& b, int cap) {
   return (int)(g < 1 ? 1 : (g > cap ? cap : g));
 }
 
+// lb, lg: the block's number and the block count of the launch over this box
+// (blockIdx.x and gridDim.x, or a box's share of a launch over many boxes)
 template <bool FILL, typename F>
-__device__ __forceinline__ void for_rows(const Box& b, F&& f) {
+__device__ __forceinline__ void for_rows_b(const Box& b, int lb, int lg, F&& f) {
   int i0, nxr, j0, nyr, k0, nzr;
   row_extent<FILL>(b, i0, nxr, j0, nyr, k0, nzr);
   const unsigned nrows = (unsigned)nyr * (unsigned)nzr;
   const int64_t plane = (int64_t)b.nx * b.ny;
-  const unsigned blk = (unsigned)xcd_tile(blockIdx.x, gridDim.x);
-  for (unsigned r = blk * ROW_TY + threadIdx.y; r < nrows; r += gridDim.x * ROW_TY) {
+  const unsigned blk = (unsigned)xcd_tile(lb, lg);
+  for (unsigned r = blk * ROW_TY + threadIdx.y; r < nrows; r += (unsigned)lg * ROW_TY) {
     const unsigned kk = r / (unsigned)nyr, jj = r - kk * (unsigned)nyr;
     const int j = j0 + (int)jj, k = k0 + (int)kk;
     const int64_t base = (int64_t)k * plane + (int64_t)j * b.nx;
@@ -79,6 +81,10 @@ __device__ __forceinline__ void for_rows(const Box& b, F&& f) {
     }
   }
 }
+template <bool FILL, typename F>
+__device__ __forceinline__ void for_rows(const Box& b, F&& f) {
+  for_rows_b<FILL>(b, (int)blockIdx.x, (int)gridDim.x, f);
+}
 
 // for_rows with U voxels of its row per lane and pass (64 apart): load(u,
 // idx) for all U first, at indices clamped to the row so that every load is
@@ -86,13 +92,13 @@ __device__ __forceinline__ void for_rows(const Box& b, F&& f) {
 // has U voxels' loads in flight at once instead of one, which latency-bound
 // stencil passes need to approach the HBM rate.
 template <bool FILL, int U, typename Ld, typename F>
-__device__ __forceinline__ void for_rows_u(const Box& b, Ld&& load, F&& use) {
+__device__ __forceinline__ void for_rows_u(const Box& b, int lb, int lg, Ld&& load, F&& use) {
   int i0, nxr, j0, nyr, k0, nzr;
   row_extent<FILL>(b, i0, nxr, j0, nyr, k0, nzr);
   const unsigned nrows = (unsigned)nyr * (unsigned)nzr;
   const int64_t plane = (int64_t)b.nx * b.ny;
-  const unsigned blk = (unsigned)xcd_tile(blockIdx.x, gridDim.x);
-  for (unsigned r = blk * ROW_TY + threadIdx.y; r < nrows; r += gridDim.x * ROW_TY) {
+  const unsigned blk = (unsigned)xcd_tile(lb, lg);
+  for (unsigned r = blk * ROW_TY + threadIdx.y; r < nrows; r += (unsigned)lg * ROW_TY) {
     const unsigned kk = r / (unsigned)nyr, jj = r - kk * (unsigned)nyr;
     const int j = j0 + (int)jj, k = k0 + (int)kk;
     const int64_t base = (int64_t)k * plane + (int64_t)j * b.nx + i0;

@@ -142,8 +142,8 @@ static void reinit_medium_dev(Session* s, real*& f, const Grid* g, const Options
 // full_labels: leave the wetting-phase labels resolved at every voxel (for
 // callers that read them back); otherwise only tile roots carry them
 // impose: phi = max(phi, impose) first, fused into the first trapping pass
-static void trap_dev(Session* s, const Grid* g, const Options* o, bool full_labels = false,
-                     const real* impose = nullptr) {
+static TrapArgs trap_args(Session* s, const Grid* g, const Options* o, bool full_labels,
+                          const real* impose) {
   ensure_labels(s);
   TrapArgs ta{};
   ta.impose_mask = impose;
@@ -156,9 +156,60 @@ static void trap_dev(Session* s, const Grid* g, const Options* o, bool full_labe
   ta.fill = -20 * g->dx[0];
   if (o->phi_nw_ind < 0 || o->phi_nw_ind >= s->n || o->phi_w_ind < 0 || o->phi_w_ind >= s->n)
     throw std::runtime_error("phi_nw_ind / phi_w_ind outside the ghost box (call getMainInd_new)");
+  return ta;
+}
+// 26, or 6 with LSMPQS_CONNECTIVITY=6
+static int label_connectivity() {
   int conn = getenv("LSMPQS_CONNECTIVITY") ? atoi(getenv("LSMPQS_CONNECTIVITY")) : 26;
-  if (conn != 6) conn = 26;
-  DCHECK(launch_trap_components(ta, conn, s->st));
+  return conn == 6 ? 6 : 26;
+}
+static void trap_dev(Session* s, const Grid* g, const Options* o, bool full_labels = false,
+                     const real* impose = nullptr) {
+  const TrapArgs ta = trap_args(s, g, o, full_labels, impose);
+  DCHECK(launch_trap_components(ta, label_connectivity(), s->st));
+}
+
+// The outer step's device work after the inner steps with trapping and a
+// single subcell-fix reinitialization iteration (dt_r = dx / 2 >= tmax_r > 0),
+// as a record a batched run executes together with other boxes': trapping,
+// reinitialization of mask_w, mask_nw (into A, then mask_w's old buffer),
+// trapped-phase velocities, merge and floor, reinitialization of phi with
+// the outer reductions (into mask_nw's old buffer).  outer_rotate() applies
+// the resulting buffer rotation to the session afterwards.
+static bool outer_batchable(const Options* o, const Grid* g) {
+  const real dt_r = real(0.5) * g->dx[0];
+  return o->check_connectivity && real(0) < o->tmax_r && !(dt_r < o->tmax_r);
+}
+static OuterJob outer_job(Session* s, const Grid* g, const Options* o, bool var, real eps,
+                          real dV) {
+  OuterJob j{};
+  j.ta = trap_args(s, g, o, false, var ? s->mask : nullptr);
+  j.fnw = trap_nw_force(j.ta);
+  j.fw = trap_w_force(j.ta);
+  j.rin[0] = s->mask_w;  j.rout[0] = s->A;
+  j.rin[1] = s->mask_nw; j.rout[1] = s->mask_w;
+  j.rin[2] = s->phi;     j.rout[2] = s->mask_nw;
+  j.dt_r = real(0.5) * g->dx[0];
+  j.dx = g->dx[0]; j.dy = g->dx[1]; j.dz = g->dx[2];
+  j.floor = s->floor;
+  j.overlap = o->overlap;
+  j.prev = s->phi_prev;
+  j.zone = o->err_check_zone; j.eps = eps; j.dV = dV;
+  j.part = s->part;
+  j.nred = reduce_pass_blocks(s->bx);
+  j.red = s->red;
+  if (var && o->conserve_imbibe) {
+    j.vn = s->vn; j.b = s->b; j.vx = s->vx; j.vy = s->vy; j.vz = s->vz;
+  }
+  j.eps_var = real(1.5) * g->dx[0];
+  return j;
+}
+static void outer_rotate(Session* s) {
+  real* a0 = s->A;
+  s->A = s->phi;
+  s->phi = s->mask_nw;
+  s->mask_nw = s->mask_w;
+  s->mask_w = a0;
 }
 
 // Sorted list of fill voxels with |vel_n| < 1e-5 and, for each, the voxel
@@ -475,6 +526,7 @@ static void outer_reduce(Session* s, const real* f, const real* f2, real zone, r
 // (reference: src/level_set_evolution/constCurvModel3d.c:50-334, 336-610).
 static real ccm_dev(Session* s, Options* o, const Grid* g, FILE* fp, bool var, real a_const) {
   const Box& bx = s->bx;
+  BatchHook* hook = batch_hook();
   real vsmall = g->dx[0] * g->dx[1] * g->dx[2];
   real eps = o->eps_coefficient * g->dx[0];
   real dV = g->dx[0] * g->dx[1] * g->dx[2];
@@ -515,9 +567,23 @@ static real ccm_dev(Session* s, Options* o, const Grid* g, FILE* fp, bool var, r
     if (fp) fprintf(fp, "1 GPU: Level set time = %lf\n", ms * 1e-3);
     const auto t3 = std::chrono::steady_clock::now();
     t += dt_sub;
-    if (var && !o->check_connectivity)
+    bool batched = false;
+    if (hook && outer_batchable(o, g)) {
+      // the batch runs this box's outer-step device work with the other
+      // boxes' and leaves the reductions in s->h_red
+      batched = hook->outer(s, outer_job(s, g, o, var, eps, dV), fast_fp32(),
+                            label_connectivity());
+      if (batched) {
+        outer_rotate(s);
+        if (var && o->conserve_imbibe) build_stale(s, g);
+      }
+    }
+    if (batched) {
+      // done
+    } else if (var && !o->check_connectivity) {
       DCHECK(launch_impose(s->phi, s->mask, s->phi, 0, s->n, s->st));
-    if (o->check_connectivity) {
+    }
+    if (!batched && o->check_connectivity) {
       trap_dev(s, g, o, false, var ? s->mask : nullptr);
       reinit_dev(s, s->mask_w, g, o, nullptr, fast_fp32());
       reinit_dev(s, s->mask_nw, g, o, nullptr, fast_fp32());
@@ -530,10 +596,12 @@ static real ccm_dev(Session* s, Options* o, const Grid* g, FILE* fp, bool var, r
                                 s->n, s->st));
     }
     if (fp) fprintf(fp, "Reinitializing....");
-    const ReduceReq rq{s->phi_prev, o->err_check_zone, eps, dV};
-    reinit_dev(s, s->phi, g, o, &rq, fast_fp32());
+    if (!batched) {
+      const ReduceReq rq{s->phi_prev, o->err_check_zone, eps, dV};
+      reinit_dev(s, s->phi, g, o, &rq, fast_fp32());
+    }
     if (fp) fprintf(fp, "Reinitialized\n");
-    read_reduce(s);
+    if (!batched) read_reduce(s);
     vol_phi = s->h_red[0];
     max_abs_err = s->h_red[1];
     satn_old = satn;

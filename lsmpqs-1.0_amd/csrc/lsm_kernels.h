@@ -241,4 +241,78 @@ hipError_t launch_label_components(const Box& bx, const uint8_t* bin, int* label
                                    int connectivity, hipStream_t st, int* roots = nullptr,
                                    int64_t roots_cap = 0, bool full = true);
 
+// The voxels the reference forces into the binary field before labelling:
+// up to four boxes of the ghost box, inclusive bounds.
+struct ForceSet {
+  int n;
+  int lo[4][3], hi[4][3];
+  void add(const Box& bx, int i0, int i1, int j0, int j1, int k0, int k1);
+};
+ForceSet trap_nw_force(const TrapArgs& a);  // the inlet voxels of the non-wetting labelling
+ForceSet trap_w_force(const TrapArgs& a);   // the outlet voxels of the wetting labelling
+
+// ---------------------------------------------------------------------------
+// One outer step of many boxes at once (batched runs).  Each launch covers
+// every box: block b serves box r, prefix[r] <= b < prefix[r + 1], as block
+// b - prefix[r] of prefix[r + 1] - prefix[r], and each box gets the grid its
+// single-box launch uses, so every voxel sees the single-box arithmetic and
+// the reductions add in the same order.
+struct Multi {
+  const int* prefix;  // device, n + 1 entries
+  int n;
+  int total;          // prefix[n]: blocks of the launch
+};
+__device__ __forceinline__ int multi_box(const Multi& m, int b, int& lb, int& lg) {
+  int lo = 0, hi = m.n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (m.prefix[mid] <= b) lo = mid; else hi = mid - 1;
+  }
+  lb = b - m.prefix[lo];
+  lg = m.prefix[lo + 1] - m.prefix[lo];
+  return lo;
+}
+
+// Everything one box's outer step needs (constCurvModel3d.c:258-292 with
+// trapping and a single subcell-fix reinitialization iteration).  The three
+// reinitializations read rin[p] and write rout[p] (mask_w, mask_nw, phi).
+struct OuterJob {
+  TrapArgs ta;
+  ForceSet fnw, fw;
+  real* rin[3];
+  real* rout[3];
+  real dt_r, dx, dy, dz;
+  real* floor;             // merge and floor: phi = ta.phi, masks rout[0], rout[1]
+  real overlap;
+  const real* prev;        // reductions of rout[2] against prev
+  real zone, eps, dV;
+  double* part;
+  int nred;                // partials of that reduction (reduce_pass_blocks)
+  double* red;
+  real *vn, *b, *vx, *vy, *vz;  // trapped-phase velocities from rout[1], or vn = NULL
+  real eps_var;
+};
+// Block counts per box of each launch kind (the single-box launchers' grids).
+int trap_pass_blocks(const Box& bx);      // trapping passes over the ghost box
+int force_pass_blocks();                  // forced inlet/outlet voxels
+int label_tile_blocks(const Box& bx);     // labelling tiles
+int label_jump_blocks(int64_t roots_cap); // root-jump passes
+int reinit_pass_blocks(const Box& bx);    // reinitialization pass
+int reduce_pass_blocks(const Box& bx);    // reinitialization pass with the outer reductions
+int stream_pass_blocks(int64_t n);        // merge and floor
+int bc_pass_blocks(const Box& bx);        // ghost extrapolation
+// The launch sequences (jobs: device copies of the records).
+hipError_t launch_trap_multi(const OuterJob* jobs, const Multi& trap, const Multi& force,
+                             const Multi& one, const Multi& tiles, const Multi& jump, int conn,
+                             hipStream_t st);
+hipError_t launch_reinit_multi(const OuterJob* jobs, int pass, const Multi& rows, bool fast,
+                               hipStream_t st);
+hipError_t launch_reinit_reduce_multi(const OuterJob* jobs, const Multi& rows, const Multi& one,
+                                      bool fast, hipStream_t st);
+hipError_t launch_merge_floor_multi(const OuterJob* jobs, const Multi& stream, hipStream_t st);
+hipError_t launch_trap_velocity_multi(const OuterJob* jobs, const Multi& rows, hipStream_t st);
+hipError_t launch_bc_multi(const OuterJob* jobs, int pass, const Multi& ghosts, hipStream_t st);
+// out[2r], out[2r + 1] = jobs[r].red[0], [1]
+hipError_t launch_gather_red(const OuterJob* jobs, int n, double* out, hipStream_t st);
+
 }  // namespace lsm

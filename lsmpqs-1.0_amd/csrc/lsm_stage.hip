@@ -681,14 +681,40 @@ int64_t ghost_count(const Box& bx) {
   return nA + nB + nC;
 }
 
+int bc_pass_blocks(const Box& bx) {
+  int64_t nA, nB, nC;
+  ghost_regions(bx, nA, nB, nC);
+  int64_t tot = nA + nB + nC;
+  int blocks = (int)std::min<int64_t>((tot + 255) / 256, 8192);
+  return blocks < 1 ? 1 : blocks;
+}
+
 hipError_t launch_bc(const BcArgs& a, bool final_mask, hipStream_t st) {
   int64_t nA, nB, nC;
   ghost_regions(a.bx, nA, nB, nC);
-  int64_t tot = nA + nB + nC;
-  int blocks = (int)std::min<int64_t>((tot + 255) / 256, 8192);
-  if (blocks < 1) blocks = 1;
+  const int blocks = bc_pass_blocks(a.bx);
   if (final_mask) hipLaunchKernelGGL(bc_kernel<true>, dim3(blocks), dim3(256), 0, st, a, nA, nB, nC);
   else hipLaunchKernelGGL(bc_kernel<false>, dim3(blocks), dim3(256), 0, st, a, nA, nB, nC);
+  return hipGetLastError();
+}
+
+// ghost extrapolation of the reinitialized field rout[pass] of many boxes
+// (batched outer steps; Multi: block -> box and its block there)
+__global__ void bc_multi_kernel(const OuterJob* J, Multi m, int pass) {
+  int lb, lg;
+  const int r = multi_box(m, blockIdx.x, lb, lg);
+  BcArgs a{};
+  a.bx = J[r].ta.bx;
+  a.src = J[r].rout[pass];
+  a.dst = J[r].rout[pass];
+  int64_t nA, nB, nC;
+  ghost_regions(a.bx, nA, nB, nC);
+  const int64_t stride = (int64_t)lg * blockDim.x;
+  for (int64_t g = (int64_t)lb * blockDim.x + threadIdx.x; g < nA + nB + nC; g += stride)
+    bc_one<false>(a, g, nA, nB);
+}
+hipError_t launch_bc_multi(const OuterJob* J, int pass, const Multi& ghosts, hipStream_t st) {
+  hipLaunchKernelGGL(bc_multi_kernel, dim3(ghosts.total), dim3(256), 0, st, J, ghosts, pass);
   return hipGetLastError();
 }
 
@@ -803,15 +829,23 @@ int stage_bricks(StageArgs& a, int zc) {
   return a.tiles_x * a.tiles_y * a.tiles_z;
 }
 
+// Each block takes a contiguous chunk of the ghost voxels of all boxes; a
+// thread finds the box of its first voxel by binary search and then steps
+// forward through the prefix sums as its voxels advance.
 template <bool FINAL>
 __global__ void bc_batch_kernel(BatchView v, int si) {
   const int n = v.count[0];
   const int64_t total = v.ghost_prefix[n];
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += stride) {
-    const int r = find_rec(v.ghost_prefix, n, g);
+  const int64_t chunk = (total + gridDim.x - 1) / gridDim.x;
+  const int64_t g0 = (int64_t)blockIdx.x * chunk;
+  const int64_t g1 = min(g0 + chunk, total);
+  int r = -1;
+  int64_t nA = 0, nB = 0, nC = 0;
+  for (int64_t g = g0 + threadIdx.x; g < g1; g += blockDim.x) {
+    if (r < 0) r = find_rec(v.ghost_prefix, n, g);
+    else
+      while (v.ghost_prefix[r + 1] <= g) ++r;
     const BcArgs& a = v.recs[r].bc[si];
-    int64_t nA, nB, nC;
     ghost_regions(a.bx, nA, nB, nC);
     bc_one<FINAL>(a, g - v.ghost_prefix[r], nA, nB);
   }

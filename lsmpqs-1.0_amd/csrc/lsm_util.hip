@@ -135,8 +135,8 @@ __global__ __launch_bounds__(256) void outer_reduce_kernel(Box bx, const real* f
   }
 }
 
-__global__ __launch_bounds__(256) void finish_reduce_kernel(const double* part, int nb, double* out,
-                                                            int mode) {
+__device__ __forceinline__ void finish_reduce_body(const double* part, int nb, double* out,
+                                                   int mode) {
   // mode 0: out[0] = sum(part[0..nb)), out[1] = max(part[RB..RB+nb))
   // mode 1: out[0] = max(part[0..nb)), out[1] = max(part[RB..RB+nb))
   double a = mode == 0 ? 0.0 : -DBL_MAX, m = -DBL_MAX;
@@ -159,6 +159,10 @@ __global__ __launch_bounds__(256) void finish_reduce_kernel(const double* part, 
     out[0] = x;
     out[1] = y;
   }
+}
+__global__ __launch_bounds__(256) void finish_reduce_kernel(const double* part, int nb, double* out,
+                                                            int mode) {
+  finish_reduce_body(part, nb, out, mode);
 }
 
 hipError_t launch_outer_reduce(const Box& bx, const real* f1, const real* f2, real zone, real eps,
@@ -375,28 +379,35 @@ __device__ __forceinline__ real reinit_first_value(const Stencil7& v, real dt_r,
 constexpr int REINIT_U = 4;  // voxels per lane and pass of the first-step kernels
 
 template <bool FAST>
-__global__ void reinit_first_step_kernel(Box bx, real* __restrict__ next,
-                                         const real* __restrict__ phi0, real dt_r, real dx,
-                                         real dy, real dz) {
+__device__ __forceinline__ void reinit_first_body(const Box& bx, real* __restrict__ next,
+                                                  const real* __restrict__ phi0, real dt_r, real dx,
+                                                  real dy, real dz, int lb, int lg) {
   real max_dx = tmax(tmax(dx, dy), dz);
   real flag = real(-1000.0) * max_dx;
   real inv_dx = real(1.0) / dx, inv_dy = real(1.0) / dy, inv_dz = real(1.0) / dz;
   int64_t sy = bx.nx, sz = (int64_t)bx.nx * bx.ny;
   Stencil7 v[REINIT_U];
   for_rows_u<true, REINIT_U>(
-      bx, [&](int u, int64_t idx) { v[u] = stencil7(phi0, idx, sy, sz); },
+      bx, lb, lg, [&](int u, int64_t idx) { v[u] = stencil7(phi0, idx, sy, sz); },
       [&](int u, int64_t idx, int, int, int) {
         next[idx] = reinit_first_value<FAST>(v[u], dt_r, inv_dx, inv_dy, inv_dz, flag, max_dx);
       });
+}
+template <bool FAST>
+__global__ void reinit_first_step_kernel(Box bx, real* __restrict__ next,
+                                         const real* __restrict__ phi0, real dt_r, real dx,
+                                         real dy, real dz) {
+  reinit_first_body<FAST>(bx, next, phi0, dt_r, dx, dy, dz, blockIdx.x, gridDim.x);
 }
 // reinit_first_step_kernel fused with the outer reductions of its result:
 // per-block FP64 partials of the smoothed volume of {next < 0} and of the
 // local max-norm |next - prev| over |next| < zone, in the same order as
 // outer_reduce_kernel (same launch shape), so the sums are the same bits.
 template <bool FAST>
-__global__ __launch_bounds__(256) void reinit_first_step_reduce_kernel(
-    Box bx, real* __restrict__ next, const real* __restrict__ phi0, real dt_r, real dx, real dy,
-    real dz, const real* __restrict__ prev, real zone, real eps, real dV, double* part) {
+__device__ __forceinline__ void reinit_first_reduce_body(
+    const Box& bx, real* __restrict__ next, const real* __restrict__ phi0, real dt_r, real dx,
+    real dy, real dz, const real* __restrict__ prev, real zone, real eps, real dV, double* part,
+    int lb, int lg) {
   const real pi = real(3.14159265358979323846), one_over_pi = real(0.31830988618379);
   real max_dx = tmax(tmax(dx, dy), dz);
   real flag = real(-1000.0) * max_dx;
@@ -409,7 +420,7 @@ __global__ __launch_bounds__(256) void reinit_first_step_reduce_kernel(
   // the voxels of a lane are visited in the order of for_rows (u runs along
   // the row), so the per-lane sums are added in the same order
   for_rows_u<true, REINIT_U>(
-      bx,
+      bx, lb, lg,
       [&](int u, int64_t idx) {
         v[u] = stencil7(phi0, idx, sy, sz);
         pv[u] = prev[idx];
@@ -434,9 +445,16 @@ __global__ __launch_bounds__(256) void reinit_first_step_reduce_kernel(
   if (lane == 0 && wid == 0) {
     double a = 0.0, m = -1.0;
     for (int w = 0; w < ROW_TY; ++w) { a += s_sum[w]; m = fmax(m, s_max[w]); }
-    part[blockIdx.x] = a;
-    part[RED_BLOCKS + blockIdx.x] = m;
+    part[lb] = a;
+    part[RED_BLOCKS + lb] = m;
   }
+}
+template <bool FAST>
+__global__ __launch_bounds__(256) void reinit_first_step_reduce_kernel(
+    Box bx, real* __restrict__ next, const real* __restrict__ phi0, real dt_r, real dx, real dy,
+    real dz, const real* __restrict__ prev, real zone, real eps, real dV, double* part) {
+  reinit_first_reduce_body<FAST>(bx, next, phi0, dt_r, dx, dy, dz, prev, zone, eps, dV, part,
+                                 blockIdx.x, gridDim.x);
 }
 hipError_t launch_reinit_first_step_reduce(const Box& bx, real* next, const real* phi0, real dt_r,
                                            real dx, real dy, real dz, const real* prev, real zone,
@@ -455,10 +473,11 @@ hipError_t launch_reinit_first_step_reduce(const Box& bx, real* next, const real
 
 // merge and imposition floor in one pass: phi = -mask_nw where mask_nw > 0
 // (merge_kernel) and floor = max(mask - overlap, mask_w, mask_nw) (floor_kernel)
-__global__ void merge_floor_kernel(real* phi, real* f, const real* m, const real* mw,
-                                   const real* mnw, real ov, int64_t n) {
-  for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n;
-       l += (int64_t)gridDim.x * blockDim.x) {
+__device__ __forceinline__ void merge_floor_body(real* phi, real* f, const real* m, const real* mw,
+                                                 const real* mnw, real ov, int64_t n, int lb,
+                                                 int lg) {
+  for (int64_t l = (int64_t)lb * blockDim.x + threadIdx.x; l < n;
+       l += (int64_t)lg * blockDim.x) {
     const real b = mnw[l];
     if (b > real(0)) phi[l] = -b;
     real v = m[l] - ov;
@@ -467,6 +486,10 @@ __global__ void merge_floor_kernel(real* phi, real* f, const real* m, const real
     v = (b > v) ? b : v;
     f[l] = v;
   }
+}
+__global__ void merge_floor_kernel(real* phi, real* f, const real* m, const real* mw,
+                                   const real* mnw, real ov, int64_t n) {
+  merge_floor_body(phi, f, m, mw, mnw, ov, n, blockIdx.x, gridDim.x);
 }
 hipError_t launch_merge_floor(real* phi, real* floor, const real* m, const real* mw,
                               const real* mnw, real overlap, int64_t n, hipStream_t st) {
@@ -547,15 +570,20 @@ hipError_t launch_var_curv_adv(const Box& bx, real* b, real* vx, real* vy, real*
 }
 
 // reference: src/utilities/qss_util3d.f:499-581
-__global__ void trap_velocity_kernel(Box bx, const real* phi_w, real* vn, real* b, real* vx,
-                                     real* vy, real* vz, real eps_var) {
-  for_rows<true>(bx, [&](int64_t idx, int i, int j, int k) {
+__device__ __forceinline__ void trap_velocity_body(const Box& bx, const real* phi_w, real* vn,
+                                                   real* b, real* vx, real* vy, real* vz,
+                                                   real eps_var, int lb, int lg) {
+  for_rows_b<true>(bx, lb, lg, [&](int64_t idx, int i, int j, int k) {
     (void)i; (void)j; (void)k;
     if (phi_w[idx] > eps_var) {
       vn[idx] = real(0); b[idx] = real(0);
       vx[idx] = real(0); vy[idx] = real(0); vz[idx] = real(0);
     }
   });
+}
+__global__ void trap_velocity_kernel(Box bx, const real* phi_w, real* vn, real* b, real* vx,
+                                     real* vy, real* vz, real eps_var) {
+  trap_velocity_body(bx, phi_w, vn, b, vx, vy, vz, eps_var, blockIdx.x, gridDim.x);
 }
 hipError_t launch_trap_velocity(const Box& bx, const real* phi_w, real* vn, real* b, real* vx,
                                 real* vy, real* vz, real dx, hipStream_t st) {
@@ -652,6 +680,90 @@ hipError_t launch_box(real* phi, const Box& bx, const real* xlo, const real* dx,
   }
   hipLaunchKernelGGL(box_kernel, dim3(stream_blocks(bx.size())), dim3(256), 0, st, phi, bx, c, cs,
                      inside);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Reinitialization, merge and floor, trapped-phase velocities and reductions
+// of many boxes at once (batched runs; Multi: block -> box and its block).
+
+int reinit_pass_blocks(const Box& bx) { return row_blocks<true>(bx, 16384); }
+int reduce_pass_blocks(const Box& bx) { return row_blocks<true>(bx, RED_BLOCKS); }
+int stream_pass_blocks(int64_t n) { return stream_blocks(n); }
+
+template <bool FAST>
+__global__ void reinit_first_multi(const OuterJob* J, Multi m, int pass) {
+  int lb, lg;
+  const int r = multi_box(m, blockIdx.x, lb, lg);
+  const OuterJob& j = J[r];
+  reinit_first_body<FAST>(j.ta.bx, j.rout[pass], j.rin[pass], j.dt_r, j.dx, j.dy, j.dz, lb, lg);
+}
+template <bool FAST>
+__global__ __launch_bounds__(256) void reinit_reduce_multi(const OuterJob* J, Multi m) {
+  int lb, lg;
+  const int r = multi_box(m, blockIdx.x, lb, lg);
+  const OuterJob& j = J[r];
+  reinit_first_reduce_body<FAST>(j.ta.bx, j.rout[2], j.rin[2], j.dt_r, j.dx, j.dy, j.dz, j.prev,
+                                 j.zone, j.eps, j.dV, j.part, lb, lg);
+}
+// one block per box, over the box's reduce_pass_blocks partials
+__global__ __launch_bounds__(256) void finish_reduce_multi(const OuterJob* J, Multi m) {
+  int lb, lg;
+  const int r = multi_box(m, blockIdx.x, lb, lg);
+  finish_reduce_body(J[r].part, J[r].nred, J[r].red, 0);
+}
+__global__ void merge_floor_multi(const OuterJob* J, Multi m) {
+  int lb, lg;
+  const int r = multi_box(m, blockIdx.x, lb, lg);
+  const OuterJob& j = J[r];
+  merge_floor_body(j.ta.phi, j.floor, j.ta.mask, j.rout[0], j.rout[1], j.overlap, j.ta.bx.size(),
+                   lb, lg);
+}
+__global__ void trap_velocity_multi(const OuterJob* J, Multi m) {
+  int lb, lg;
+  const int r = multi_box(m, blockIdx.x, lb, lg);
+  const OuterJob& j = J[r];
+  if (j.vn) trap_velocity_body(j.ta.bx, j.rout[1], j.vn, j.b, j.vx, j.vy, j.vz, j.eps_var, lb, lg);
+}
+__global__ void gather_red_kernel(const OuterJob* J, int n, double* out) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) {
+    out[2 * r] = J[r].red[0];
+    out[2 * r + 1] = J[r].red[1];
+  }
+}
+
+hipError_t launch_reinit_multi(const OuterJob* J, int pass, const Multi& rows, bool fast,
+                               hipStream_t st) {
+  if (fast)
+    hipLaunchKernelGGL(reinit_first_multi<true>, dim3(rows.total), dim3(ROW_TX, ROW_TY), 0, st, J,
+                       rows, pass);
+  else
+    hipLaunchKernelGGL(reinit_first_multi<false>, dim3(rows.total), dim3(ROW_TX, ROW_TY), 0, st, J,
+                       rows, pass);
+  return hipGetLastError();
+}
+hipError_t launch_reinit_reduce_multi(const OuterJob* J, const Multi& rows, const Multi& one,
+                                      bool fast, hipStream_t st) {
+  if (fast)
+    hipLaunchKernelGGL(reinit_reduce_multi<true>, dim3(rows.total), dim3(ROW_TX, ROW_TY), 0, st, J,
+                       rows);
+  else
+    hipLaunchKernelGGL(reinit_reduce_multi<false>, dim3(rows.total), dim3(ROW_TX, ROW_TY), 0, st,
+                       J, rows);
+  hipLaunchKernelGGL(finish_reduce_multi, dim3(one.total), dim3(256), 0, st, J, one);
+  return hipGetLastError();
+}
+hipError_t launch_merge_floor_multi(const OuterJob* J, const Multi& stream, hipStream_t st) {
+  hipLaunchKernelGGL(merge_floor_multi, dim3(stream.total), dim3(256), 0, st, J, stream);
+  return hipGetLastError();
+}
+hipError_t launch_trap_velocity_multi(const OuterJob* J, const Multi& rows, hipStream_t st) {
+  hipLaunchKernelGGL(trap_velocity_multi, dim3(rows.total), dim3(ROW_TX, ROW_TY), 0, st, J, rows);
+  return hipGetLastError();
+}
+hipError_t launch_gather_red(const OuterJob* J, int n, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(gather_red_kernel, dim3((n + 63) / 64), dim3(64), 0, st, J, n, out);
   return hipGetLastError();
 }
 

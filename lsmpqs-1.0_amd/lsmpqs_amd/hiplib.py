@@ -28,7 +28,8 @@ class Timing(C.Structure):
 class BatchStats(C.Structure):
     _fields_ = [("boxes", C.c_int), ("batched_steps", C.c_longlong), ("box_steps", C.c_longlong),
                 ("brick_depth", C.c_int), ("workgroups", C.c_int), ("batched_ms", C.c_double),
-                ("autotune_best_ms", C.c_double)]
+                ("autotune_best_ms", C.c_double), ("batched_outer", C.c_longlong),
+                ("box_outer", C.c_longlong), ("batched_outer_ms", C.c_double)]
 
 
 # every exported entry point declared in include/lsmpqs_hip.h and qss_data_arrays.h

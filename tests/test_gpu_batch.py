@@ -19,11 +19,12 @@ SHAPES = [(20, 24, 18), (28, 20, 22), (16, 30, 24), (24, 24, 24), (18, 22, 30)]
 STARTS = [0.02, 0.025, 0.018, 0.03, 0.022]
 
 
-def boxes(dt):
+def boxes(dt, theta_deg=0.0):
     out = []
     for i, (shape, a) in enumerate(zip(SHAPES, STARTS)):
         g, mask, phi0 = case(shape, dt, seed=2000 + i)
-        o = options(dt, space=3, time=3, a=a, b=0.04, dc=0.25, steps=3 + (i % 2), tmax=15.0)
+        o = options(dt, space=3, time=3, theta_deg=theta_deg, a=a, b=0.04, dc=0.25,
+                    steps=3 + (i % 2), tmax=15.0)
         A = Arrays(g, dt)
         A["mask"] = mask
         A["phi"] = phi0
@@ -41,12 +42,17 @@ def run_single(lib, items, drain):
 @pytest.mark.gpu
 @pytest.mark.parametrize("dt", [np.float32, np.float64])
 @pytest.mark.parametrize("drain", [True, False])
-def test_batched_boxes_match_single_runs(dt, drain):
-    print(f"case: batch {np.dtype(dt).name} drain={drain} boxes={len(SHAPES)}", flush=True)
+@pytest.mark.parametrize("theta_deg", [0.0, 45.0])
+def test_batched_boxes_match_single_runs(dt, drain, theta_deg):
+    """theta 0: constCurvModel3dNoVar's path; 45 deg: the contact-angle path,
+    whose imbibition also zeroes the coefficients of trapped wetting voxels
+    in the outer steps."""
+    print(f"case: batch {np.dtype(dt).name} drain={drain} theta={theta_deg} "
+          f"boxes={len(SHAPES)}", flush=True)
     lib = hiplib.load(dt)
     lib.lsmpqs_set_dt_slabs(4)
     os.environ["LSMPQS_NO_STEP_FILES"] = "1"
-    single, batched = boxes(dt), boxes(dt)
+    single, batched = boxes(dt, theta_deg), boxes(dt, theta_deg)
     if not drain:  # imbibe from a partly drained state
         run_single(lib, single, True)
         run_single(lib, batched, True)
@@ -65,8 +71,10 @@ def test_batched_boxes_match_single_runs(dt, drain):
         lib.lsmpqs_release(A.ptr())
     assert st.boxes == n and st.batched_steps > 0 and st.box_steps >= st.batched_steps
     assert st.brick_depth in (8, 16, 32, 64) and st.workgroups > 0
+    assert st.batched_outer > 0 and st.box_outer >= st.batched_outer
     print(f"batched steps {st.batched_steps}, box steps {st.box_steps}, brick depth "
-          f"{st.brick_depth}, workgroups {st.workgroups}", flush=True)
+          f"{st.brick_depth}, workgroups {st.workgroups}; batched outer steps "
+          f"{st.batched_outer} serving {st.box_outer}", flush=True)
     for (g1, o1, A1), (g2, o2, A2) in zip(single, batched):
         for f in ("phi", "mask_w", "mask_nw"):
             assert np.array_equal(A1[f], A2[f]), f
