@@ -17,6 +17,7 @@
 #include <vector>
 #include <memory>
 #include "lsm_batch.h"
+#include "lsm_xfer.h"
 #include "lsm_gz.h"
 #include "lsm_session.h"
 #include "lsmpqs_hip.h"
@@ -32,12 +33,15 @@ namespace lsm {
 
 static size_t bytes(const Session* s) { return (size_t)s->n * sizeof(real); }
 
+// whole fields between the caller's host arrays and the session, in stream
+// order; both return once the copy is complete (copy_to_device and
+// copy_to_host stage large fields through pinned buffers)
 static void up(Session* s, real* d, const real* h) {
   if (!h) throw std::runtime_error("required host array is NULL");
-  DCHECK(hipMemcpyAsync(d, h, bytes(s), hipMemcpyHostToDevice, s->st));
+  DCHECK(copy_to_device(d, h, bytes(s), s->st));
 }
 static void down(Session* s, real* h, const real* d) {
-  if (h) DCHECK(hipMemcpyAsync(h, d, bytes(s), hipMemcpyDeviceToHost, s->st));
+  if (h) DCHECK(copy_to_host(h, d, bytes(s), s->st));
 }
 static void sync(Session* s) { DCHECK(hipStreamSynchronize(s->st)); }
 
