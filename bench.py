@@ -361,12 +361,25 @@ def adjoint_leg(dtype, n=128, steps=200, checkpoint_every=20):
                          checkpoint_every=checkpoint_every)
     tg = time.perf_counter() - w0
     nvox = n ** 3
-    return {"grid": n, "steps": steps, "checkpoint_every": checkpoint_every,
-            "forward_s": round(tf, 4), "gradient_s": round(tg, 4),
-            "gradient_over_forward": round(tg / tf, 2),
-            "forward_voxel_updates_per_s": round(nvox * steps / tf, 1),
-            "sw": round(rg.sw, 8), "loss": rg.loss, "dloss_db": rg.dloss_db,
-            "same_loss_as_forward": rf.loss == rg.loss}
+    out = {"grid": n, "steps": steps, "checkpoint_every": checkpoint_every,
+           "forward_s": round(tf, 4), "gradient_s": round(tg, 4),
+           "gradient_over_forward": round(tg / tf, 2),
+           "forward_voxel_updates_per_s": round(nvox * steps / tf, 1),
+           "sw": round(rg.sw, 8), "loss": rg.loss, "dloss_db": rg.dloss_db,
+           "same_loss_as_forward": rf.loss == rg.loss}
+    if np.dtype(dtype) == np.float32:
+        # fit of the per-voxel angle field to the saturation of the same field 8 deg wider
+        from lsmpqs_amd.adjoint import fit_angle_field
+        shifted = (theta + np.float32(np.radians(8.0))).astype(dtype)
+        tgt = saturation_loss(phi0, mask, shifted, g, o, steps, a0, b, target, dt=rg.dt,
+                              gradient=False).sw
+        w0 = time.perf_counter()
+        _, losses = fit_angle_field(phi0, mask, theta, g, o, steps, a0, b, tgt, rg.dt,
+                                    iterations=6, checkpoint_every=checkpoint_every)
+        out["field_fit"] = {"target_sw": round(tgt, 8), "losses": [float("%.4e" % x) for x in losses],
+                            "gauss_newton_steps": len(losses) - 1,
+                            "wall_s": round(time.perf_counter() - w0, 3)}
+    return out
 
 
 def batch_leg(tmax=20.0, seed=1004, sequential=True):

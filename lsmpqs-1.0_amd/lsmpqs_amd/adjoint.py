@@ -10,7 +10,8 @@ with respect to the contact-angle field and the curvature coefficient b,
 computed by the library's hand-written reverse pass on the GPU.
 
 `ThetaLoss` wraps it as a torch.autograd.Function so a contact-angle field
-(and b) can be fitted with torch optimisers.  Tensors travel to the library
+(and b) can be fitted with torch optimisers; `fit_angle_field` fits a
+per-voxel angle field to a target saturation by Gauss-Newton steps.  Tensors travel to the library
 as host arrays over the ghost box; there is no CPU fallback.
 """
 import ctypes as C
@@ -88,6 +89,40 @@ def saturation_loss(phi0, mask, theta, grid, options, nsteps, a0, b, target_sw, 
         # every path, so a later record at the same address starts clean
         lib.lsmpqs_release(C.c_void_p(C.addressof(rec)))
     return AdjointResult(a, dtheta, phi)
+
+
+def fit_angle_field(phi0, mask, theta0, grid, options, nsteps, a0, b, target_sw, dt,
+                    iterations=6, checkpoint_every=None, lo=0.0, hi=0.5 * np.pi, region=None):
+    """Fit a per-voxel contact-angle field to a target wetting saturation by
+    Gauss-Newton steps on the whole field: with the residual r = S_w - target
+    and g = dS_w/dtheta = (dloss/dtheta) / (2 r) from the reverse pass, the
+    minimum-norm step theta -= r g / (g . g), clipped to [lo, hi], on the
+    voxels of `region` (a boolean mask over the ghost box; default the fill
+    box, where the gradient lives).  Stops early when the loss is 0 or the
+    gradient vanishes.  Returns (theta, [loss of each evaluated field])."""
+    theta = np.array(theta0, copy=True)
+    if region is None:
+        region = np.zeros(theta.shape, dtype=bool)
+        region[grid.klo_fb:grid.khi_fb + 1, grid.jlo_fb:grid.jhi_fb + 1,
+               grid.ilo_fb:grid.ihi_fb + 1] = True
+    losses = []
+    for _ in range(iterations):
+        r = saturation_loss(phi0, mask, theta, grid, options, nsteps, a0, b, target_sw, dt=dt,
+                            checkpoint_every=checkpoint_every)
+        losses.append(r.loss)
+        res = r.sw - target_sw
+        if res == 0.0:
+            break
+        g = np.where(region, r.dtheta.astype(np.float64) / (2.0 * res), 0.0)
+        gg = float(np.sum(g * g))
+        if gg == 0.0:
+            break
+        step = theta.astype(np.float64) - (res / gg) * g
+        theta = np.clip(step, lo, hi).astype(theta.dtype)
+    else:
+        losses.append(saturation_loss(phi0, mask, theta, grid, options, nsteps, a0, b, target_sw,
+                                      dt=dt, gradient=False).loss)
+    return theta, losses
 
 
 def _torch():

@@ -8,7 +8,8 @@ field (and along b), for each spatial scheme and Runge-Kutta order, to
 a switch of the scheme are detected and left out, at most 4 of 16); the
 FP32 library's gradient against the FP64 one; the result against the
 checkpoint interval; and the torch.autograd wrapper against the raw entry
-point, with a fit of a uniform contact angle to a target saturation.  The
+point, with a fit of a uniform contact angle to a target saturation and a
+fit of the full-size per-voxel angle field.  The
 forward map is the variable-coefficient inner step of the reference's
 constCurvModel3d (src/level_set_evolution/constCurvModel3d.c:127-230) with
 setVarNorm3dTheta's per-voxel angle
@@ -243,6 +244,37 @@ def test_contact_angle_field_128():
     assert rel <= 2e-3
     assert derr <= FD_TOL
     assert abs(r32.dloss_db - r64.dloss_db) <= 1e-3 * abs(r64.dloss_db)
+
+
+@pytest.mark.gpu
+def test_fit_of_a_contact_angle_field_128():
+    """The full-size case fitted: the per-voxel angle field (128^3 pack, 40
+    deg + Gaussian random field, FP32, 200 WENO5 TVD-RK3 steps, checkpoints
+    every 20) is moved by Gauss-Newton steps on the whole field
+    (fit_angle_field) to the saturation another field gives (the same field
+    8 deg wider, a target the map reaches); the loss falls by 10^3 or more
+    within six reverse passes and the fitted field stays in [0, 90] deg."""
+    from lsmpqs_amd.adjoint import fit_angle_field
+    from lsmpqs_amd.inputs import contact_angle_case
+    steps, b, a0, ck = 200, 0.04, 0.15 * 0.04, 20
+    print("case: fit of a 128^3 angle field, fp32 WENO5 RK3 200 steps", flush=True)
+    g, mask, phi0, theta = contact_angle_case(128, dtype=np.float32)
+    o = default_options(np.float32)
+    o.order_space_accur, o.order_time_accur = 5, 3
+    r0 = saturation_loss(phi0, mask, theta, g, o, steps, a0, b, 0.45, checkpoint_every=ck,
+                         gradient=False)
+    shifted = (theta + np.float32(np.radians(8.0))).astype(np.float32)
+    target = saturation_loss(phi0, mask, shifted, g, o, steps, a0, b, 0.45, dt=r0.dt,
+                             gradient=False).sw
+    fitted, losses = fit_angle_field(phi0, mask, theta, g, o, steps, a0, b, target, r0.dt,
+                                     iterations=6, checkpoint_every=ck)
+    f = fill_of(g)
+    moved = np.degrees(np.abs(fitted[f].astype(np.float64) - theta[f]))
+    print(f"S_w start {r0.sw:.6f}, target {target:.6f}; losses {['%.3e' % x for x in losses]}; "
+          f"angle change max {moved.max():.2f} deg, rms {np.sqrt(np.mean(moved ** 2)):.3f} deg, "
+          f"{int((moved > 0.01).sum())} voxels moved by more than 0.01 deg", flush=True)
+    assert losses[-1] <= 1e-3 * losses[0]
+    assert fitted.min() >= 0.0 and fitted.max() <= np.float32(0.5 * np.pi)
 
 
 def _theta_fixture():
