@@ -192,35 +192,31 @@ __device__ __forceinline__ void line_derivs(const T* l, real h, real inv_axis, r
   }
 }
 
-// adj[line voxel o] += lam (fp d plus/d phi_o + fm d minus/d phi_o) along one
-// axis line (stride st through voxel idx)
+// w[o] = fp d plus/d phi_o + fm d minus/d phi_o along one axis line (stride
+// st through voxel idx), o = 0..6 for phi(-3..+3); zero when fp = fm = 0
 template <int S, bool FW>
-__device__ __forceinline__ void scatter_axis(const real* phi, real* adj, int64_t idx, int64_t st,
-                                             real h, real inv_axis, real inv_x, real fp, real fm,
-                                             real lam) {
+__device__ __forceinline__ void axis_weights(const real* phi, int64_t idx, int64_t st, real h,
+                                             real inv_axis, real inv_x, real fp, real fm,
+                                             real* w) {
+#pragma unroll
+  for (int o = 0; o < 7; ++o) w[o] = real(0);
   if (fp == real(0) && fm == real(0)) return;
   if constexpr (FW && S == WENO5) {
     // reverse mode through the two face reconstructions
-    real l[7], dl[7] = {0, 0, 0, 0, 0, 0, 0};
+    real l[7];
 #pragma unroll
     for (int o = 0; o < 7; ++o) l[o] = phi[idx + (o - 3) * st];
     const FaceScale<real> sc = face_scale(h);
-    weno5_face_adj(l, sc, fm, real(0), dl);      // phi^- from phi(-3..+2)
-    weno5_face_adj(l + 1, sc, real(0), fp, dl + 1);  // phi^+ from phi(-2..+3)
-#pragma unroll
-    for (int o = 0; o < 7; ++o)
-      if (dl[o] != real(0)) atomicAdd(&adj[idx + (o - 3) * st], lam * dl[o]);
+    weno5_face_adj(l, sc, fm, real(0), w);      // phi^- from phi(-3..+2)
+    weno5_face_adj(l + 1, sc, real(0), fp, w + 1);  // phi^+ from phi(-2..+3)
     return;
   }
   if constexpr (!FW && S == WENO5) {
     // reverse mode through the reference-order reconstruction
-    real l[7], dl[7] = {0, 0, 0, 0, 0, 0, 0};
+    real l[7];
 #pragma unroll
     for (int o = 0; o < 7; ++o) l[o] = phi[idx + (o - 3) * st];
-    weno5_line_adj(l, inv_axis, fp, fm, dl);
-#pragma unroll
-    for (int o = 0; o < 7; ++o)
-      if (dl[o] != real(0)) atomicAdd(&adj[idx + (o - 3) * st], lam * dl[o]);
+    weno5_line_adj(l, inv_axis, fp, fm, w);
     return;
   }
   Dual<7> l[7];
@@ -232,26 +228,22 @@ __device__ __forceinline__ void scatter_axis(const real* phi, real* adj, int64_t
   Dual<7> plus, minus;
   line_derivs<Dual<7>, S, FW>(l, h, inv_axis, inv_x, plus, minus);
 #pragma unroll
-  for (int o = 0; o < 7; ++o) {
-    const real w = fp * plus.d[o] + fm * minus.d[o];
-    if (w != real(0)) atomicAdd(&adj[idx + (o - 3) * st], lam * w);
-  }
+  for (int o = 0; o < 7; ++o) w[o] = fp * plus.d[o] + fm * minus.d[o];
 }
 
+// One thread per fill voxel with lam != 0, x rows per wave (for_rows).  The
+// weights of the three axis lines and of the curvature stencil are summed
+// per target voxel before the atomic adds: 31 targets (the x line, the y and
+// z lines without the centre, the 12 cross points) instead of 21 + 25.
 template <int S, bool FW>
 __global__ __launch_bounds__(256) void adj_rhs_kernel(AdjRhsArgs a) {
   const Box bx = a.bx;
-  FillIter it(bx);
   const real fx = real(0.5) / a.dx, fy = real(0.5) / a.dy, fz = real(0.5) / a.dz;
   const real ix = real(1.0) / a.dx, iy = real(1.0) / a.dy, iz = real(1.0) / a.dz;
   const int64_t sy = bx.nx, sz = (int64_t)bx.nx * bx.ny;
-  for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < it.n;
-       f += (int64_t)gridDim.x * blockDim.x) {
-    int i, j, k;
-    it.ijk(f, i, j, k);
-    const int64_t idx = bx.at(i, j, k);
+  for_rows<true>(bx, [&](int64_t idx, int i, int j, int k) {
     const real lam = a.lam[idx] * a.coef;
-    if (lam == real(0)) continue;
+    if (lam == real(0)) return;
     // the six one-sided derivatives, as the stage kernel forms them
     real lx[7], ly[7], lz[7], d[6];
 #pragma unroll
@@ -288,17 +280,29 @@ __global__ __launch_bounds__(256) void adj_rhs_kernel(AdjRhsArgs a) {
     const real curv = curv_partials(cv, fx, fy, fz, cw);
     a.g_vn[idx] += lam * fvn;
     a.g_b[idx] += lam * curv;
+    // one-sided derivatives, one axis line at a time, then the curvature
+    // stencil's centre and axis points onto the lines (kCurvOff order)
+    real wx[7], wy[7], wz[7];
+    axis_weights<S, FW>(a.phi, idx, 1, a.dx, ix, ix, fd[0], fd[1], wx);
+    axis_weights<S, FW>(a.phi, idx, sy, a.dy, iy, ix, fd[2], fd[3], wy);
+    axis_weights<S, FW>(a.phi, idx, sz, a.dz, iz, ix, fd[4], fd[5], wz);
+    wx[3] += wy[3] + wz[3] + bb * cw[0];
+    wx[1] += bb * cw[1]; wx[2] += bb * cw[2]; wx[4] += bb * cw[3]; wx[5] += bb * cw[4];
+    wy[1] += bb * cw[5]; wy[2] += bb * cw[6]; wy[4] += bb * cw[7]; wy[5] += bb * cw[8];
+    wz[1] += bb * cw[9]; wz[2] += bb * cw[10]; wz[4] += bb * cw[11]; wz[5] += bb * cw[12];
 #pragma unroll
-    for (int q = 0; q < 25; ++q) {
+    for (int o = 0; o < 7; ++o) {
+      if (wx[o] != real(0)) atomicAdd(&a.adj[idx + (o - 3)], lam * wx[o]);
+      if (o != 3 && wy[o] != real(0)) atomicAdd(&a.adj[idx + (o - 3) * sy], lam * wy[o]);
+      if (o != 3 && wz[o] != real(0)) atomicAdd(&a.adj[idx + (o - 3) * sz], lam * wz[o]);
+    }
+#pragma unroll
+    for (int q = 13; q < 25; ++q) {
       const real w = bb * cw[q];
       if (w != real(0))
         atomicAdd(&a.adj[bx.at(i + kCurvOff[q][0], j + kCurvOff[q][1], k + kCurvOff[q][2])], lam * w);
     }
-    // one-sided derivatives, one axis line at a time
-    scatter_axis<S, FW>(a.phi, a.adj, idx, 1, a.dx, ix, ix, fd[0], fd[1], lam);
-    scatter_axis<S, FW>(a.phi, a.adj, idx, sy, a.dy, iy, ix, fd[2], fd[3], lam);
-    scatter_axis<S, FW>(a.phi, a.adj, idx, sz, a.dz, iz, ix, fd[4], fd[5], lam);
-  }
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -559,12 +563,13 @@ struct Adjoint {
     r.g_vn = g_vn; r.g_b = g_b;
     r.vn = s->vn; r.b = s->b; r.vx = s->vx; r.vy = s->vy; r.vz = s->vz;
     r.dx = c.dx; r.dy = c.dy; r.dz = c.dz; r.norm_sign = c.norm_sign;
+    const dim3 g(row_blocks<true>(s->bx, 16384)), b(ROW_TX, ROW_TY);  // for_rows' launch shape
     switch (c.scheme) {
-    case 2: hipLaunchKernelGGL((adj_rhs_kernel<ENO2, false>), dim3(kBlocks), dim3(256), 0, s->st, r); break;
-    case 3: hipLaunchKernelGGL((adj_rhs_kernel<ENO3, false>), dim3(kBlocks), dim3(256), 0, s->st, r); break;
+    case 2: hipLaunchKernelGGL((adj_rhs_kernel<ENO2, false>), g, b, 0, s->st, r); break;
+    case 3: hipLaunchKernelGGL((adj_rhs_kernel<ENO3, false>), g, b, 0, s->st, r); break;
     default:
-      if (kFast) hipLaunchKernelGGL((adj_rhs_kernel<WENO5, true>), dim3(kBlocks), dim3(256), 0, s->st, r);
-      else hipLaunchKernelGGL((adj_rhs_kernel<WENO5, false>), dim3(kBlocks), dim3(256), 0, s->st, r);
+      if (kFast) hipLaunchKernelGGL((adj_rhs_kernel<WENO5, true>), g, b, 0, s->st, r);
+      else hipLaunchKernelGGL((adj_rhs_kernel<WENO5, false>), g, b, 0, s->st, r);
       break;
     }
     ACHECK(hipGetLastError());
