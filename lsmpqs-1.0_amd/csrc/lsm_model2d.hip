@@ -88,6 +88,7 @@ struct Rhs2Args {
   real dx, dy;
   const real* dt;
   int nslabs;
+  const int* stale_src;  // VAR: the voxel whose normal term each voxel reuses, or -1 (stale_src2d)
 };
 
 template <int S>
@@ -133,25 +134,58 @@ __device__ __forceinline__ void slab_of(const Box2& b, int nslabs, int j, int& l
   hi = b.jlo + (int)((int64_t)nsl * (t + 1) / nslabs) - 1;
 }
 
+// Where the reference leaves temp_norm unassigned (|vel_n| < 1e-5) a voxel
+// takes the normal term of the last voxel of its j-slab, in loop order (j,
+// then i), that assigned it (qss_spatial_derivatives2d.f); stale_src2d lists
+// that voxel for every fill voxel once per change of vel_n, in O(n).
 template <int S, bool VAR>
 __device__ real stale_norm2d(const Rhs2Args& a, int i, int j) {
   if (!VAR) return real(0);  // a scalar |a| < 1e-5 leaves every voxel unassigned
   const Box2& b = a.b;
-  int lo, hi;
-  slab_of(b, a.nslabs, j, lo, hi);
+  const int src = a.stale_src[b.at(i, j)];
+  if (src < 0) return real(0);
+  const int jj = src / b.nx, ii = src - jj * b.nx;
   const real inv_dx = real(1.0) / a.dx, inv_dy = real(1.0) / a.dy;
-  int ii = i, jj = j;
-  while (true) {
-    if (ii > b.ilo) --ii;
-    else if (jj > lo) { --jj; ii = b.ihi; }
-    else return real(0);
-    const real v = VAR ? a.vn[b.at(ii, jj)] : a.a_const;
-    if (fabs(v) >= real(1.e-5)) {
-      real d[4];
-      derivs2d<S>(a.in, b, ii, jj, inv_dx, inv_dy, d);
-      return norm2d(v, d);
+  real d[4];
+  derivs2d<S>(a.in, b, ii, jj, inv_dx, inv_dy, d);
+  return norm2d(a.vn[src], d);
+}
+
+// last voxel of each fill row with |vel_n| >= 1e-5, or -1 (one thread per row)
+__global__ void stale_rows2d_kernel(Box2 b, const real* vn, int* row_last) {
+  const int j = b.jlo + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (j > b.jhi) return;
+  int last = -1;
+  for (int i = b.ilo; i <= b.ihi; ++i)
+    if (fabs(vn[b.at(i, j)]) >= real(1.e-5)) last = b.at(i, j);
+  row_last[j - b.jlo] = last;
+}
+// src[v]: the last voxel before v in its slab's loop order with |vel_n| >=
+// 1e-5 (-1 if none, and for those voxels themselves); one thread per row
+__global__ void stale_src2d_kernel(Box2 b, const real* vn, int nslabs, const int* row_last,
+                                   int* src) {
+  const int j = b.jlo + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (j > b.jhi) return;
+  int lo, hi;
+  slab_of(b, nslabs, j, lo, hi);
+  int carry = -1;
+  for (int jj = j - 1; jj >= lo && carry < 0; --jj) carry = row_last[jj - b.jlo];
+  for (int i = b.ilo; i <= b.ihi; ++i) {
+    const int idx = b.at(i, j);
+    if (fabs(vn[idx]) >= real(1.e-5)) {
+      src[idx] = -1;
+      carry = idx;
+    } else {
+      src[idx] = carry;
     }
   }
+}
+void launch_stale_src2d(const Box2& b, const real* vn, int nslabs, int* row_last, int* src,
+                        hipStream_t st) {
+  const int rows = b.jhi - b.jlo + 1;
+  const dim3 g((rows + 63) / 64), t(64);
+  hipLaunchKernelGGL(stale_rows2d_kernel, g, t, 0, st, b, vn, row_last);
+  hipLaunchKernelGGL(stale_src2d_kernel, g, t, 0, st, b, vn, nslabs, (const int*)row_last, src);
 }
 
 template <int S, bool VAR, int STAGE>
@@ -454,6 +488,7 @@ struct Run2 {
   double *part = nullptr, *red = nullptr;
   uint8_t* bin = nullptr;
   int *labels = nullptr, *troot = nullptr, *roots = nullptr;
+  int *stale_src = nullptr, *stale_rows = nullptr;  // stale_src2d of vn
   int64_t roots_cap = 0;
   real dx, dy;
   int nslabs = 1;
@@ -487,6 +522,13 @@ struct Run2 {
     troot = alloc<int>(n);
     roots_cap = n / 16 + 1;
     roots = alloc<int>(roots_cap + 1);
+    stale_src = alloc<int>(n);
+    stale_rows = alloc<int>(b.jhi - b.jlo + 1);
+  }
+  // the normal-term sources of the voxels with |vn| < 1e-5, after vn changed
+  void stale() {
+    launch_stale_src2d(b, vn, nslabs, stale_rows, stale_src, st);
+    X2CHECK(hipGetLastError());
   }
   ~Run2() {
     for (void* p : owned) (void)hipFree(p);
@@ -611,6 +653,8 @@ real ccm2(Run2& r, Options* o, const Grid* g, FILE* fp, bool var, real a_const) 
   Rhs2Args a{};
   a.b = r.b; a.vn = r.vn; a.bb = r.bb; a.vx = r.vx; a.vy = r.vy;
   a.a_const = a_const; a.b_const = o->b; a.dx = r.dx; a.dy = r.dy; a.nslabs = r.nslabs;
+  a.stale_src = r.stale_src;
+  if (var) r.stale();
   while ((t < o->tmax) && (max_abs_err > o->eps_stop) &&
          (var || ((vol_phi > vsmall) && (satn < max_satn)))) {
     X2CHECK(hipMemcpyAsync(r.prev, r.phi, r.n * sizeof(real), hipMemcpyDeviceToDevice, r.st));
@@ -643,6 +687,7 @@ real ccm2(Run2& r, Options* o, const Grid* g, FILE* fp, bool var, real a_const) 
         hipLaunchKernelGGL(trapvel2d_kernel, dim3(blocks2(r.b.nfill())), dim3(256), 0, r.st, r.b,
                            r.mask_w, r.mask_nw, r.vn, r.bb, r.vx, r.vy, g->dx[0]);
         X2CHECK(hipGetLastError());
+        r.stale();
       }
       X2CHECK(launch_merge(r.phi, r.mask_nw, r.n, r.st));
     }
@@ -844,6 +889,12 @@ struct Dev2 {
     if (h) X2CHECK(hipMemcpy(d, h, (size_t)n * sizeof(real), hipMemcpyHostToDevice));
     return (real*)d;
   }
+  int* ints(int64_t n) {
+    void* d = nullptr;
+    X2CHECK(hipMalloc(&d, (size_t)(n > 0 ? n : 1) * sizeof(int)));
+    p.push_back(d);
+    return (int*)d;
+  }
   static void get(real* h, const real* d, int64_t n) {
     X2CHECK(hipDeviceSynchronize());
     X2CHECK(hipMemcpy(h, d, (size_t)n * sizeof(real), hipMemcpyDeviceToHost));
@@ -870,6 +921,12 @@ void rhs2d_call(const real* phi, const real* vn, const real* bb, const real* vx,
   a.dx = dx;
   a.dy = dy;
   a.nslabs = 1;
+  if (VAR) {
+    int* src = dv.ints(n);
+    launch_stale_src2d(b, a.vn, 1, dv.ints(b.jhi - b.jlo + 1), src, 0);
+    X2CHECK(hipGetLastError());
+    a.stale_src = src;
+  }
   a.out = dv.put(lse_rhs, n);
   real* h = dv.put(nullptr, 1);
   hipLaunchKernelGGL((rhs2d_kernel<S, VAR, 0>), dim3(blocks2(b.nfill())), dim3(256), 0, 0, a);
