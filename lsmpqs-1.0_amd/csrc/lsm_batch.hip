@@ -265,6 +265,8 @@ class Batch final : public BatchHook {
       if (p) (void)hipHostFree(p);
     for (hipEvent_t e : pass_ev_)
       if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : inner_ev_)
+      if (e) (void)hipEventDestroy(e);
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
     if (st_) (void)hipStreamDestroy(st_);
@@ -295,15 +297,24 @@ class Batch final : public BatchHook {
     BCHECK(hipMemcpyAsync(d_ghost_, h_ghost_, sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, st_));
   }
 
-  void enqueue_step() {
+  // timed: events between the parts (host-time tracing; not captured)
+  void enqueue_step(bool timed = false) {
+    if (timed) mark_inner(0);
     BCHECK(launch_batch_reset(d_counters_, kCounters, st_));
     BCHECK(launch_probe_batch(view_, scheme_, nmax_, st_));
     BCHECK(hipMemcpyAsync(h_dts_, d_dts_, sizeof(real) * nmax_, hipMemcpyDeviceToHost, st_));
     for (int k = 0; k < nstage_; ++k) {
       const bool fin = k == nstage_ - 1;
+      if (timed) mark_inner(1 + 2 * k);
       BCHECK(launch_stage_batch(view_, k, scheme_, var_, k + 1, fin, fast_, grid_, st_));
+      if (timed) mark_inner(2 + 2 * k);
       BCHECK(launch_bc_batch(view_, k, fin, bc_grid_, st_));
     }
+    if (timed) mark_inner(1 + 2 * nstage_);
+  }
+  void mark_inner(int q) {
+    if (!inner_ev_[q]) BCHECK(hipEventCreate(&inner_ev_[q]));
+    BCHECK(hipEventRecord(inner_ev_[q], st_));
   }
 
   // brick depth and workgroup count: fastest first stage over this round
@@ -480,9 +491,11 @@ class Batch final : public BatchHook {
 
  public:
   double pass_ms[4] = {0, 0, 0, 0};  // trapping, mask reinitializations, merge, phi reinitialization
+  double inner_ms[7] = {0, 0, 0, 0, 0, 0, 0};  // probe, then per stage: stage kernel, ghost extrapolation
 
  private:
   hipEvent_t pass_ev_[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t inner_ev_[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
 
   void launch_inner(const std::vector<Request*>& round) {
     if (!configured_) {
@@ -497,12 +510,20 @@ class Batch final : public BatchHook {
     const int zc = round_depth(round);
     upload(round, zc);
     if (!exec_) capture();
+    const bool tr = trace_on();
     BCHECK(hipEventRecord(ev0_, st_));
-    BCHECK(hipGraphLaunch(exec_, st_));
+    if (tr) enqueue_step(true);  // the same launches, uncaptured, with events between them
+    else BCHECK(hipGraphLaunch(exec_, st_));
     BCHECK(hipEventRecord(ev1_, st_));
     BCHECK(hipStreamSynchronize(st_));
     float ms = 0;
     BCHECK(hipEventElapsedTime(&ms, ev0_, ev1_));
+    if (tr)
+      for (int q = 0; q < 1 + 2 * nstage_; ++q) {
+        float m = 0;
+        BCHECK(hipEventElapsedTime(&m, inner_ev_[q], inner_ev_[q + 1]));
+        inner_ms[q] += m;
+      }
     stats.batched_steps += 1;
     stats.batched_ms += ms;
     stats.box_steps += (long long)round.size();
@@ -637,6 +658,12 @@ extern "C" int lsmpqs_batch_model3d(int n, Options** options, QSS_DataArrays** p
             "%.3f, merge and floor %.3f, phi reinitialization and reductions %.3f s\n",
             batch.pass_ms[0] * 1e-3, batch.pass_ms[1] * 1e-3, batch.pass_ms[2] * 1e-3,
             batch.pass_ms[3] * 1e-3);
+    fprintf(stderr,
+            "lsmpqs batch trace: batched inner steps (uncaptured): probe %.3f, stages %.3f / "
+            "%.3f / %.3f, ghost extrapolations %.3f / %.3f / %.3f s\n",
+            batch.inner_ms[0] * 1e-3, batch.inner_ms[1] * 1e-3, batch.inner_ms[3] * 1e-3,
+            batch.inner_ms[5] * 1e-3, batch.inner_ms[2] * 1e-3, batch.inner_ms[4] * 1e-3,
+            batch.inner_ms[6] * 1e-3);
   }
   if (stats) {
     *stats = batch.stats;
