@@ -11,9 +11,11 @@ namespace lsm {
 // dq[0..5] += sm * d(minus_right)/dq + sp * d(plus_left)/dq for the window
 // q[0..5] = phi(i-2..i+3), with the forward's discrete choices (flat-window
 // test, the largest-difference operand) taken as the forward takes them.
-// The forward is restated in double-free working precision; rounding
-// differences from the forward's contracted arithmetic do not matter to a
-// derivative.
+// The forward is restated in working precision; rounding differences from
+// the forward's contracted arithmetic do not matter to a derivative, so
+// divisions by constants are products with their reciprocals and the three
+// divisions by data are reciprocals (fast_rcp: the hardware reciprocal in
+// single precision).
 template <typename T>
 __host__ __device__ inline void weno5_face_adj(const T* q, const FaceScale<T>& sc, T sm, T sp,
                                                T* dq) {
@@ -30,6 +32,7 @@ __host__ __device__ inline void weno5_face_adj(const T* q, const FaceScale<T>& s
   int j03 = a01 >= a23 ? j01 : j23;
   const T m = a03 >= fabs(r[4]) ? a03 : fabs(r[4]);
   const int jm = a03 >= fabs(r[4]) ? j03 : 4;
+  const T i6 = T(1.0 / 6.0), i3 = T(1.0 / 3.0), i20 = T(1.0 / 20.0), i30 = T(1.0 / 30.0);
   if (m > sc.thr) {
     const T D1 = r[1] - r[0], D2 = r[2] - r[1], D3 = r[3] - r[2], D4 = r[4] - r[3];
     const T t1 = D2 - D1, t2 = D3 - D2, t3 = D4 - D3;
@@ -38,7 +41,7 @@ __host__ __device__ inline void weno5_face_adj(const T* q, const FaceScale<T>& s
     const T S1 = c13 * (t1 * t1) + T(0.25) * (u1 * u1);
     const T S2 = c13 * (t2 * t2) + T(0.25) * (u2 * u2);
     const T S3 = c13 * (t3 * t3) + T(0.25) * (u3 * u3);
-    const T im = T(1) / m, im2 = im * im, ce = sc.c1 * im;
+    const T im = fast_rcp(m), im2 = im * im, ce = sc.c1 * im;
     const T eps = T(1e-6) + ce * ce;
     const T a1 = S1 * im2 + eps, a2 = S2 * im2 + eps, a3 = S3 * im2 + eps;
     const T Q1 = a1 * a1, Q2 = a2 * a2, Q3 = a3 * a3;
@@ -48,14 +51,14 @@ __host__ __device__ inline void weno5_face_adj(const T* q, const FaceScale<T>& s
     const T mid = T(0.6) * q13;
     const T sum_m = T(0.1) * q23 + T(0.3) * q12 + mid;
     const T sum_p = T(0.1) * q12 + T(0.3) * q23 + mid;
-    const T num_m = U / T(30) + V / T(20);
-    const T num_p = -(U / T(20) + V / T(30));
-    const T rm = T(1) / sum_m, rp = T(1) / sum_p;
+    const T num_m = U * i30 + V * i20;
+    const T num_p = -(U * i20 + V * i30);
+    const T rm = fast_rcp(sum_m), rp = fast_rcp(sum_p);
     // outputs: (pm2 + num_m rm) inv, (pp2 + num_p rp) inv
     const T dnum_m = dA_m * rm, dsum_m = -dA_m * num_m * rm * rm;
     const T dnum_p = dA_p * rp, dsum_p = -dA_p * num_p * rp * rp;
-    const T dU = dnum_m / T(30) - dnum_p / T(20);
-    const T dV = dnum_m / T(20) - dnum_p / T(30);
+    const T dU = dnum_m * i30 - dnum_p * i20;
+    const T dV = dnum_m * i20 - dnum_p * i30;
     T dq23 = T(0.1) * dsum_m + T(0.3) * dsum_p + dU * g1;
     T dq12 = T(0.3) * dsum_m + T(0.1) * dsum_p + dV * g2;
     const T dq13 = T(0.6) * (dsum_m + dsum_p);
@@ -82,12 +85,14 @@ __host__ __device__ inline void weno5_face_adj(const T* q, const FaceScale<T>& s
     dr[2] += dD2 - dD3;
     dr[3] += dD3 - dD4;
     dr[4] += dD4;
-    dr[jm] += dm * (r[jm] < T(0) ? T(-1) : T(1));
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+      if (k == jm) dr[k] += dm * (r[k] < T(0) ? T(-1) : T(1));
   }
   // middle candidates: pm2 = -r1/6 + 5 r2/6 + r3/3, pp2 = -r3/6 + 5 r2/6 + r1/3
-  dr[1] += -dpm2 / T(6) + dpp2 / T(3);
+  dr[1] += -dpm2 * i6 + dpp2 * i3;
   dr[2] += T(5) / T(6) * (dpm2 + dpp2);
-  dr[3] += dpm2 / T(3) - dpp2 / T(6);
+  dr[3] += dpm2 * i3 - dpp2 * i6;
   dq[0] -= dr[0];
   dq[1] += dr[0] - dr[1];
   dq[2] += dr[1] - dr[2];
@@ -120,15 +125,18 @@ __host__ __device__ inline void weno5_combine_adj(const T* v, T s, T* dv) {
   const T S1 = c13 * (t1 * t1) + T(0.25) * (u1 * u1);
   const T S2 = c13 * (t2 * t2) + T(0.25) * (u2 * u2);
   const T S3 = c13 * (t3 * t3) + T(0.25) * (u3 * u3);
+  // four reciprocals instead of the thirteen divisions of the formulas below
   const T e1 = S1 + eps, e2 = S2 + eps, e3 = S3 + eps;
-  const T a1 = T(0.1) / (e1 * e1), a2 = T(0.6) / (e2 * e2), a3 = T(0.3) / (e3 * e3);
+  const T ie1 = T(1) / e1, ie2 = T(1) / e2, ie3 = T(1) / e3;
+  const T a1 = T(0.1) * (ie1 * ie1), a2 = T(0.6) * (ie2 * ie2), a3 = T(0.3) * (ie3 * ie3);
   const T A = a1 + a2 + a3;
-  const T out = (a1 * p1 + a2 * p2 + a3 * p3) / A;
+  const T iA = T(1) / A, sA = s * iA;
+  const T out = (a1 * p1 + a2 * p2 + a3 * p3) * iA;
   // out = sum a_k p_k / A: d/dp_k = a_k / A, d/da_k = (p_k - out) / A
-  const T dp1 = s * a1 / A, dp2 = s * a2 / A, dp3 = s * a3 / A;
-  const T da1 = s * (p1 - out) / A, da2 = s * (p2 - out) / A, da3 = s * (p3 - out) / A;
+  const T dp1 = sA * a1, dp2 = sA * a2, dp3 = sA * a3;
+  const T da1 = sA * (p1 - out), da2 = sA * (p2 - out), da3 = sA * (p3 - out);
   // a_k = c_k / e_k^2: d/de_k = -2 a_k / e_k
-  const T de1 = -T(2) * da1 * a1 / e1, de2 = -T(2) * da2 * a2 / e2, de3 = -T(2) * da3 * a3 / e3;
+  const T de1 = -T(2) * da1 * a1 * ie1, de2 = -T(2) * da2 * a2 * ie2, de3 = -T(2) * da3 * a3 * ie3;
   const T deps = de1 + de2 + de3;
   const T dt1 = T(2) * c13 * t1 * de1, du1 = T(0.5) * u1 * de1;
   const T dt2 = T(2) * c13 * t2 * de2, du2 = T(0.5) * u2 * de2;
@@ -139,7 +147,9 @@ __host__ __device__ inline void weno5_combine_adj(const T* v, T s, T* dv) {
            T(2) * dt2 + dt3 + T(3) * du3;
   dv[3] += T(1.0 / 3.0) * dp2 + T(5.0 / 6.0) * dp3 + dt2 - du2 - T(2) * dt3 - T(4) * du3;
   dv[4] += -T(1.0 / 6.0) * dp3 + dt3 + du3;
-  dv[jm] += deps * T(1e-6) * T(2) * v[jm];
+#pragma unroll
+  for (int k = 0; k < 5; ++k)
+    if (k == jm) dv[k] += deps * T(1e-6) * T(2) * v[k];
 }
 
 // dp[0..6] += sp * d plus/dp + sm * d minus/dp for weno5(p, inv_d) of

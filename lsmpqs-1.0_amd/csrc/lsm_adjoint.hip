@@ -231,78 +231,128 @@ __device__ __forceinline__ void axis_weights(const real* phi, int64_t idx, int64
   for (int o = 0; o < 7; ++o) w[o] = fp * plus.d[o] + fm * minus.d[o];
 }
 
-// One thread per fill voxel with lam != 0, x rows per wave (for_rows).  The
-// weights of the three axis lines and of the curvature stencil are summed
-// per target voxel before the atomic adds: 31 targets (the x line, the y and
-// z lines without the centre, the 12 cross points) instead of 21 + 25.
+// d/dd_q of the Godunov |grad phi| = sqrt(godunov_sq(vn, d)) (lsm_device.h),
+// with the forward's selections: per axis x = max(s d+, -s d-, 0) with
+// s = sign(vel_n), the first operand on ties, so |grad phi| = sqrt(x^2 + y^2
+// + z^2) depends on one of the two one-sided derivatives of an axis (or
+// neither), with derivative +-s x / |grad phi|.  Returns |grad phi|.
+__device__ __forceinline__ real godunov_partials(real vn, const real* d, real* g) {
+  const real s = vn > real(0) ? real(1) : real(-1);
+  real x[3];
+  int pick[3];  // 1: d[2a+1] selected, 0: d[2a], -1: neither
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const real hi = s * d[2 * a + 1], lo = -(s * d[2 * a]);
+    const bool first = hi >= lo;
+    const real v = first ? hi : lo;
+    x[a] = v >= real(0) ? v : real(0);
+    pick[a] = v >= real(0) ? (first ? 1 : 0) : -1;
+  }
+  const real g2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+  const real gn = sqrt(g2);
+  const real f = g2 > real(0) ? real(1) / gn : real(0);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const real w = s * x[a] * f;
+    g[2 * a] = pick[a] == 0 ? -w : real(0);
+    g[2 * a + 1] = pick[a] == 1 ? w : real(0);
+  }
+  return gn;
+}
+
+// Transposed right-hand side, one thread per fill voxel with lam != 0 (x
+// fastest, blocks of 256; in a loop over several voxels per thread the
+// compiler overlapped consecutive voxels' loads and needed 172 VGPRs, two
+// waves per SIMD, instead of 78, six).  The curvature stencil's axis points
+// are added to the axis lines' weights and the centre's contributions are
+// summed before the atomic adds: 31 targets (6 per axis line, the centre,
+// the 12 cross points) instead of 21 + 25.  Each axis line's weights are
+// added as soon as they are formed.
 template <int S, bool FW>
 __global__ __launch_bounds__(256) void adj_rhs_kernel(AdjRhsArgs a) {
   const Box bx = a.bx;
-  const real fx = real(0.5) / a.dx, fy = real(0.5) / a.dy, fz = real(0.5) / a.dz;
+  const int nxf = bx.ihi - bx.ilo + 1, nyf = bx.jhi - bx.jlo + 1;
+  const int64_t f = (int64_t)blockIdx.x * 256 + threadIdx.y * ROW_TX + threadIdx.x;
+  if (f >= (int64_t)nxf * nyf * (bx.khi - bx.klo + 1)) return;
+  const int64_t row = f / nxf;
+  const int64_t idx = bx.at(bx.ilo + (int)(f - row * nxf), bx.jlo + (int)(row % nyf),
+                            bx.klo + (int)(row / nyf));
   const real ix = real(1.0) / a.dx, iy = real(1.0) / a.dy, iz = real(1.0) / a.dz;
   const int64_t sy = bx.nx, sz = (int64_t)bx.nx * bx.ny;
-  for_rows<true>(bx, [&](int64_t idx, int i, int j, int k) {
+  {
     const real lam = a.lam[idx] * a.coef;
     if (lam == real(0)) return;
     // the six one-sided derivatives, as the stage kernel forms them
-    real lx[7], ly[7], lz[7], d[6];
+    real d[6];
+    {
+      real v[7];
 #pragma unroll
-    for (int o = 0; o < 7; ++o) {
-      lx[o] = a.phi[idx + (o - 3)];
-      ly[o] = a.phi[idx + (o - 3) * sy];
-      lz[o] = a.phi[idx + (o - 3) * sz];
+      for (int o = 0; o < 7; ++o) v[o] = a.phi[idx + (o - 3)];
+      line_derivs<real, S, FW>(v, a.dx, ix, ix, d[0], d[1]);
+#pragma unroll
+      for (int o = 0; o < 7; ++o) v[o] = a.phi[idx + (o - 3) * sy];
+      line_derivs<real, S, FW>(v, a.dy, iy, ix, d[2], d[3]);
+#pragma unroll
+      for (int o = 0; o < 7; ++o) v[o] = a.phi[idx + (o - 3) * sz];
+      line_derivs<real, S, FW>(v, a.dz, iz, ix, d[4], d[5]);
     }
-    line_derivs<real, S, FW>(lx, a.dx, ix, ix, d[0], d[1]);
-    line_derivs<real, S, FW>(ly, a.dy, iy, ix, d[2], d[3]);
-    line_derivs<real, S, FW>(lz, a.dz, iz, ix, d[4], d[5]);
     // partials of the right-hand side w.r.t. them and vel_n
-    const real vn = a.vn[idx], bb = a.b[idx];
+    const real vn = a.vn[idx];
     const real vxv = a.vx[idx], vyv = a.vy[idx], vzv = a.vz[idx];
     real fd[6] = {0, 0, 0, 0, 0, 0};
     real fvn = 0;
     if (fabs(vn) >= real(1.e-5)) {
-      Dual<6> D[6];
+      real g[6];
+      const real gn = godunov_partials(vn, d, g);
 #pragma unroll
-      for (int q = 0; q < 6; ++q) { D[q] = Dual<6>(d[q]); D[q].d[q] = 1; }
-      Dual<6> G = sqrt(godunov_sq(Dual<6>(vn), D));
-#pragma unroll
-      for (int q = 0; q < 6; ++q) fd[q] = a.norm_sign * vn * G.d[q];
-      fvn = a.norm_sign * G.v;
+      for (int q = 0; q < 6; ++q) fd[q] = a.norm_sign * vn * g[q];
+      fvn = a.norm_sign * gn;
     }
     fd[vxv >= real(0) ? 1 : 0] -= vxv;
     fd[vyv >= real(0) ? 3 : 2] -= vyv;
     fd[vzv >= real(0) ? 5 : 4] -= vzv;
-    // curvature term
-    real cv[25], cw[25];
-#pragma unroll
-    for (int q = 0; q < 25; ++q)
-      cv[q] = a.phi[bx.at(i + kCurvOff[q][0], j + kCurvOff[q][1], k + kCurvOff[q][2])];
-    const real curv = curv_partials(cv, fx, fy, fz, cw);
     a.g_vn[idx] += lam * fvn;
-    a.g_b[idx] += lam * curv;
-    // one-sided derivatives, one axis line at a time, then the curvature
-    // stencil's centre and axis points onto the lines (kCurvOff order)
-    real wx[7], wy[7], wz[7];
-    axis_weights<S, FW>(a.phi, idx, 1, a.dx, ix, ix, fd[0], fd[1], wx);
-    axis_weights<S, FW>(a.phi, idx, sy, a.dy, iy, ix, fd[2], fd[3], wy);
-    axis_weights<S, FW>(a.phi, idx, sz, a.dz, iz, ix, fd[4], fd[5], wz);
-    wx[3] += wy[3] + wz[3] + bb * cw[0];
-    wx[1] += bb * cw[1]; wx[2] += bb * cw[2]; wx[4] += bb * cw[3]; wx[5] += bb * cw[4];
-    wy[1] += bb * cw[5]; wy[2] += bb * cw[6]; wy[4] += bb * cw[7]; wy[5] += bb * cw[8];
-    wz[1] += bb * cw[9]; wz[2] += bb * cw[10]; wz[4] += bb * cw[11]; wz[5] += bb * cw[12];
+    // curvature term: partials w.r.t. its 25 stencil values (kCurvOff order);
+    // the cross points' contributions are added here
+    const real bb = a.b[idx];
+    real cw[25];
+    {
+      const real fx = real(0.5) / a.dx, fy = real(0.5) / a.dy, fz = real(0.5) / a.dz;
+      auto off = [&](int q) {  // block-uniform offset of stencil point q from idx
+        return (int64_t)kCurvOff[q][0] + kCurvOff[q][1] * sy + kCurvOff[q][2] * sz;
+      };
+      const real* pb = a.phi + idx;
+      real cv[25];
 #pragma unroll
-    for (int o = 0; o < 7; ++o) {
-      if (wx[o] != real(0)) atomicAdd(&a.adj[idx + (o - 3)], lam * wx[o]);
-      if (o != 3 && wy[o] != real(0)) atomicAdd(&a.adj[idx + (o - 3) * sy], lam * wy[o]);
-      if (o != 3 && wz[o] != real(0)) atomicAdd(&a.adj[idx + (o - 3) * sz], lam * wz[o]);
-    }
+      for (int q = 0; q < 25; ++q) cv[q] = pb[off(q)];
+      const real curv = curv_partials(cv, fx, fy, fz, cw);
+      a.g_b[idx] += lam * curv;
 #pragma unroll
-    for (int q = 13; q < 25; ++q) {
-      const real w = bb * cw[q];
-      if (w != real(0))
-        atomicAdd(&a.adj[bx.at(i + kCurvOff[q][0], j + kCurvOff[q][1], k + kCurvOff[q][2])], lam * w);
+      for (int q = 0; q < 25; ++q) cw[q] *= bb;
+      real* ab = a.adj + idx;
+#pragma unroll
+      for (int q = 13; q < 25; ++q)
+        if (cw[q] != real(0)) atomicAdd(ab + off(q), lam * cw[q]);
     }
-  });
+    // one-sided derivatives, one axis line at a time, with the curvature
+    // stencil's points on that line (offsets -2, -1, +1, +2)
+    real cen = cw[0];
+    real w[7];
+    auto line = [&](int64_t st, int c) {
+      w[1] += cw[c]; w[2] += cw[c + 1]; w[4] += cw[c + 2]; w[5] += cw[c + 3];
+      cen += w[3];
+#pragma unroll
+      for (int o = 0; o < 7; ++o)
+        if (o != 3 && w[o] != real(0)) atomicAdd(&a.adj[idx + (o - 3) * st], lam * w[o]);
+    };
+    axis_weights<S, FW>(a.phi, idx, 1, a.dx, ix, ix, fd[0], fd[1], w);
+    line(1, 1);
+    axis_weights<S, FW>(a.phi, idx, sy, a.dy, iy, ix, fd[2], fd[3], w);
+    line(sy, 5);
+    axis_weights<S, FW>(a.phi, idx, sz, a.dz, iz, ix, fd[4], fd[5], w);
+    line(sz, 9);
+    if (cen != real(0)) atomicAdd(&a.adj[idx], lam * cen);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -503,6 +553,14 @@ struct Buffers {
     all.push_back(p);
     return p;
   }
+  // count fields of n values in one allocation (each 256-byte aligned)
+  std::vector<real*> slab(int count, int64_t n) {
+    const int64_t stride = (n + 63) / 64 * 64;
+    real* p = get(stride * count);
+    std::vector<real*> f(count);
+    for (int q = 0; q < count; ++q) f[q] = p + stride * q;
+    return f;
+  }
   ~Buffers() {
     for (real* p : all) (void)hipFree(p);
   }
@@ -563,13 +621,16 @@ struct Adjoint {
     r.g_vn = g_vn; r.g_b = g_b;
     r.vn = s->vn; r.b = s->b; r.vx = s->vx; r.vy = s->vy; r.vz = s->vz;
     r.dx = c.dx; r.dy = c.dy; r.dz = c.dz; r.norm_sign = c.norm_sign;
-    const dim3 g(row_blocks<true>(s->bx, 16384)), b(ROW_TX, ROW_TY);  // for_rows' launch shape
+    const dim3 b(ROW_TX, ROW_TY);
+    const Box& fb = s->bx;
+    const int64_t nf = (int64_t)(fb.ihi - fb.ilo + 1) * (fb.jhi - fb.jlo + 1) * (fb.khi - fb.klo + 1);
+    const dim3 gv((unsigned)((nf + 255) / 256));  // one fill voxel per thread
     switch (c.scheme) {
-    case 2: hipLaunchKernelGGL((adj_rhs_kernel<ENO2, false>), g, b, 0, s->st, r); break;
-    case 3: hipLaunchKernelGGL((adj_rhs_kernel<ENO3, false>), g, b, 0, s->st, r); break;
+    case 2: hipLaunchKernelGGL((adj_rhs_kernel<ENO2, false>), gv, b, 0, s->st, r); break;
+    case 3: hipLaunchKernelGGL((adj_rhs_kernel<ENO3, false>), gv, b, 0, s->st, r); break;
     default:
-      if (kFast) hipLaunchKernelGGL((adj_rhs_kernel<WENO5, true>), g, b, 0, s->st, r);
-      else hipLaunchKernelGGL((adj_rhs_kernel<WENO5, false>), g, b, 0, s->st, r);
+      if (kFast) hipLaunchKernelGGL((adj_rhs_kernel<WENO5, true>), gv, b, 0, s->st, r);
+      else hipLaunchKernelGGL((adj_rhs_kernel<WENO5, false>), gv, b, 0, s->st, r);
       break;
     }
     ACHECK(hipGetLastError());
@@ -584,9 +645,11 @@ struct Adjoint {
     ACHECK(hipGetLastError());
   }
 
-  // lam: adjoint of u_{n+1} in, adjoint of u_n out (u: state u_n)
-  void reverse_step(const real* u) {
-    forward_step(u, u3);  // u1, u2, u3 and the pre-imposition face layers of this step
+  // lam: adjoint of u_{n+1} in, adjoint of u_n out (u: state u_n).  The
+  // step's stage values (u1, u2, u3 and the pre-imposition face layers) are
+  // recomputed first unless the caller points those members at stored ones.
+  void reverse_step(const real* u, bool recompute = true) {
+    if (recompute) forward_step(u, u3);
     static const real alpha3[3] = {0, real(0.75), real(1.0 / 3.0)};
     static const real beta3[3] = {1, real(0.25), real(2.0 / 3.0)};
     static const real alpha2[2] = {0, real(0.75)};
@@ -634,11 +697,14 @@ extern "C" int lsmpqs_adjoint_run(Options* o, QSS_DataArrays* p, Grid* g, const 
     Buffers buf;
     Adjoint A{};
     A.s = s;
-    A.u1 = buf.get(n); A.u2 = buf.get(n); A.u3 = buf.get(n); A.pre = buf.get(n);
-    A.lam = buf.get(n); A.lamn = buf.get(n); A.lams[0] = buf.get(n); A.lams[1] = buf.get(n);
-    A.g_vn = buf.get(n); A.g_b = buf.get(n); A.dvn = buf.get(n);
-    real* d_theta = buf.get(n);
-    real* u = buf.get(n);
+    // one allocation for the fixed fields (an allocation costs about as much
+    // as a few time steps at 128^3)
+    const std::vector<real*> fx = buf.slab(13, n);
+    A.u1 = fx[0]; A.u2 = fx[1]; A.u3 = fx[2]; A.pre = fx[3];
+    A.lam = fx[4]; A.lamn = fx[5]; A.lams[0] = fx[6]; A.lams[1] = fx[7];
+    A.g_vn = fx[8]; A.g_b = fx[9]; A.dvn = fx[10];
+    real* d_theta = fx[11];
+    real* u = fx[12];
     ACHECK(hipMemcpyAsync(u, p->phi, bytes, hipMemcpyHostToDevice, s->st));
     ACHECK(hipMemcpyAsync(s->mask, p->mask, bytes, hipMemcpyHostToDevice, s->st));
     ACHECK(hipMemcpyAsync(d_theta, theta, bytes, hipMemcpyHostToDevice, s->st));
@@ -693,9 +759,13 @@ extern "C" int lsmpqs_adjoint_run(Options* o, QSS_DataArrays* p, Grid* g, const 
     const int N = args->nsteps, K = std::min(args->checkpoint_every, args->nsteps);
     const int nseg = (N + K - 1) / K;
     std::vector<real*> ck(nseg), seg(K);
-    for (int m = 0; m < nseg; ++m) ck[m] = buf.get(n);
-    for (int q = 0; q < K; ++q) seg[q] = buf.get(n);
-    real* un = buf.get(n);
+    real* un = nullptr;
+    {
+      const std::vector<real*> f = buf.slab(nseg + K + 1, n);
+      for (int m = 0; m < nseg; ++m) ck[m] = f[m];
+      for (int q = 0; q < K; ++q) seg[q] = f[nseg + q];
+      un = f[nseg + K];
+    }
     for (int step = 0; step < N; ++step) {
       if (step % K == 0) A.copy(ck[step / K], u);
       A.forward_step(u, un);
@@ -730,12 +800,47 @@ extern "C" int lsmpqs_adjoint_run(Options* o, QSS_DataArrays* p, Grid* g, const 
     hipLaunchKernelGGL(adj_loss_kernel, dim3(kBlocks), dim3(256), 0, s->st, bx, A.lam, u,
                        (real)(2.0 * r * (double)dV / vol[1]), eps);
     ACHECK(hipGetLastError());
-    for (int m = nseg - 1; m >= 0; --m) {
-      const int s0 = m * K, s1 = std::min(N, s0 + K);
-      A.copy(seg[0], ck[m]);
-      for (int step = s0 + 1; step < s1; ++step) A.forward_step(seg[step - 1 - s0], seg[step - s0]);
-      for (int step = s1 - 1; step >= s0; --step) A.reverse_step(seg[step - s0]);
+    // The segment's recompute keeps every step's stage values when they fit
+    // in half the free device memory (3 fields per step and one for the last
+    // output), so that the reverse steps need not recompute them: the
+    // forward then runs twice in all instead of three times.  The stage
+    // kernels are deterministic, so the stored values are the recomputed
+    // ones bit for bit.
+    size_t mem_free = 0, mem_total = 0;
+    const bool store = hipMemGetInfo(&mem_free, &mem_total) == hipSuccess &&
+                       (size_t)(3 * K + 1) * bytes < mem_free / 2;
+    std::vector<real*> su1, su2, spre;
+    real* slast = nullptr;
+    real* const own[4] = {A.u1, A.u2, A.u3, A.pre};
+    if (store) {
+      const std::vector<real*> f = buf.slab(3 * K + 1, n);
+      su1.assign(f.begin(), f.begin() + K);
+      su2.assign(f.begin() + K, f.begin() + 2 * K);
+      spre.assign(f.begin() + 2 * K, f.begin() + 3 * K);
+      slast = f[3 * K];
     }
+    for (int m = nseg - 1; m >= 0; --m) {
+      const int s0 = m * K, s1 = std::min(N, s0 + K), len = s1 - s0;
+      A.copy(seg[0], ck[m]);
+      if (store) {
+        auto use = [&](int q) {
+          A.u1 = su1[q]; A.u2 = su2[q]; A.pre = spre[q];
+          A.u3 = q + 1 < len ? seg[q + 1] : slast;
+        };
+        for (int q = 0; q < len; ++q) {
+          use(q);
+          A.forward_step(seg[q], A.u3);
+        }
+        for (int q = len - 1; q >= 0; --q) {
+          use(q);
+          A.reverse_step(seg[q], false);
+        }
+      } else {
+        for (int step = s0 + 1; step < s1; ++step) A.forward_step(seg[step - 1 - s0], seg[step - s0]);
+        for (int step = s1 - 1; step >= s0; --step) A.reverse_step(seg[step - s0]);
+      }
+    }
+    A.u1 = own[0]; A.u2 = own[1]; A.u3 = own[2]; A.pre = own[3];
     // theta and b gradients
     double* part = nullptr;
     ACHECK(hipMalloc((void**)&part, sizeof(double) * kBlocks));

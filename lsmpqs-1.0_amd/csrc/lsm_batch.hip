@@ -119,7 +119,12 @@ class Batch final : public BatchHook {
 
   // box thread: wait until a round has served r
   void park(Request& r) {
-    BCHECK(hipStreamSynchronize(r.s->st));
+    // the box's own stream is idle between batched requests except after
+    // set-up and curvature-step work; a synchronization costs a marker and
+    // its wait, a query only reads the stream's state
+    const hipError_t q = hipStreamQuery(r.s->st);
+    if (q == hipErrorNotReady) BCHECK(hipStreamSynchronize(r.s->st));
+    else BCHECK(q);
     release_slot();
     {
       std::unique_lock<std::mutex> lk(mu_);

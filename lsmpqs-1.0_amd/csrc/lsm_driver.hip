@@ -556,17 +556,26 @@ static real ccm_dev(Session* s, Options* o, const Grid* g, FILE* fp, bool var, r
     // buffer and the two are swapped (RK1 copies)
     if (c.order >= 2) s->redirect = true;
     else DCHECK(hipMemcpyAsync(s->phi_prev, s->phi, bytes(s), hipMemcpyDeviceToDevice, s->st));
-    DCHECK(hipEventRecord(s->ev_a, s->st));
+    // A batched box's inner steps run on the batch's stream, not on s->st:
+    // their time is the host time of the inner loop, and the box's stream
+    // is synchronized only when the batch takes its next request.
+    const auto t_in = std::chrono::steady_clock::now();
+    if (!hook) DCHECK(hipEventRecord(s->ev_a, s->st));
     real last_dt = 0;
     real dt_sub = inner_loop(s, c, o->tplot, &last_dt);
     if (s->redirect) {  // no inner step ran
       s->redirect = false;
       DCHECK(hipMemcpyAsync(s->phi_prev, s->phi, bytes(s), hipMemcpyDeviceToDevice, s->st));
     }
-    DCHECK(hipEventRecord(s->ev_b, s->st));
-    sync(s);
     float ms = 0;
-    (void)hipEventElapsedTime(&ms, s->ev_a, s->ev_b);
+    if (!hook) {
+      DCHECK(hipEventRecord(s->ev_b, s->st));
+      sync(s);
+      (void)hipEventElapsedTime(&ms, s->ev_a, s->ev_b);
+    } else {
+      ms = (float)(1e3 * std::chrono::duration<double>(std::chrono::steady_clock::now() - t_in)
+                             .count());
+    }
     s->tm.inner_ms += ms;
     if (fp) fprintf(fp, "1 GPU: Level set time = %lf\n", ms * 1e-3);
     const auto t3 = std::chrono::steady_clock::now();
