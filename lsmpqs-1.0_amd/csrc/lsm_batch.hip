@@ -19,6 +19,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <mutex>
@@ -45,7 +46,9 @@ struct Request {
   Session* s;
   StepCfg c;
   real dt = 0;
-  bool done = false;
+  bool done = false;  // under m; the box waits on cv
+  std::mutex m;
+  std::condition_variable cv;
   const OuterJob* job = nullptr;  // an outer step instead of an inner step
   bool fast = false;
   int conn = 26;
@@ -61,16 +64,24 @@ class Batch final : public BatchHook {
   // Box threads run their own work (set-up, outer steps) only while holding
   // one of `slots` run slots: a box takes a slot when it starts and after
   // each batched step, and gives it back when it parks for a step or
-  // finishes, so at most `slots` boxes issue device work from the host at a
-  // time (the other live boxes are parked or waiting for a slot).  A box
+  // finishes, so at most `slots` boxes issue runtime calls from the host at
+  // a time (the other live boxes are parked or waiting for a slot).  A box
   // holding a slot never waits for another box, so every slot comes back.
+  // Without the limit after set-up, the ~30 boxes a round wakes spent 0.6 -
+  // 1.4 ms each in their few runtime calls (64 subvolumes of a 768^3 pack,
+  // 16 host cores) instead of 0.1 ms.
   void set_slots(int n) { slots_ = n < 1 ? 1 : n; }
   void acquire_slot() {
     const double t0 = now_s();
     std::unique_lock<std::mutex> lk(slot_mu_);
     cv_slot_.wait(lk, [&] { return slots_ > 0; });
     slots_ -= 1;
+    holding() = true;
     trace().slot_s += now_s() - t0;
+  }
+  static bool& holding() {
+    static thread_local bool h = false;
+    return h;
   }
 
   // Host-time breakdown of one box thread (LSMPQS_BATCH_TRACE=1 prints the
@@ -82,6 +93,9 @@ class Batch final : public BatchHook {
     long long aways = 0;
     double slot_s = 0, parked_s = 0;
     long long steps = 0;
+    // per served request: round served to wake-up, wake-up to the next
+    // request (host work)
+    double wake_s = 0, host_s = 0, slot_at = -1;
   };
   static double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
@@ -92,8 +106,10 @@ class Batch final : public BatchHook {
     return t;
   }
   std::vector<BoxTrace> traces;  // filled by box_finished
-  double sched_launch_s = 0, sched_wait_s = 0;
+  double sched_launch_s = 0, sched_wait_s = 0, first_wait_s = -1;
   void release_slot() {
+    if (!holding()) return;
+    holding() = false;
     {
       std::lock_guard<std::mutex> lk(slot_mu_);
       slots_ += 1;
@@ -119,6 +135,10 @@ class Batch final : public BatchHook {
 
   // box thread: wait until a round has served r
   void park(Request& r) {
+    {
+      BoxTrace& t = trace();
+      if (t.slot_at >= 0) t.host_s += now_s() - t.slot_at;
+    }
     // the box's own stream is idle between batched requests except after
     // set-up and curvature-step work; a synchronization costs a marker and
     // its wait, a query only reads the stream's state
@@ -134,22 +154,24 @@ class Batch final : public BatchHook {
         BoxTrace& t = trace();
         if (t.away_since >= 0) { t.away_s += now_s() - t.away_since; t.aways += 1; t.away_since = -1; }
       }
-      if (!failure_.empty()) {
-        lk.unlock();
-        acquire_slot();
-        throw std::runtime_error(failure_);
-      }
+      if (!failure_.empty()) throw std::runtime_error(failure_);
       parked_.push_back(&r);
-      cv_sched_.notify_one();
-      BoxTrace& tr = trace();
-      const double t0 = now_s();
-      if (tr.first_step < 0) tr.first_step = t0;
-      cv_box_.wait(lk, [&] { return r.done; });
-      tr.last_step = now_s();
-      tr.parked_s += tr.last_step - t0;
-      tr.steps += 1;
+      if (ready_locked()) cv_sched_.notify_one();
     }
+    // each request is woken on its own (no herd on mu_ after a round)
+    BoxTrace& tr = trace();
+    const double t0 = now_s();
+    if (tr.first_step < 0) tr.first_step = t0;
+    {
+      std::unique_lock<std::mutex> lk(r.m);
+      r.cv.wait(lk, [&] { return r.done; });
+    }
+    tr.last_step = now_s();
+    tr.parked_s += tr.last_step - t0;
+    tr.steps += 1;
+    tr.wake_s += tr.last_step - done_at_.load(std::memory_order_relaxed);
     acquire_slot();
+    tr.slot_at = now_s();
     std::lock_guard<std::mutex> lk(mu_);
     if (!failure_.empty()) throw std::runtime_error(failure_);
   }
@@ -167,7 +189,7 @@ class Batch final : public BatchHook {
       away_ += 1;
       trace().away_since = now_s();
     }
-    cv_sched_.notify_one();
+    if (ready_locked()) cv_sched_.notify_one();
   }
 
   void box_finished() {
@@ -180,7 +202,14 @@ class Batch final : public BatchHook {
       away_ -= 1;
     }
     live_ -= 1;
-    cv_sched_.notify_one();
+    if (ready_locked()) cv_sched_.notify_one();
+  }
+
+  // (mu_ held) a round can start: every live box is parked or away, or none
+  // is left.  Box threads wake the scheduler only then, not at every park:
+  // a round of 30 boxes otherwise woke it 30 times.
+  bool ready_locked() const {
+    return live_ == 0 || (!parked_.empty() && (int)parked_.size() + away_ == live_);
   }
 
   // calling thread: serve rounds until every box has finished.  A round
@@ -195,14 +224,13 @@ class Batch final : public BatchHook {
       const double w0 = now_s();
       {
         std::unique_lock<std::mutex> lk(mu_);
-        cv_sched_.wait(lk, [&] {
-          return live_ == 0 || (!parked_.empty() && (int)parked_.size() + away_ == live_);
-        });
+        cv_sched_.wait(lk, [&] { return ready_locked(); });
         if (live_ == 0) break;
         round.swap(parked_);
       }
       const double w1 = now_s();
       sched_wait_s += w1 - w0;
+      if (first_wait_s < 0) first_wait_s = w1 - w0;
       try {
         launch(round);
         sched_launch_s += now_s() - w1;
@@ -210,9 +238,14 @@ class Batch final : public BatchHook {
         std::lock_guard<std::mutex> lk(mu_);
         failure_ = std::string("batched step: ") + e.what();
       }
-      std::lock_guard<std::mutex> lk(mu_);
-      for (Request* r : round) r->done = true;
-      cv_box_.notify_all();
+      done_at_.store(now_s(), std::memory_order_relaxed);
+      // notify under the request's lock: once it is released the box may
+      // return and its request (on the box's stack) is gone
+      for (Request* r : round) {
+        std::lock_guard<std::mutex> g(r->m);
+        r->done = true;
+        r->cv.notify_one();
+      }
     }
   }
 
@@ -536,7 +569,7 @@ class Batch final : public BatchHook {
   }
 
   std::mutex mu_, slot_mu_;
-  std::condition_variable cv_sched_, cv_box_, cv_slot_;
+  std::condition_variable cv_sched_, cv_slot_;
   int slots_ = 1 << 30;
   std::vector<Request*> parked_;
   int live_;
@@ -546,6 +579,7 @@ class Batch final : public BatchHook {
     return a;
   }
   std::string failure_;
+  std::atomic<double> done_at_{0};  // when the last round's requests were marked done
 
   int nmax_;
   int scheme_ = 0, order_ = 0, nstage_ = 0;
@@ -633,8 +667,8 @@ extern "C" int lsmpqs_batch_model3d(int n, Options** options, QSS_DataArrays** p
     // steps), own work between steps, tail (last step to end), slot waits
     double su = 0, pk = 0, own = 0, tl = 0, sl = 0, msu = 0, mtl = 0, mend = 0;
     double ph[5] = {0, 0, 0, 0, 0};  // session, model set-up, coefficients, first trapping, to step
-    double aw = 0;
-    long long naw = 0;
+    double aw = 0, wk = 0, hs = 0;
+    long long naw = 0, nst = 0;
     for (const auto& t : batch.traces) {
       double prev = t.start;
       for (int q = 0; q < 4; ++q)
@@ -642,6 +676,7 @@ extern "C" int lsmpqs_batch_model3d(int n, Options** options, QSS_DataArrays** p
       if (t.first_step >= 0) ph[4] += t.first_step - prev;
       aw += t.away_s;
       naw += t.aways;
+      wk += t.wake_s; hs += t.host_s; nst += t.steps;
       const double setup = (t.first_step < 0 ? t.end : t.first_step) - t.start;
       const double tail = t.end - (t.first_step < 0 ? t.end : t.last_step);
       su += setup; pk += t.parked_s; tl += tail; sl += t.slot_s;
@@ -649,15 +684,20 @@ extern "C" int lsmpqs_batch_model3d(int n, Options** options, QSS_DataArrays** p
       msu = std::max(msu, setup); mtl = std::max(mtl, tail); mend = std::max(mend, t.end - t_run);
     }
     fprintf(stderr,
-            "lsmpqs batch trace: %d boxes, wall %.3f s; scheduler waiting %.3f s, in steps %.3f s; "
-            "per box sums: set-up %.3f (max %.3f), parked %.3f, own work %.3f, tail %.3f (max "
-            "%.3f), slot waits %.3f s\n",
-            n, mend, batch.sched_wait_s, batch.sched_launch_s, su, msu, pk, own, tl, mtl, sl);
+            "lsmpqs batch trace: %d boxes, wall %.3f s; scheduler waiting %.3f s (%.3f before the "
+            "first round), in steps %.3f s; per box sums: set-up %.3f (max %.3f), parked %.3f, own "
+            "work %.3f, tail %.3f (max %.3f), slot waits %.3f s\n",
+            n, mend, batch.sched_wait_s, batch.first_wait_s, batch.sched_launch_s, su, msu, pk,
+            own, tl, mtl, sl);
     fprintf(stderr,
             "lsmpqs batch trace: set-up sums: session %.3f, model set-up %.3f, coefficients "
             "%.3f, first trapping %.3f, to the first step %.3f s; outer-step work %.3f s in %lld "
             "stretches\n",
             ph[0], ph[1], ph[2], ph[3], ph[4], aw, naw);
+    fprintf(stderr,
+            "lsmpqs batch trace: per served request: wake-up %.1f us, host work to the next "
+            "request %.1f us (%lld requests)\n",
+            1e6 * wk / std::max(1LL, nst), 1e6 * hs / std::max(1LL, nst), nst);
     fprintf(stderr,
             "lsmpqs batch trace: batched outer steps: trapping %.3f, mask reinitializations "
             "%.3f, merge and floor %.3f, phi reinitialization and reductions %.3f s\n",
