@@ -2,15 +2,17 @@
 // with its own options and curvature schedule) drained or imbibed together.
 //
 // Each box runs the ordinary model driver on a host thread of its own, with
-// its own device session and stream, so every outer-step operation
-// (trapping, reinitialization, reductions, schedule) is the single-box code.
-// Its inner time steps are handed to the scheduler on the calling thread
-// instead: once every live box is waiting for an inner step, one batched
-// step runs for all of them -- a batched time-step probe, then per RK stage
-// one persistent stage kernel that takes 64 x 8 x zc bricks of all boxes
-// from a work queue, and one batched ghost extrapolation.  That sequence is
-// captured once as a HIP graph; each step only uploads the per-box records
-// (pointers, geometry, brick and ghost prefix sums) and replays the graph.
+// its own device session and stream, so the schedule, the stopping tests
+// and the host logic are the single-box code.  Its inner time steps and the
+// device work of its outer steps are handed to the scheduler on the calling
+// thread instead: once every live box of a group (half of the boxes) is
+// waiting, one batched round runs for them -- for inner steps a batched
+// time-step probe, then per RK stage one persistent stage kernel that takes
+// 64 x 8 x zc bricks of all boxes from a work queue, and one batched ghost
+// extrapolation; for outer steps one launch per pass over all the boxes.
+// The inner sequence is captured once per group as a HIP graph; each step
+// only uploads the per-box records (pointers, geometry, brick and ghost
+// prefix sums) and replays the graph.
 // The brick depth zc and the number of persistent workgroups are picked on
 // the first step by timing the first stage over the actual boxes.  Every
 // voxel sees exactly the arithmetic of the single-box path, so a batched run
@@ -22,6 +24,7 @@
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <deque>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -58,8 +61,18 @@ constexpr int kTables = 9;  // block-prefix tables of a batched outer step
 
 class Batch final : public BatchHook {
  public:
-  explicit Batch(int nbox) : live_(nbox), nmax_(nbox) {}
+  // Boxes 0 .. n/2 - 1 form group 0, the rest group 1.  A round serves the
+  // parked requests of one group once each of its live boxes is parked or
+  // away; the two groups' rounds alternate on the batch stream, so that one
+  // group's boxes wake up, run their host logic and park again while the
+  // other group's round runs on the GPU (with one group, every round waited
+  // for that turnaround with the GPU idle).
+  explicit Batch(int nbox) : nmax_(nbox) {
+    live_[0] = (nbox + 1) / 2;
+    live_[1] = nbox - live_[0];
+  }
   ~Batch() override { release(); }
+  static int group_of(int box, int nbox) { return box < (nbox + 1) / 2 ? 0 : 1; }
 
   // Box threads run their own work (set-up, outer steps) only while holding
   // one of `slots` run slots: a box takes a slot when it starts and after
@@ -82,6 +95,10 @@ class Batch final : public BatchHook {
   static bool& holding() {
     static thread_local bool h = false;
     return h;
+  }
+  static int& group() {  // the calling box thread's group
+    static thread_local int g = 0;
+    return g;
   }
 
   // Host-time breakdown of one box thread (LSMPQS_BATCH_TRACE=1 prints the
@@ -146,17 +163,18 @@ class Batch final : public BatchHook {
     if (q == hipErrorNotReady) BCHECK(hipStreamSynchronize(r.s->st));
     else BCHECK(q);
     release_slot();
+    const int g = group();
     {
       std::unique_lock<std::mutex> lk(mu_);
       if (box_away()) {
         box_away() = false;
-        away_ -= 1;
+        away_[g] -= 1;
         BoxTrace& t = trace();
         if (t.away_since >= 0) { t.away_s += now_s() - t.away_since; t.aways += 1; t.away_since = -1; }
       }
       if (!failure_.empty()) throw std::runtime_error(failure_);
-      parked_.push_back(&r);
-      if (ready_locked()) cv_sched_.notify_one();
+      parked_[g].push_back(&r);
+      if (ready_locked(g)) cv_sched_.notify_one();
     }
     // each request is woken on its own (no herd on mu_ after a round)
     BoxTrace& tr = trace();
@@ -183,68 +201,113 @@ class Batch final : public BatchHook {
 
   // box thread: outer-step work until the next step()
   void away() override {
+    const int g = group();
     std::lock_guard<std::mutex> lk(mu_);
     if (!box_away()) {
       box_away() = true;
-      away_ += 1;
+      away_[g] += 1;
       trace().away_since = now_s();
     }
-    if (ready_locked()) cv_sched_.notify_one();
+    if (ready_locked(g)) cv_sched_.notify_one();
   }
 
   void box_finished() {
     release_slot();
+    const int g = group();
     std::lock_guard<std::mutex> lk(mu_);
     trace().end = now_s();
     traces.push_back(trace());
     if (box_away()) {
       box_away() = false;
-      away_ -= 1;
+      away_[g] -= 1;
     }
-    live_ -= 1;
-    if (ready_locked()) cv_sched_.notify_one();
+    live_[g] -= 1;
+    if (ready_locked(g) || live_[0] + live_[1] == 0) cv_sched_.notify_one();
   }
 
-  // (mu_ held) a round can start: every live box is parked or away, or none
-  // is left.  Box threads wake the scheduler only then, not at every park:
-  // a round of 30 boxes otherwise woke it 30 times.
-  bool ready_locked() const {
-    return live_ == 0 || (!parked_.empty() && (int)parked_.size() + away_ == live_);
+  // (mu_ held) a round of group g can start: every live box of g is parked
+  // or away, and one is parked.  Box threads wake the scheduler only then,
+  // not at every park.
+  bool ready_locked(int g) const {
+    return !parked_[g].empty() && (int)parked_[g].size() + away_[g] == live_[g];
   }
 
   // calling thread: serve rounds until every box has finished.  A round
-  // runs the step of every parked box once each live box is parked or away
-  // (in outer-step work, which then overlaps the round on the GPU).  Before
-  // the first step no box is away, so the first round, which also picks the
-  // brick shape, holds every box.  Which round serves a box's step changes
-  // nothing in its arithmetic.
+  // runs the requests of every parked box of a group once each live box of
+  // that group is parked or away (in outer-step work, which then overlaps
+  // the round on the GPU).  A group's next round is launched while the
+  // other group's runs; its results are delivered once the stream has
+  // passed its last event (rounds complete in launch order: one stream).
+  // Which round serves a box's request changes nothing in its arithmetic.
   void run() {
+    std::deque<int> inflight;  // lanes with a round on the stream, oldest first
+    const bool sync = trace_on();  // per-part timing needs one round at a time
     for (;;) {
+      int g = -1;
       std::vector<Request*> round;
       const double w0 = now_s();
       {
         std::unique_lock<std::mutex> lk(mu_);
-        cv_sched_.wait(lk, [&] { return ready_locked(); });
-        if (live_ == 0) break;
-        round.swap(parked_);
+        auto pick = [&] {
+          for (int q = 0; q < 2; ++q) {
+            const int c = (next_ + q) & 1;
+            if (!lanes_[c].busy && ready_locked(c)) return c;
+          }
+          return -1;
+        };
+        if (inflight.empty()) {
+          cv_sched_.wait(lk, [&] { return live_[0] + live_[1] == 0 || pick() >= 0; });
+          if (live_[0] + live_[1] == 0) break;
+        } else if (inflight.size() < 2) {
+          // the other group's boxes are in their host turnaround: take their
+          // round as soon as they are parked, else look at the stream again
+          cv_sched_.wait_for(lk, std::chrono::microseconds(40), [&] { return pick() >= 0; });
+        }
+        g = pick();
+        if (g >= 0) {
+          round.swap(parked_[g]);
+          lanes_[g].busy = true;
+          next_ = g ^ 1;
+        }
       }
-      const double w1 = now_s();
-      sched_wait_s += w1 - w0;
-      if (first_wait_s < 0) first_wait_s = w1 - w0;
-      try {
-        launch(round);
-        sched_launch_s += now_s() - w1;
-      } catch (const std::exception& e) {
-        std::lock_guard<std::mutex> lk(mu_);
-        failure_ = std::string("batched step: ") + e.what();
+      if (g >= 0) {
+        const double w1 = now_s();
+        sched_wait_s += w1 - w0;
+        if (first_wait_s < 0) first_wait_s = w1 - w0;
+        Lane& L = lanes_[g];
+        L.round = std::move(round);
+        try {
+          launch(L);
+          sched_launch_s += now_s() - w1;
+        } catch (const std::exception& e) {
+          {
+            std::lock_guard<std::mutex> lk(mu_);
+            failure_ = std::string("batched step: ") + e.what();
+          }
+          L.failed = true;
+        }
+        inflight.push_back(g);
+        if (!sync && !L.failed) continue;  // perhaps the other group is ready too
       }
-      done_at_.store(now_s(), std::memory_order_relaxed);
-      // notify under the request's lock: once it is released the box may
-      // return and its request (on the box's stack) is gone
-      for (Request* r : round) {
-        std::lock_guard<std::mutex> g(r->m);
-        r->done = true;
-        r->cv.notify_one();
+      // deliver the oldest round once the stream has passed it (wait for it
+      // when both lanes are busy: nothing else can be launched meanwhile)
+      if (inflight.empty()) continue;
+      Lane& O = lanes_[inflight.front()];
+      bool done = O.failed;
+      if (!done) {
+        if (sync || inflight.size() == 2) {
+          const hipError_t e = hipEventSynchronize(O.ev_done);
+          if (e != hipSuccess) note_failure(e, O);
+          done = true;
+        } else {
+          const hipError_t e = hipEventQuery(O.ev_done);
+          if (e == hipSuccess) done = true;
+          else if (e != hipErrorNotReady) { note_failure(e, O); done = true; }
+        }
+      }
+      if (done) {
+        inflight.pop_front();
+        finish(O);
       }
     }
   }
@@ -252,11 +315,50 @@ class Batch final : public BatchHook {
   lsmpqs_batch_stats stats{};
 
  private:
+  // One group's buffers: the per-box records and prefix tables of its inner
+  // steps (with their captured graph), the jobs and tables of its outer
+  // steps, the pinned read-backs, and the round it has in flight.
+  struct Lane {
+    BatchView view{};
+    BatchRec* d_recs = nullptr;
+    int *d_count = nullptr, *d_brick = nullptr, *d_counters = nullptr;
+    int64_t* d_ghost = nullptr;
+    real* d_dts = nullptr;
+    BatchRec* h_recs = nullptr;
+    int *h_count = nullptr, *h_brick = nullptr;
+    int64_t* h_ghost = nullptr;
+    real* h_dts = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    OuterJob *d_jobs = nullptr, *h_jobs = nullptr;
+    int *d_tab = nullptr, *h_tab = nullptr;
+    double *d_redout = nullptr, *h_redout = nullptr;
+    hipEvent_t ev_in0 = nullptr, ev_in1 = nullptr, ev_out0 = nullptr, ev_out1 = nullptr,
+               ev_done = nullptr;
+    std::vector<Request*> round, inner, outer;
+    bool busy = false, failed = false;
+  };
+  Lane lanes_[2];
+  int next_ = 0;  // lane looked at first
+
+  void note_failure(hipError_t e, Lane& L) {
+    std::lock_guard<std::mutex> lk(mu_);
+    failure_ = std::string("batched step: ") + hipGetErrorString(e);
+    L.failed = true;
+  }
+
   void ensure_stream() {
     if (st_) return;
     BCHECK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
     BCHECK(hipEventCreate(&ev0_));
     BCHECK(hipEventCreate(&ev1_));
+    for (Lane& L : lanes_) {
+      BCHECK(hipEventCreate(&L.ev_in0));
+      BCHECK(hipEventCreate(&L.ev_in1));
+      BCHECK(hipEventCreate(&L.ev_out0));
+      BCHECK(hipEventCreate(&L.ev_out1));
+      BCHECK(hipEventCreateWithFlags(&L.ev_done, hipEventDisableTiming));
+    }
   }
 
   void configure(const Request& first) {
@@ -266,23 +368,25 @@ class Batch final : public BatchHook {
     fast_ = sizeof(real) == 4 && !(quirks() & EXACT_FP32);
     nstage_ = order_ <= 1 ? 1 : (order_ == 2 ? 2 : 3);
     ensure_stream();
-    BCHECK(hipMalloc((void**)&d_recs_, sizeof(BatchRec) * nmax_));
-    BCHECK(hipMalloc((void**)&d_count_, sizeof(int) * 4));
-    BCHECK(hipMalloc((void**)&d_brick_, sizeof(int) * (nmax_ + 1)));
-    BCHECK(hipMalloc((void**)&d_ghost_, sizeof(int64_t) * (nmax_ + 1)));
-    BCHECK(hipMalloc((void**)&d_counters_, sizeof(int) * kCounters));
-    BCHECK(hipMalloc((void**)&d_dts_, sizeof(real) * nmax_));
-    BCHECK(hipHostMalloc((void**)&h_recs_, sizeof(BatchRec) * nmax_, hipHostMallocDefault));
-    BCHECK(hipHostMalloc((void**)&h_count_, sizeof(int) * 4, hipHostMallocDefault));
-    BCHECK(hipHostMalloc((void**)&h_brick_, sizeof(int) * (nmax_ + 1), hipHostMallocDefault));
-    BCHECK(hipHostMalloc((void**)&h_ghost_, sizeof(int64_t) * (nmax_ + 1), hipHostMallocDefault));
-    BCHECK(hipHostMalloc((void**)&h_dts_, sizeof(real) * nmax_, hipHostMallocDefault));
-    view_.recs = d_recs_;
-    view_.count = d_count_;
-    view_.brick_prefix = d_brick_;
-    view_.ghost_prefix = d_ghost_;
-    view_.counters = d_counters_;
-    view_.dts = d_dts_;
+    for (Lane& L : lanes_) {
+      BCHECK(hipMalloc((void**)&L.d_recs, sizeof(BatchRec) * nmax_));
+      BCHECK(hipMalloc((void**)&L.d_count, sizeof(int) * 4));
+      BCHECK(hipMalloc((void**)&L.d_brick, sizeof(int) * (nmax_ + 1)));
+      BCHECK(hipMalloc((void**)&L.d_ghost, sizeof(int64_t) * (nmax_ + 1)));
+      BCHECK(hipMalloc((void**)&L.d_counters, sizeof(int) * kCounters));
+      BCHECK(hipMalloc((void**)&L.d_dts, sizeof(real) * nmax_));
+      BCHECK(hipHostMalloc((void**)&L.h_recs, sizeof(BatchRec) * nmax_, hipHostMallocDefault));
+      BCHECK(hipHostMalloc((void**)&L.h_count, sizeof(int) * 4, hipHostMallocDefault));
+      BCHECK(hipHostMalloc((void**)&L.h_brick, sizeof(int) * (nmax_ + 1), hipHostMallocDefault));
+      BCHECK(hipHostMalloc((void**)&L.h_ghost, sizeof(int64_t) * (nmax_ + 1), hipHostMallocDefault));
+      BCHECK(hipHostMalloc((void**)&L.h_dts, sizeof(real) * nmax_, hipHostMallocDefault));
+      L.view.recs = L.d_recs;
+      L.view.count = L.d_count;
+      L.view.brick_prefix = L.d_brick;
+      L.view.ghost_prefix = L.d_ghost;
+      L.view.counters = L.d_counters;
+      L.view.dts = L.d_dts;
+    }
     int dev = 0, cus = 256;
     BCHECK(hipGetDevice(&dev));
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
@@ -292,15 +396,24 @@ class Batch final : public BatchHook {
   }
 
   void release() {
-    if (exec_) (void)hipGraphExecDestroy(exec_);
-    if (graph_) (void)hipGraphDestroy(graph_);
-    void* d[] = {d_recs_, d_count_, d_brick_, d_ghost_, d_counters_, d_dts_, d_jobs_, d_tab_,
-                 d_redout_};
-    for (void* p : d)
-      if (p) (void)hipFree(p);
-    void* h[] = {h_recs_, h_count_, h_brick_, h_ghost_, h_dts_, h_jobs_, h_tab_, h_redout_};
-    for (void* p : h)
-      if (p) (void)hipHostFree(p);
+    // rounds still on the stream (after a failure) finish before their
+    // buffers go
+    if (st_) (void)hipStreamSynchronize(st_);
+    for (Lane& L : lanes_) {
+      if (L.exec) (void)hipGraphExecDestroy(L.exec);
+      if (L.graph) (void)hipGraphDestroy(L.graph);
+      void* d[] = {L.d_recs, L.d_count, L.d_brick, L.d_ghost, L.d_counters, L.d_dts, L.d_jobs,
+                   L.d_tab, L.d_redout};
+      for (void* p : d)
+        if (p) (void)hipFree(p);
+      void* h[] = {L.h_recs, L.h_count, L.h_brick, L.h_ghost, L.h_dts, L.h_jobs, L.h_tab,
+                   L.h_redout};
+      for (void* p : h)
+        if (p) (void)hipHostFree(p);
+      hipEvent_t ev[] = {L.ev_in0, L.ev_in1, L.ev_out0, L.ev_out1, L.ev_done};
+      for (hipEvent_t e : ev)
+        if (e) (void)hipEventDestroy(e);
+    }
     for (hipEvent_t e : pass_ev_)
       if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : inner_ev_)
@@ -311,14 +424,14 @@ class Batch final : public BatchHook {
   }
 
   // records of the round's boxes for brick depth zc, copied to the device
-  void upload(const std::vector<Request*>& round, int zc) {
+  void upload(Lane& L, const std::vector<Request*>& round, int zc) {
     const int n = (int)round.size();
-    h_count_[0] = n;
-    h_brick_[0] = 0;
-    h_ghost_[0] = 0;
+    L.h_count[0] = n;
+    L.h_brick[0] = 0;
+    L.h_ghost[0] = 0;
     for (int i = 0; i < n; ++i) {
       StepArgs a = make_step_args(round[i]->s, round[i]->c);
-      BatchRec& r = h_recs_[i];
+      BatchRec& r = L.h_recs[i];
       r.probe = a.probe;
       int bricks = 0;
       for (int k = 0; k < 3; ++k) {
@@ -326,27 +439,28 @@ class Batch final : public BatchHook {
         r.bc[k] = a.bc[k];
         bricks = stage_bricks(r.stage[k], zc);
       }
-      h_brick_[i + 1] = h_brick_[i] + bricks;
-      h_ghost_[i + 1] = h_ghost_[i] + ghost_count(a.probe.bx);
+      L.h_brick[i + 1] = L.h_brick[i] + bricks;
+      L.h_ghost[i + 1] = L.h_ghost[i] + ghost_count(a.probe.bx);
     }
-    BCHECK(hipMemcpyAsync(d_recs_, h_recs_, sizeof(BatchRec) * n, hipMemcpyHostToDevice, st_));
-    BCHECK(hipMemcpyAsync(d_count_, h_count_, sizeof(int), hipMemcpyHostToDevice, st_));
-    BCHECK(hipMemcpyAsync(d_brick_, h_brick_, sizeof(int) * (n + 1), hipMemcpyHostToDevice, st_));
-    BCHECK(hipMemcpyAsync(d_ghost_, h_ghost_, sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, st_));
+    BCHECK(hipMemcpyAsync(L.d_recs, L.h_recs, sizeof(BatchRec) * n, hipMemcpyHostToDevice, st_));
+    BCHECK(hipMemcpyAsync(L.d_count, L.h_count, sizeof(int), hipMemcpyHostToDevice, st_));
+    BCHECK(hipMemcpyAsync(L.d_brick, L.h_brick, sizeof(int) * (n + 1), hipMemcpyHostToDevice, st_));
+    BCHECK(hipMemcpyAsync(L.d_ghost, L.h_ghost, sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice,
+                          st_));
   }
 
   // timed: events between the parts (host-time tracing; not captured)
-  void enqueue_step(bool timed = false) {
+  void enqueue_step(Lane& L, bool timed = false) {
     if (timed) mark_inner(0);
-    BCHECK(launch_batch_reset(d_counters_, kCounters, st_));
-    BCHECK(launch_probe_batch(view_, scheme_, nmax_, st_));
-    BCHECK(hipMemcpyAsync(h_dts_, d_dts_, sizeof(real) * nmax_, hipMemcpyDeviceToHost, st_));
+    BCHECK(launch_batch_reset(L.d_counters, kCounters, st_));
+    BCHECK(launch_probe_batch(L.view, scheme_, nmax_, st_));
+    BCHECK(hipMemcpyAsync(L.h_dts, L.d_dts, sizeof(real) * nmax_, hipMemcpyDeviceToHost, st_));
     for (int k = 0; k < nstage_; ++k) {
       const bool fin = k == nstage_ - 1;
       if (timed) mark_inner(1 + 2 * k);
-      BCHECK(launch_stage_batch(view_, k, scheme_, var_, k + 1, fin, fast_, grid_, st_));
+      BCHECK(launch_stage_batch(L.view, k, scheme_, var_, k + 1, fin, fast_, grid_, st_));
       if (timed) mark_inner(2 + 2 * k);
-      BCHECK(launch_bc_batch(view_, k, fin, bc_grid_, st_));
+      BCHECK(launch_bc_batch(L.view, k, fin, bc_grid_, st_));
     }
     if (timed) mark_inner(1 + 2 * nstage_);
   }
@@ -356,21 +470,21 @@ class Batch final : public BatchHook {
   }
 
   // brick depth and workgroup count: fastest first stage over this round
-  void autotune(const std::vector<Request*>& round) {
+  void autotune(Lane& L, const std::vector<Request*>& round) {
     const int depths[] = {8, 16, 32, 64};
     const int per_cu[] = {1, 2, 4};
     float best = 1e30f;
-    upload(round, depths[0]);
-    BCHECK(launch_probe_batch(view_, scheme_, nmax_, st_));  // stage 1 reads this step's dt
+    upload(L, round, depths[0]);
+    BCHECK(launch_probe_batch(L.view, scheme_, nmax_, st_));  // stage 1 reads this step's dt
     for (int zc : depths) {
-      upload(round, zc);
+      upload(L, round, zc);
       for (int pc : per_cu) {
         const int grid = pc * cus_;
         float t = 1e30f;
         for (int rep = 0; rep < 2; ++rep) {
-          BCHECK(launch_batch_reset(d_counters_, kCounters, st_));
+          BCHECK(launch_batch_reset(L.d_counters, kCounters, st_));
           BCHECK(hipEventRecord(ev0_, st_));
-          BCHECK(launch_stage_batch(view_, 0, scheme_, var_, 1, nstage_ == 1, fast_, grid, st_));
+          BCHECK(launch_stage_batch(L.view, 0, scheme_, var_, 1, nstage_ == 1, fast_, grid, st_));
           BCHECK(hipEventRecord(ev1_, st_));
           BCHECK(hipEventSynchronize(ev1_));
           float ms = 0;
@@ -412,60 +526,129 @@ class Batch final : public BatchHook {
     return 8;
   }
 
-  void capture() {
+  void capture(Lane& L) {
+    BCHECK(hipStreamSynchronize(st_));  // once per lane: nothing of the other lane pending
     BCHECK(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
     try {
-      enqueue_step();
+      enqueue_step(L);
     } catch (...) {
       hipGraph_t g = nullptr;
       (void)hipStreamEndCapture(st_, &g);
       if (g) (void)hipGraphDestroy(g);
       throw;
     }
-    BCHECK(hipStreamEndCapture(st_, &graph_));
-    BCHECK(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
+    BCHECK(hipStreamEndCapture(st_, &L.graph));
+    BCHECK(hipGraphInstantiate(&L.exec, L.graph, nullptr, nullptr, 0));
   }
 
-  void launch(const std::vector<Request*>& round) {
-    std::vector<Request*> inner, outer;
-    for (Request* r : round) (r->job ? outer : inner).push_back(r);
-    if (!outer.empty()) launch_outer(outer);
-    if (!inner.empty()) launch_inner(inner);
-  }
-
-  // One outer step of each box of `reqs`: every pass one launch over all of
-  // them, in the single-box order (lsm_driver.hip: trap_dev, reinit_dev of
-  // mask_w and mask_nw, trapped-phase velocities, merge and floor, reinit_dev
-  // of phi with the outer reductions), then the reductions read back.
-  void launch_outer(const std::vector<Request*>& reqs) {
+  // enqueue the round of lane L (outer steps, then inner steps) and the
+  // event that marks its end; finish(L) delivers the results
+  void launch(Lane& L) {
+    L.inner.clear();
+    L.outer.clear();
+    L.failed = false;
+    for (Request* r : L.round) (r->job ? L.outer : L.inner).push_back(r);
     ensure_stream();
+    if (!L.outer.empty()) launch_outer(L);
+    if (!L.inner.empty()) launch_inner(L);
+    BCHECK(hipEventRecord(L.ev_done, st_));
+  }
+
+  // results of lane L's round (its stream work is complete, or it failed)
+  // to the boxes, which are then woken
+  void finish(Lane& L) {
+    if (!L.failed) {
+      try {
+        collect(L);
+      } catch (const std::exception& e) {
+        std::lock_guard<std::mutex> lk(mu_);
+        failure_ = std::string("batched step: ") + e.what();
+      }
+    }
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      L.busy = false;
+    }
+    done_at_.store(now_s(), std::memory_order_relaxed);
+    // notify under the request's lock: once it is released the box may
+    // return and its request (on the box's stack) is gone
+    for (Request* r : L.round) {
+      std::lock_guard<std::mutex> g(r->m);
+      r->done = true;
+      r->cv.notify_one();
+    }
+    L.round.clear();
+  }
+  void collect(Lane& L) {
+    const bool tr = trace_on();
+    if (!L.outer.empty()) {
+      const int n = (int)L.outer.size();
+      float ms = 0;
+      BCHECK(hipEventElapsedTime(&ms, L.ev_out0, L.ev_out1));
+      stats.batched_outer += 1;
+      stats.box_outer += n;
+      stats.batched_outer_ms += ms;
+      if (tr)
+        for (int q = 0; q < 4; ++q) {
+          float m = 0;
+          BCHECK(hipEventElapsedTime(&m, pass_ev_[q], pass_ev_[q + 1]));
+          pass_ms[q] += m;
+        }
+      for (int i = 0; i < n; ++i) {
+        L.outer[i]->s->h_red[0] = L.h_redout[2 * i];
+        L.outer[i]->s->h_red[1] = L.h_redout[2 * i + 1];
+      }
+    }
+    if (!L.inner.empty()) {
+      float ms = 0;
+      BCHECK(hipEventElapsedTime(&ms, L.ev_in0, L.ev_in1));
+      if (tr)
+        for (int q = 0; q < 1 + 2 * nstage_; ++q) {
+          float m = 0;
+          BCHECK(hipEventElapsedTime(&m, inner_ev_[q], inner_ev_[q + 1]));
+          inner_ms[q] += m;
+        }
+      stats.batched_steps += 1;
+      stats.batched_ms += ms;
+      stats.box_steps += (long long)L.inner.size();
+      for (size_t i = 0; i < L.inner.size(); ++i) L.inner[i]->dt = L.h_dts[i];
+    }
+  }
+
+  // One outer step of each box of the lane's round: every pass one launch
+  // over all of them, in the single-box order (lsm_driver.hip: trap_dev,
+  // reinit_dev of mask_w and mask_nw, trapped-phase velocities, merge and
+  // floor, reinit_dev of phi with the outer reductions), then the
+  // reductions read back.
+  void launch_outer(Lane& L) {
+    const std::vector<Request*>& reqs = L.outer;
     const int n = (int)reqs.size();
     const int conn = reqs[0]->conn;
     const bool fast = reqs[0]->fast;
     for (Request* r : reqs)
       if (r->conn != conn || r->fast != fast)
         throw std::runtime_error("batched outer steps need one connectivity and arithmetic mode");
-    if (!d_jobs_) {
-      BCHECK(hipMalloc((void**)&d_jobs_, sizeof(OuterJob) * nmax_));
-      BCHECK(hipMalloc((void**)&d_tab_, sizeof(int) * kTables * (nmax_ + 1)));
-      BCHECK(hipMalloc((void**)&d_redout_, sizeof(double) * 2 * nmax_));
-      BCHECK(hipHostMalloc((void**)&h_jobs_, sizeof(OuterJob) * nmax_, hipHostMallocDefault));
-      BCHECK(hipHostMalloc((void**)&h_tab_, sizeof(int) * kTables * (nmax_ + 1),
+    if (!L.d_jobs) {
+      BCHECK(hipMalloc((void**)&L.d_jobs, sizeof(OuterJob) * nmax_));
+      BCHECK(hipMalloc((void**)&L.d_tab, sizeof(int) * kTables * (nmax_ + 1)));
+      BCHECK(hipMalloc((void**)&L.d_redout, sizeof(double) * 2 * nmax_));
+      BCHECK(hipHostMalloc((void**)&L.h_jobs, sizeof(OuterJob) * nmax_, hipHostMallocDefault));
+      BCHECK(hipHostMalloc((void**)&L.h_tab, sizeof(int) * kTables * (nmax_ + 1),
                            hipHostMallocDefault));
-      BCHECK(hipHostMalloc((void**)&h_redout_, sizeof(double) * 2 * nmax_, hipHostMallocDefault));
+      BCHECK(hipHostMalloc((void**)&L.h_redout, sizeof(double) * 2 * nmax_, hipHostMallocDefault));
     }
     if (n > nmax_) throw std::runtime_error("more outer-step requests than boxes");
     bool velocities = false;
     for (int i = 0; i < n; ++i) {
-      h_jobs_[i] = *reqs[i]->job;
-      velocities = velocities || h_jobs_[i].vn != nullptr;
+      L.h_jobs[i] = *reqs[i]->job;
+      velocities = velocities || L.h_jobs[i].vn != nullptr;
     }
     // per launch kind: a prefix table of the boxes' single-box grids
     auto table = [&](int k, auto count) {
-      int* p = h_tab_ + k * (nmax_ + 1);
+      int* p = L.h_tab + k * (nmax_ + 1);
       p[0] = 0;
-      for (int i = 0; i < n; ++i) p[i + 1] = p[i] + count(h_jobs_[i]);
-      return Multi{d_tab_ + k * (nmax_ + 1), n, p[n]};
+      for (int i = 0; i < n; ++i) p[i + 1] = p[i] + count(L.h_jobs[i]);
+      return Multi{L.d_tab + k * (nmax_ + 1), n, p[n]};
     };
     const Multi trap = table(0, [](const OuterJob& j) { return trap_pass_blocks(j.ta.bx); });
     const Multi force = table(1, [](const OuterJob&) { return force_pass_blocks(); });
@@ -476,45 +659,30 @@ class Batch final : public BatchHook {
     const Multi red = table(6, [](const OuterJob& j) { return j.nred; });
     const Multi stream = table(7, [](const OuterJob& j) { return stream_pass_blocks(j.ta.bx.size()); });
     const Multi ghosts = table(8, [](const OuterJob& j) { return bc_pass_blocks(j.ta.bx); });
-    BCHECK(hipMemcpyAsync(d_jobs_, h_jobs_, sizeof(OuterJob) * n, hipMemcpyHostToDevice, st_));
-    BCHECK(hipMemcpyAsync(d_tab_, h_tab_, sizeof(int) * kTables * (nmax_ + 1),
+    OuterJob* jobs = L.d_jobs;
+    BCHECK(hipMemcpyAsync(jobs, L.h_jobs, sizeof(OuterJob) * n, hipMemcpyHostToDevice, st_));
+    BCHECK(hipMemcpyAsync(L.d_tab, L.h_tab, sizeof(int) * kTables * (nmax_ + 1),
                           hipMemcpyHostToDevice, st_));
-    BCHECK(hipEventRecord(ev0_, st_));
+    BCHECK(hipEventRecord(L.ev_out0, st_));
     const bool tr = trace_on();
     if (tr) mark_pass(0);
-    BCHECK(launch_trap_multi(d_jobs_, trap, force, one, tiles, jump, conn, st_));
+    BCHECK(launch_trap_multi(jobs, trap, force, one, tiles, jump, conn, st_));
     if (tr) mark_pass(1);
     for (int pass = 0; pass < 2; ++pass) {
-      BCHECK(launch_reinit_multi(d_jobs_, pass, rows, fast, st_));
-      BCHECK(launch_bc_multi(d_jobs_, pass, ghosts, st_));
+      BCHECK(launch_reinit_multi(jobs, pass, rows, fast, st_));
+      BCHECK(launch_bc_multi(jobs, pass, ghosts, st_));
     }
     if (tr) mark_pass(2);
-    if (velocities) BCHECK(launch_trap_velocity_multi(d_jobs_, rows, st_));
-    BCHECK(launch_merge_floor_multi(d_jobs_, stream, st_));
+    if (velocities) BCHECK(launch_trap_velocity_multi(jobs, rows, st_));
+    BCHECK(launch_merge_floor_multi(jobs, stream, st_));
     if (tr) mark_pass(3);
-    BCHECK(launch_reinit_reduce_multi(d_jobs_, red, one, fast, st_));
-    BCHECK(launch_bc_multi(d_jobs_, 2, ghosts, st_));
-    BCHECK(launch_gather_red(d_jobs_, n, d_redout_, st_));
+    BCHECK(launch_reinit_reduce_multi(jobs, red, one, fast, st_));
+    BCHECK(launch_bc_multi(jobs, 2, ghosts, st_));
+    BCHECK(launch_gather_red(jobs, n, L.d_redout, st_));
     if (tr) mark_pass(4);
-    BCHECK(hipEventRecord(ev1_, st_));
-    BCHECK(hipMemcpyAsync(h_redout_, d_redout_, sizeof(double) * 2 * n, hipMemcpyDeviceToHost,
+    BCHECK(hipEventRecord(L.ev_out1, st_));
+    BCHECK(hipMemcpyAsync(L.h_redout, L.d_redout, sizeof(double) * 2 * n, hipMemcpyDeviceToHost,
                           st_));
-    BCHECK(hipStreamSynchronize(st_));
-    float ms = 0;
-    BCHECK(hipEventElapsedTime(&ms, ev0_, ev1_));
-    stats.batched_outer += 1;
-    stats.box_outer += n;
-    stats.batched_outer_ms += ms;
-    if (tr)
-      for (int q = 0; q < 4; ++q) {
-        float m = 0;
-        BCHECK(hipEventElapsedTime(&m, pass_ev_[q], pass_ev_[q + 1]));
-        pass_ms[q] += m;
-      }
-    for (int i = 0; i < n; ++i) {
-      reqs[i]->s->h_red[0] = h_redout_[2 * i];
-      reqs[i]->s->h_red[1] = h_redout_[2 * i + 1];
-    }
   }
 
   // per-pass timing of the batched outer steps (LSMPQS_BATCH_TRACE=1)
@@ -535,7 +703,8 @@ class Batch final : public BatchHook {
   hipEvent_t pass_ev_[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   hipEvent_t inner_ev_[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
 
-  void launch_inner(const std::vector<Request*>& round) {
+  void launch_inner(Lane& L) {
+    const std::vector<Request*>& round = L.inner;
     if (!configured_) {
       configure(*round[0]);
       configured_ = true;
@@ -544,36 +713,22 @@ class Batch final : public BatchHook {
       if (r->c.scheme != scheme_ || r->c.var != var_ || r->c.order != order_)
         throw std::runtime_error(
             "all boxes of a batch need the same spatial scheme, time order and contact-angle path");
-    if (zc_ == 0) autotune(round);
+    if (zc_ == 0) autotune(L, round);
     const int zc = round_depth(round);
-    upload(round, zc);
-    if (!exec_) capture();
-    const bool tr = trace_on();
-    BCHECK(hipEventRecord(ev0_, st_));
-    if (tr) enqueue_step(true);  // the same launches, uncaptured, with events between them
-    else BCHECK(hipGraphLaunch(exec_, st_));
-    BCHECK(hipEventRecord(ev1_, st_));
-    BCHECK(hipStreamSynchronize(st_));
-    float ms = 0;
-    BCHECK(hipEventElapsedTime(&ms, ev0_, ev1_));
-    if (tr)
-      for (int q = 0; q < 1 + 2 * nstage_; ++q) {
-        float m = 0;
-        BCHECK(hipEventElapsedTime(&m, inner_ev_[q], inner_ev_[q + 1]));
-        inner_ms[q] += m;
-      }
-    stats.batched_steps += 1;
-    stats.batched_ms += ms;
-    stats.box_steps += (long long)round.size();
-    for (size_t i = 0; i < round.size(); ++i) round[i]->dt = h_dts_[i];
+    upload(L, round, zc);
+    if (!L.exec) capture(L);
+    BCHECK(hipEventRecord(L.ev_in0, st_));
+    if (trace_on()) enqueue_step(L, true);  // the same launches, uncaptured, with events between them
+    else BCHECK(hipGraphLaunch(L.exec, st_));
+    BCHECK(hipEventRecord(L.ev_in1, st_));
   }
 
   std::mutex mu_, slot_mu_;
   std::condition_variable cv_sched_, cv_slot_;
   int slots_ = 1 << 30;
-  std::vector<Request*> parked_;
-  int live_;
-  int away_ = 0;  // live boxes between leaving an inner loop and their next step
+  std::vector<Request*> parked_[2];
+  int live_[2] = {0, 0};
+  int away_[2] = {0, 0};  // live boxes between leaving an inner loop and their next request
   static bool& box_away() {
     static thread_local bool a = false;
     return a;
@@ -588,24 +743,7 @@ class Batch final : public BatchHook {
   double ratio_ = 0;  // bricks per persistent workgroup of the tuned round
   hipStream_t st_ = nullptr;
   bool configured_ = false;
-  hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
-  OuterJob* d_jobs_ = nullptr;
-  OuterJob* h_jobs_ = nullptr;
-  int* d_tab_ = nullptr;
-  int* h_tab_ = nullptr;
-  double* d_redout_ = nullptr;
-  double* h_redout_ = nullptr;
-  hipGraph_t graph_ = nullptr;
-  hipGraphExec_t exec_ = nullptr;
-  BatchView view_{};
-  BatchRec* d_recs_ = nullptr;
-  int *d_count_ = nullptr, *d_brick_ = nullptr, *d_counters_ = nullptr;
-  int64_t* d_ghost_ = nullptr;
-  real* d_dts_ = nullptr;
-  BatchRec* h_recs_ = nullptr;
-  int *h_count_ = nullptr, *h_brick_ = nullptr;
-  int64_t* h_ghost_ = nullptr;
-  real* h_dts_ = nullptr;
+  hipEvent_t ev0_ = nullptr, ev1_ = nullptr;  // autotuning
 };
 
 }  // namespace
@@ -646,6 +784,7 @@ extern "C" int lsmpqs_batch_model3d(int n, Options** options, QSS_DataArrays** p
   threads.reserve(n);
   for (int i = 0; i < n; ++i) {
     threads.emplace_back([&, i] {
+      Batch::group() = Batch::group_of(i, n);
       Batch::trace().start = Batch::now_s();
       batch.acquire_slot();
       try {

@@ -213,11 +213,17 @@ hipError_t launch_coeff_reduce(const Box& bx, const real* b, const real* vx, con
 // Subcell-fix reinitialization (reference:
 // src/reinitialization/qss_reinitialization3d.f:476-746)
 
-// Bitwise, not short-circuit, operators: the test of all six neighbours is
-// straight-line code (with && / || each neighbour became a branch that
-// waited for that neighbour's load before the next one was examined).
-__device__ __forceinline__ bool sign_change(real a, real c) {
-  return ((a > real(0)) & (c <= real(0))) | ((a <= real(0)) & (c > real(0)));
+// Whether a face neighbour lies across zero from c, the reference's test
+// of the six neighbours one by one ((a > 0 and c <= 0) or (a <= 0 and c > 0),
+// :518-570): with c > 0 some neighbour is <= 0, i.e. their minimum is; with
+// c <= 0 some neighbour is > 0, i.e. their maximum is; a NaN c or neighbour
+// makes no crossing either way.  Two min3/max3 chains instead of twelve
+// comparisons and their selects.
+__device__ __forceinline__ bool crosses_zero(real c, real xp, real xm, real yp, real ym, real zp,
+                                             real zm) {
+  const real lo = fmin(fmin(fmin(xp, xm), fmin(yp, ym)), fmin(zp, zm));
+  const real hi = fmax(fmax(fmax(xp, xm), fmax(yp, ym)), fmax(zp, zm));
+  return (c > real(0)) ? (lo <= real(0)) : ((c <= real(0)) & (hi > real(0)));
 }
 
 // Quotient a / b: IEEE division, or in the fast single-precision mode a
@@ -239,9 +245,7 @@ __device__ __forceinline__ real rsqrt_(real x) {
 template <bool FAST>
 __device__ __forceinline__ real reinit_d0(real c, real xp, real xm, real yp, real ym, real zp,
                                           real zm, real flag) {
-  const bool near = sign_change(xp, c) | sign_change(xm, c) | sign_change(yp, c) |
-                    sign_change(ym, c) | sign_change(zp, c) | sign_change(zm, c);
-  if (!near) return flag;
+  if (!crosses_zero(c, xp, xm, yp, ym, zp, zm)) return flag;
   real delta = tmax(tmax(tmax(fabs(xp - xm) / real(2), fabs(xp - c)), fabs(c - xm)), real(1.e-5));
   delta = tmax(tmax(tmax(fabs(yp - ym) / real(2), fabs(yp - c)), fabs(c - ym)), delta);
   delta = tmax(tmax(tmax(fabs(zp - zm) / real(2), fabs(zp - c)), fabs(c - zm)), delta);
@@ -256,14 +260,13 @@ __device__ __forceinline__ real reinit_rhs(real c, const real* gp, const real* g
                                            real d0, real flag, real max_dx) {
   real sgn0 = fabs(p0) > real(1.e-5) ? copysign(real(1), p0) : real(0);
   if (d0 != flag) return d0 - rdiv<FAST>(sgn0 * fabs(c), max_dx);
+  // per axis the reference's max(max(-gp, 0), max(gm, 0)) for c > 0 and
+  // max(max(gp, 0), max(-gm, 0)) otherwise, as max(max(-s gp, s gm), 0) with
+  // s = +-1 (the same values, no branch on c)
+  const real sc = c > real(0) ? real(1) : real(-1);
   real st[3];
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    real a, b;
-    if (c > real(0)) { a = tmax(-gp[d], real(0)); b = tmax(gm[d], real(0)); }
-    else { a = tmax(gp[d], real(0)); b = tmax(-gm[d], real(0)); }
-    st[d] = tmax(a, b);
-  }
+  for (int d = 0; d < 3; ++d) st[d] = tmax(tmax(-(sc * gp[d]), sc * gm[d]), real(0));
   if (!(fabs(c) > real(1.e-5))) return real(0);
   real n2 = st[0] * st[0] + st[1] * st[1] + st[2] * st[2];
   return sgn0 - sgn0 * rsqrt_<FAST>(n2);
