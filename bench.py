@@ -444,13 +444,16 @@ def batch_leg(tmax=20.0, seed=1004, sequential=True):
         for _, _, rec, _ in items:
             lib.lsmpqs_release(C.byref(rec))
         out = {"boxes": n, "voxels": int(sum(np.prod(b["dims"]) for b in boxes)),
-               "batched_wall_s": round(wall, 3), "batched_gpu_s": round(st.batched_ms * 1e-3, 3),
+               "batched_wall_s": round(wall, 3),
+               "voxel_updates": vu,
+               "voxel_updates_per_s_wall": round(vu / wall, 1),
                "batched_steps": int(st.batched_steps), "box_steps": int(st.box_steps),
                "brick_depth": int(st.brick_depth), "workgroups": int(st.workgroups),
-               "voxel_updates": vu,
-               "voxel_updates_per_s_batched_gpu": round(vu / (st.batched_ms * 1e-3), 1),
                "batched_outer_steps": int(st.batched_outer), "box_outer_steps": int(st.box_outer),
-               "batched_outer_gpu_s": round(st.batched_outer_ms * 1e-3, 3)}
+               # summed event intervals of the rounds; the two box groups'
+               # rounds run on their own streams and overlap on the GPU
+               "inner_round_gpu_s": round(st.batched_ms * 1e-3, 3),
+               "outer_round_gpu_s": round(st.batched_outer_ms * 1e-3, 3)}
         if sequential:
             items2 = records()
             w0 = time.perf_counter()

@@ -266,8 +266,9 @@ unsigned lsmpqs_set_quirks(unsigned mask);
    (subvolumes of any size, each with its own QSS_DataArrays, Grid and
    Options, hence its own curvature schedule).  No counterpart in the
    reference, which runs one box per process.  Each box runs the single-box
-   driver on a host thread of its own; the inner time steps of all boxes
-   waiting for one run together: a batched time-step probe, per RK stage one
+   driver on a host thread of its own; the boxes form two groups (first and
+   second half), each with its own stream, and the inner time steps of all
+   boxes of a group waiting for one run together: a batched time-step probe, per RK stage one
    persistent stage kernel fed by a work queue of 64 x 8 x zc voxel bricks of
    all boxes, and a batched ghost extrapolation, captured once as a HIP graph
    and replayed.  zc and the number of persistent workgroups are picked on
@@ -285,11 +286,12 @@ typedef struct {
   long long box_steps;        /* inner steps summed over the boxes */
   int brick_depth;            /* z planes per brick chosen by the tuner */
   int workgroups;             /* persistent workgroups chosen by the tuner */
-  double batched_ms;          /* GPU time of the batched steps */
+  double batched_ms;          /* GPU time of the batched steps (sum over the
+                                 rounds; the two groups' rounds may overlap) */
   double autotune_best_ms;    /* fastest first-stage launch seen while tuning */
   long long batched_outer;    /* batched outer steps (launch sequences) */
   long long box_outer;        /* outer steps they served, summed over the boxes */
-  double batched_outer_ms;    /* GPU time of the batched outer steps */
+  double batched_outer_ms;    /* GPU time of the batched outer steps (likewise) */
 } lsmpqs_batch_stats;
 int lsmpqs_batch_model3d(int n, Options **options, QSS_DataArrays **p, Grid **g, FILE **fp_out,
                          int drain, lsmpqs_batch_stats *stats);

@@ -63,10 +63,10 @@ class Batch final : public BatchHook {
  public:
   // Boxes 0 .. n/2 - 1 form group 0, the rest group 1.  A round serves the
   // parked requests of one group once each of its live boxes is parked or
-  // away; the two groups' rounds alternate on the batch stream, so that one
-  // group's boxes wake up, run their host logic and park again while the
-  // other group's round runs on the GPU (with one group, every round waited
-  // for that turnaround with the GPU idle).
+  // away; each group's rounds run on its own stream, so that one group's
+  // boxes wake up, run their host logic and park again while the other
+  // group's round runs on the GPU (with one group, every round waited for
+  // that turnaround with the GPU idle).
   explicit Batch(int nbox) : nmax_(nbox) {
     live_[0] = (nbox + 1) / 2;
     live_[1] = nbox - live_[0];
@@ -235,9 +235,10 @@ class Batch final : public BatchHook {
   // calling thread: serve rounds until every box has finished.  A round
   // runs the requests of every parked box of a group once each live box of
   // that group is parked or away (in outer-step work, which then overlaps
-  // the round on the GPU).  A group's next round is launched while the
-  // other group's runs; its results are delivered once the stream has
-  // passed its last event (rounds complete in launch order: one stream).
+  // the round on the GPU).  Each group has its own stream, so one group's
+  // round runs while the other's boxes turn around, and the two rounds'
+  // kernels may overlap on the GPU (each fills the other's tails); a
+  // round's results are delivered once its stream has passed its end event.
   // Which round serves a box's request changes nothing in its arithmetic.
   void run() {
     std::deque<int> inflight;  // lanes with a round on the stream, oldest first
@@ -289,25 +290,35 @@ class Batch final : public BatchHook {
         inflight.push_back(g);
         if (!sync && !L.failed) continue;  // perhaps the other group is ready too
       }
-      // deliver the oldest round once the stream has passed it (wait for it
-      // when both lanes are busy: nothing else can be launched meanwhile)
+      // deliver every round whose stream has passed its end event (the
+      // groups' streams are independent, so in any order); with both groups
+      // in flight and neither done, or one round at a time, wait for one
       if (inflight.empty()) continue;
-      Lane& O = lanes_[inflight.front()];
-      bool done = O.failed;
-      if (!done) {
-        if (sync || inflight.size() == 2) {
-          const hipError_t e = hipEventSynchronize(O.ev_done);
-          if (e != hipSuccess) note_failure(e, O);
-          done = true;
-        } else {
+      bool any = false;
+      for (size_t q = 0; q < inflight.size();) {
+        Lane& O = lanes_[inflight[q]];
+        bool done = O.failed;
+        if (!done) {
           const hipError_t e = hipEventQuery(O.ev_done);
           if (e == hipSuccess) done = true;
           else if (e != hipErrorNotReady) { note_failure(e, O); done = true; }
         }
+        if (done) {
+          inflight.erase(inflight.begin() + q);
+          finish(O);
+          any = true;
+        } else {
+          ++q;
+        }
       }
-      if (done) {
+      if (!any && sync) {
+        Lane& O = lanes_[inflight.front()];
+        const hipError_t e = hipEventSynchronize(O.ev_done);
+        if (e != hipSuccess) note_failure(e, O);
         inflight.pop_front();
         finish(O);
+      } else if (!any && inflight.size() == 2) {
+        std::this_thread::sleep_for(std::chrono::microseconds(20));
       }
     }
   }
@@ -333,6 +344,7 @@ class Batch final : public BatchHook {
     OuterJob *d_jobs = nullptr, *h_jobs = nullptr;
     int *d_tab = nullptr, *h_tab = nullptr;
     double *d_redout = nullptr, *h_redout = nullptr;
+    hipStream_t st = nullptr;  // the group's own stream (default priority)
     hipEvent_t ev_in0 = nullptr, ev_in1 = nullptr, ev_out0 = nullptr, ev_out1 = nullptr,
                ev_done = nullptr;
     std::vector<Request*> round, inner, outer;
@@ -348,11 +360,11 @@ class Batch final : public BatchHook {
   }
 
   void ensure_stream() {
-    if (st_) return;
-    BCHECK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+    if (ev0_) return;
     BCHECK(hipEventCreate(&ev0_));
     BCHECK(hipEventCreate(&ev1_));
     for (Lane& L : lanes_) {
+      BCHECK(hipStreamCreateWithFlags(&L.st, hipStreamNonBlocking));
       BCHECK(hipEventCreate(&L.ev_in0));
       BCHECK(hipEventCreate(&L.ev_in1));
       BCHECK(hipEventCreate(&L.ev_out0));
@@ -396,9 +408,10 @@ class Batch final : public BatchHook {
   }
 
   void release() {
-    // rounds still on the stream (after a failure) finish before their
+    // rounds still on the streams (after a failure) finish before their
     // buffers go
-    if (st_) (void)hipStreamSynchronize(st_);
+    for (Lane& L : lanes_)
+      if (L.st) (void)hipStreamSynchronize(L.st);
     for (Lane& L : lanes_) {
       if (L.exec) (void)hipGraphExecDestroy(L.exec);
       if (L.graph) (void)hipGraphDestroy(L.graph);
@@ -413,6 +426,7 @@ class Batch final : public BatchHook {
       hipEvent_t ev[] = {L.ev_in0, L.ev_in1, L.ev_out0, L.ev_out1, L.ev_done};
       for (hipEvent_t e : ev)
         if (e) (void)hipEventDestroy(e);
+      if (L.st) (void)hipStreamDestroy(L.st);
     }
     for (hipEvent_t e : pass_ev_)
       if (e) (void)hipEventDestroy(e);
@@ -420,7 +434,6 @@ class Batch final : public BatchHook {
       if (e) (void)hipEventDestroy(e);
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
-    if (st_) (void)hipStreamDestroy(st_);
   }
 
   // records of the round's boxes for brick depth zc, copied to the device
@@ -442,31 +455,31 @@ class Batch final : public BatchHook {
       L.h_brick[i + 1] = L.h_brick[i] + bricks;
       L.h_ghost[i + 1] = L.h_ghost[i] + ghost_count(a.probe.bx);
     }
-    BCHECK(hipMemcpyAsync(L.d_recs, L.h_recs, sizeof(BatchRec) * n, hipMemcpyHostToDevice, st_));
-    BCHECK(hipMemcpyAsync(L.d_count, L.h_count, sizeof(int), hipMemcpyHostToDevice, st_));
-    BCHECK(hipMemcpyAsync(L.d_brick, L.h_brick, sizeof(int) * (n + 1), hipMemcpyHostToDevice, st_));
+    BCHECK(hipMemcpyAsync(L.d_recs, L.h_recs, sizeof(BatchRec) * n, hipMemcpyHostToDevice, L.st));
+    BCHECK(hipMemcpyAsync(L.d_count, L.h_count, sizeof(int), hipMemcpyHostToDevice, L.st));
+    BCHECK(hipMemcpyAsync(L.d_brick, L.h_brick, sizeof(int) * (n + 1), hipMemcpyHostToDevice, L.st));
     BCHECK(hipMemcpyAsync(L.d_ghost, L.h_ghost, sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice,
-                          st_));
+                          L.st));
   }
 
   // timed: events between the parts (host-time tracing; not captured)
   void enqueue_step(Lane& L, bool timed = false) {
-    if (timed) mark_inner(0);
-    BCHECK(launch_batch_reset(L.d_counters, kCounters, st_));
-    BCHECK(launch_probe_batch(L.view, scheme_, nmax_, st_));
-    BCHECK(hipMemcpyAsync(L.h_dts, L.d_dts, sizeof(real) * nmax_, hipMemcpyDeviceToHost, st_));
+    if (timed) mark_inner(0, L.st);
+    BCHECK(launch_batch_reset(L.d_counters, kCounters, L.st));
+    BCHECK(launch_probe_batch(L.view, scheme_, nmax_, L.st));
+    BCHECK(hipMemcpyAsync(L.h_dts, L.d_dts, sizeof(real) * nmax_, hipMemcpyDeviceToHost, L.st));
     for (int k = 0; k < nstage_; ++k) {
       const bool fin = k == nstage_ - 1;
-      if (timed) mark_inner(1 + 2 * k);
-      BCHECK(launch_stage_batch(L.view, k, scheme_, var_, k + 1, fin, fast_, grid_, st_));
-      if (timed) mark_inner(2 + 2 * k);
-      BCHECK(launch_bc_batch(L.view, k, fin, bc_grid_, st_));
+      if (timed) mark_inner(1 + 2 * k, L.st);
+      BCHECK(launch_stage_batch(L.view, k, scheme_, var_, k + 1, fin, fast_, grid_, L.st));
+      if (timed) mark_inner(2 + 2 * k, L.st);
+      BCHECK(launch_bc_batch(L.view, k, fin, bc_grid_, L.st));
     }
-    if (timed) mark_inner(1 + 2 * nstage_);
+    if (timed) mark_inner(1 + 2 * nstage_, L.st);
   }
-  void mark_inner(int q) {
+  void mark_inner(int q, hipStream_t st) {
     if (!inner_ev_[q]) BCHECK(hipEventCreate(&inner_ev_[q]));
-    BCHECK(hipEventRecord(inner_ev_[q], st_));
+    BCHECK(hipEventRecord(inner_ev_[q], st));
   }
 
   // brick depth and workgroup count: fastest first stage over this round
@@ -475,17 +488,17 @@ class Batch final : public BatchHook {
     const int per_cu[] = {1, 2, 4};
     float best = 1e30f;
     upload(L, round, depths[0]);
-    BCHECK(launch_probe_batch(L.view, scheme_, nmax_, st_));  // stage 1 reads this step's dt
+    BCHECK(launch_probe_batch(L.view, scheme_, nmax_, L.st));  // stage 1 reads this step's dt
     for (int zc : depths) {
       upload(L, round, zc);
       for (int pc : per_cu) {
         const int grid = pc * cus_;
         float t = 1e30f;
         for (int rep = 0; rep < 2; ++rep) {
-          BCHECK(launch_batch_reset(L.d_counters, kCounters, st_));
-          BCHECK(hipEventRecord(ev0_, st_));
-          BCHECK(launch_stage_batch(L.view, 0, scheme_, var_, 1, nstage_ == 1, fast_, grid, st_));
-          BCHECK(hipEventRecord(ev1_, st_));
+          BCHECK(launch_batch_reset(L.d_counters, kCounters, L.st));
+          BCHECK(hipEventRecord(ev0_, L.st));
+          BCHECK(launch_stage_batch(L.view, 0, scheme_, var_, 1, nstage_ == 1, fast_, grid, L.st));
+          BCHECK(hipEventRecord(ev1_, L.st));
           BCHECK(hipEventSynchronize(ev1_));
           float ms = 0;
           BCHECK(hipEventElapsedTime(&ms, ev0_, ev1_));
@@ -527,17 +540,17 @@ class Batch final : public BatchHook {
   }
 
   void capture(Lane& L) {
-    BCHECK(hipStreamSynchronize(st_));  // once per lane: nothing of the other lane pending
-    BCHECK(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
+    BCHECK(hipStreamSynchronize(L.st));  // once per group: its uploads and tuning done
+    BCHECK(hipStreamBeginCapture(L.st, hipStreamCaptureModeThreadLocal));
     try {
       enqueue_step(L);
     } catch (...) {
       hipGraph_t g = nullptr;
-      (void)hipStreamEndCapture(st_, &g);
+      (void)hipStreamEndCapture(L.st, &g);
       if (g) (void)hipGraphDestroy(g);
       throw;
     }
-    BCHECK(hipStreamEndCapture(st_, &L.graph));
+    BCHECK(hipStreamEndCapture(L.st, &L.graph));
     BCHECK(hipGraphInstantiate(&L.exec, L.graph, nullptr, nullptr, 0));
   }
 
@@ -551,7 +564,7 @@ class Batch final : public BatchHook {
     ensure_stream();
     if (!L.outer.empty()) launch_outer(L);
     if (!L.inner.empty()) launch_inner(L);
-    BCHECK(hipEventRecord(L.ev_done, st_));
+    BCHECK(hipEventRecord(L.ev_done, L.st));
   }
 
   // results of lane L's round (its stream work is complete, or it failed)
@@ -660,29 +673,29 @@ class Batch final : public BatchHook {
     const Multi stream = table(7, [](const OuterJob& j) { return stream_pass_blocks(j.ta.bx.size()); });
     const Multi ghosts = table(8, [](const OuterJob& j) { return bc_pass_blocks(j.ta.bx); });
     OuterJob* jobs = L.d_jobs;
-    BCHECK(hipMemcpyAsync(jobs, L.h_jobs, sizeof(OuterJob) * n, hipMemcpyHostToDevice, st_));
+    BCHECK(hipMemcpyAsync(jobs, L.h_jobs, sizeof(OuterJob) * n, hipMemcpyHostToDevice, L.st));
     BCHECK(hipMemcpyAsync(L.d_tab, L.h_tab, sizeof(int) * kTables * (nmax_ + 1),
-                          hipMemcpyHostToDevice, st_));
-    BCHECK(hipEventRecord(L.ev_out0, st_));
+                          hipMemcpyHostToDevice, L.st));
+    BCHECK(hipEventRecord(L.ev_out0, L.st));
     const bool tr = trace_on();
-    if (tr) mark_pass(0);
-    BCHECK(launch_trap_multi(jobs, trap, force, one, tiles, jump, conn, st_));
-    if (tr) mark_pass(1);
+    if (tr) mark_pass(0, L.st);
+    BCHECK(launch_trap_multi(jobs, trap, force, one, tiles, jump, conn, L.st));
+    if (tr) mark_pass(1, L.st);
     for (int pass = 0; pass < 2; ++pass) {
-      BCHECK(launch_reinit_multi(jobs, pass, rows, fast, st_));
-      BCHECK(launch_bc_multi(jobs, pass, ghosts, st_));
+      BCHECK(launch_reinit_multi(jobs, pass, rows, fast, L.st));
+      BCHECK(launch_bc_multi(jobs, pass, ghosts, L.st));
     }
-    if (tr) mark_pass(2);
-    if (velocities) BCHECK(launch_trap_velocity_multi(jobs, rows, st_));
-    BCHECK(launch_merge_floor_multi(jobs, stream, st_));
-    if (tr) mark_pass(3);
-    BCHECK(launch_reinit_reduce_multi(jobs, red, one, fast, st_));
-    BCHECK(launch_bc_multi(jobs, 2, ghosts, st_));
-    BCHECK(launch_gather_red(jobs, n, L.d_redout, st_));
-    if (tr) mark_pass(4);
-    BCHECK(hipEventRecord(L.ev_out1, st_));
+    if (tr) mark_pass(2, L.st);
+    if (velocities) BCHECK(launch_trap_velocity_multi(jobs, rows, L.st));
+    BCHECK(launch_merge_floor_multi(jobs, stream, L.st));
+    if (tr) mark_pass(3, L.st);
+    BCHECK(launch_reinit_reduce_multi(jobs, red, one, fast, L.st));
+    BCHECK(launch_bc_multi(jobs, 2, ghosts, L.st));
+    BCHECK(launch_gather_red(jobs, n, L.d_redout, L.st));
+    if (tr) mark_pass(4, L.st);
+    BCHECK(hipEventRecord(L.ev_out1, L.st));
     BCHECK(hipMemcpyAsync(L.h_redout, L.d_redout, sizeof(double) * 2 * n, hipMemcpyDeviceToHost,
-                          st_));
+                          L.st));
   }
 
   // per-pass timing of the batched outer steps (LSMPQS_BATCH_TRACE=1)
@@ -690,9 +703,9 @@ class Batch final : public BatchHook {
     const char* e = getenv("LSMPQS_BATCH_TRACE");
     return e && *e && atoi(e) != 0;
   }
-  void mark_pass(int q) {
+  void mark_pass(int q, hipStream_t st) {
     if (!pass_ev_[q]) BCHECK(hipEventCreate(&pass_ev_[q]));
-    BCHECK(hipEventRecord(pass_ev_[q], st_));
+    BCHECK(hipEventRecord(pass_ev_[q], st));
   }
 
  public:
@@ -717,10 +730,10 @@ class Batch final : public BatchHook {
     const int zc = round_depth(round);
     upload(L, round, zc);
     if (!L.exec) capture(L);
-    BCHECK(hipEventRecord(L.ev_in0, st_));
+    BCHECK(hipEventRecord(L.ev_in0, L.st));
     if (trace_on()) enqueue_step(L, true);  // the same launches, uncaptured, with events between them
-    else BCHECK(hipGraphLaunch(L.exec, st_));
-    BCHECK(hipEventRecord(L.ev_in1, st_));
+    else BCHECK(hipGraphLaunch(L.exec, L.st));
+    BCHECK(hipEventRecord(L.ev_in1, L.st));
   }
 
   std::mutex mu_, slot_mu_;
@@ -741,7 +754,6 @@ class Batch final : public BatchHook {
   bool var_ = false, fast_ = false;
   int cus_ = 256, zc_ = 0, grid_ = 0, bc_grid_ = 1024;
   double ratio_ = 0;  // bricks per persistent workgroup of the tuned round
-  hipStream_t st_ = nullptr;
   bool configured_ = false;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;  // autotuning
 };
