@@ -266,9 +266,10 @@ unsigned lsmpqs_set_quirks(unsigned mask);
    (subvolumes of any size, each with its own QSS_DataArrays, Grid and
    Options, hence its own curvature schedule).  No counterpart in the
    reference, which runs one box per process.  Each box runs the single-box
-   driver on a host thread of its own; the boxes form two groups (first and
-   second half), each with its own stream, and the inner time steps of all
-   boxes of a group waiting for one run together: a batched time-step probe, per RK stage one
+   driver on a host thread of its own; the boxes form up to four groups of
+   consecutive indices (LSMPQS_BATCH_GROUPS, default 4), each with its own
+   stream, and the inner time steps of all boxes of a group waiting for one
+   run together: a batched time-step probe, per RK stage one
    persistent stage kernel fed by a work queue of 64 x 8 x zc voxel bricks of
    all boxes, and a batched ghost extrapolation, captured once as a HIP graph
    and replayed.  zc and the number of persistent workgroups are picked on
@@ -287,7 +288,7 @@ typedef struct {
   int brick_depth;            /* z planes per brick chosen by the tuner */
   int workgroups;             /* persistent workgroups chosen by the tuner */
   double batched_ms;          /* GPU time of the batched steps (sum over the
-                                 rounds; the two groups' rounds may overlap) */
+                                 rounds; the groups' rounds may overlap) */
   double autotune_best_ms;    /* fastest first-stage launch seen while tuning */
   long long batched_outer;    /* batched outer steps (launch sequences) */
   long long box_outer;        /* outer steps they served, summed over the boxes */

@@ -61,18 +61,19 @@ constexpr int kTables = 9;  // block-prefix tables of a batched outer step
 
 class Batch final : public BatchHook {
  public:
-  // Boxes 0 .. n/2 - 1 form group 0, the rest group 1.  A round serves the
+  // The boxes form ngroups groups of consecutive indices.  A round serves the
   // parked requests of one group once each of its live boxes is parked or
   // away; each group's rounds run on its own stream, so that one group's
-  // boxes wake up, run their host logic and park again while the other
-  // group's round runs on the GPU (with one group, every round waited for
-  // that turnaround with the GPU idle).
-  explicit Batch(int nbox) : nmax_(nbox) {
-    live_[0] = (nbox + 1) / 2;
-    live_[1] = nbox - live_[0];
+  // boxes wake up, run their host logic and park again while other groups'
+  // rounds run on the GPU (with one group, every round waited for that
+  // turnaround with the GPU idle), and the groups' kernels overlap.
+  Batch(int nbox, int ngroups) : nmax_(nbox) {
+    ng_ = std::max(1, std::min({ngroups, kMaxGroups, nbox}));
+    for (int i = 0; i < nbox; ++i) live_[group_of(i, nbox, ng_)] += 1;
   }
   ~Batch() override { release(); }
-  static int group_of(int box, int nbox) { return box < (nbox + 1) / 2 ? 0 : 1; }
+  static int group_of(int box, int nbox, int ng) { return (int)((int64_t)box * ng / nbox); }
+  int groups() const { return ng_; }
 
   // Box threads run their own work (set-up, outer steps) only while holding
   // one of `slots` run slots: a box takes a slot when it starts and after
@@ -222,7 +223,7 @@ class Batch final : public BatchHook {
       away_[g] -= 1;
     }
     live_[g] -= 1;
-    if (ready_locked(g) || live_[0] + live_[1] == 0) cv_sched_.notify_one();
+    if (ready_locked(g) || live_total() == 0) cv_sched_.notify_one();
   }
 
   // (mu_ held) a round of group g can start: every live box of g is parked
@@ -230,6 +231,11 @@ class Batch final : public BatchHook {
   // not at every park.
   bool ready_locked(int g) const {
     return !parked_[g].empty() && (int)parked_[g].size() + away_[g] == live_[g];
+  }
+  int live_total() const {
+    int t = 0;
+    for (int g = 0; g < ng_; ++g) t += live_[g];
+    return t;
   }
 
   // calling thread: serve rounds until every box has finished.  A round
@@ -250,16 +256,16 @@ class Batch final : public BatchHook {
       {
         std::unique_lock<std::mutex> lk(mu_);
         auto pick = [&] {
-          for (int q = 0; q < 2; ++q) {
-            const int c = (next_ + q) & 1;
+          for (int q = 0; q < ng_; ++q) {
+            const int c = (next_ + q) % ng_;
             if (!lanes_[c].busy && ready_locked(c)) return c;
           }
           return -1;
         };
         if (inflight.empty()) {
-          cv_sched_.wait(lk, [&] { return live_[0] + live_[1] == 0 || pick() >= 0; });
-          if (live_[0] + live_[1] == 0) break;
-        } else if (inflight.size() < 2) {
+          cv_sched_.wait(lk, [&] { return live_total() == 0 || pick() >= 0; });
+          if (live_total() == 0) break;
+        } else if ((int)inflight.size() < ng_) {
           // the other group's boxes are in their host turnaround: take their
           // round as soon as they are parked, else look at the stream again
           cv_sched_.wait_for(lk, std::chrono::microseconds(40), [&] { return pick() >= 0; });
@@ -268,7 +274,7 @@ class Batch final : public BatchHook {
         if (g >= 0) {
           round.swap(parked_[g]);
           lanes_[g].busy = true;
-          next_ = g ^ 1;
+          next_ = (g + 1) % ng_;
         }
       }
       if (g >= 0) {
@@ -317,7 +323,7 @@ class Batch final : public BatchHook {
         if (e != hipSuccess) note_failure(e, O);
         inflight.pop_front();
         finish(O);
-      } else if (!any && inflight.size() == 2) {
+      } else if (!any && (int)inflight.size() == ng_) {
         std::this_thread::sleep_for(std::chrono::microseconds(20));
       }
     }
@@ -350,7 +356,9 @@ class Batch final : public BatchHook {
     std::vector<Request*> round, inner, outer;
     bool busy = false, failed = false;
   };
-  Lane lanes_[2];
+  static constexpr int kMaxGroups = 4;
+  Lane lanes_[kMaxGroups];
+  int ng_ = 1;
   int next_ = 0;  // lane looked at first
 
   void note_failure(hipError_t e, Lane& L) {
@@ -363,7 +371,8 @@ class Batch final : public BatchHook {
     if (ev0_) return;
     BCHECK(hipEventCreate(&ev0_));
     BCHECK(hipEventCreate(&ev1_));
-    for (Lane& L : lanes_) {
+    for (int q = 0; q < ng_; ++q) {
+      Lane& L = lanes_[q];
       BCHECK(hipStreamCreateWithFlags(&L.st, hipStreamNonBlocking));
       BCHECK(hipEventCreate(&L.ev_in0));
       BCHECK(hipEventCreate(&L.ev_in1));
@@ -380,7 +389,8 @@ class Batch final : public BatchHook {
     fast_ = sizeof(real) == 4 && !(quirks() & EXACT_FP32);
     nstage_ = order_ <= 1 ? 1 : (order_ == 2 ? 2 : 3);
     ensure_stream();
-    for (Lane& L : lanes_) {
+    for (int q = 0; q < ng_; ++q) {
+      Lane& L = lanes_[q];
       BCHECK(hipMalloc((void**)&L.d_recs, sizeof(BatchRec) * nmax_));
       BCHECK(hipMalloc((void**)&L.d_count, sizeof(int) * 4));
       BCHECK(hipMalloc((void**)&L.d_brick, sizeof(int) * (nmax_ + 1)));
@@ -739,9 +749,9 @@ class Batch final : public BatchHook {
   std::mutex mu_, slot_mu_;
   std::condition_variable cv_sched_, cv_slot_;
   int slots_ = 1 << 30;
-  std::vector<Request*> parked_[2];
-  int live_[2] = {0, 0};
-  int away_[2] = {0, 0};  // live boxes between leaving an inner loop and their next request
+  std::vector<Request*> parked_[kMaxGroups];
+  int live_[kMaxGroups] = {0, 0, 0, 0};
+  int away_[kMaxGroups] = {0, 0, 0, 0};  // live boxes between leaving an inner loop and their next request
   static bool& box_away() {
     static thread_local bool a = false;
     return a;
@@ -780,7 +790,15 @@ extern "C" int lsmpqs_batch_model3d(int n, Options** options, QSS_DataArrays** p
     set_error("lsmpqs_batch_model3d: no HIP device");
     return -2;
   }
-  Batch batch(n);
+  // box groups, each with its own stream and rounds (LSMPQS_BATCH_GROUPS,
+  // 1 - 4, default 4: 64 boxes of a 768^3 pack take 1.96 s with one group,
+  // 1.70 - 1.75 s with two, 1.66 - 1.71 s with three, 1.65 - 1.70 s with four)
+  int groups = 4;
+  {
+    const char* e = getenv("LSMPQS_BATCH_GROUPS");
+    if (e && *e) groups = atoi(e);
+  }
+  Batch batch(n, groups);
   // boxes doing host-driven work at once (LSMPQS_BATCH_THREADS, default 16,
   // a GPU's share of host cores; 0: all).  64 boxes of a 768^3 pack: the
   // batched drainage takes 2.09 s with 64 active box threads, 2.10 s with
@@ -796,7 +814,7 @@ extern "C" int lsmpqs_batch_model3d(int n, Options** options, QSS_DataArrays** p
   threads.reserve(n);
   for (int i = 0; i < n; ++i) {
     threads.emplace_back([&, i] {
-      Batch::group() = Batch::group_of(i, n);
+      Batch::group() = Batch::group_of(i, n, batch.groups());
       Batch::trace().start = Batch::now_s();
       batch.acquire_slot();
       try {
