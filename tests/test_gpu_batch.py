@@ -81,6 +81,36 @@ def test_batched_boxes_match_single_runs(dt, drain, theta_deg):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("groups", [1, 2, 3])
+def test_box_groups_do_not_change_results(groups, monkeypatch):
+    """The batch runs its boxes in groups of consecutive indices, each with its
+    own stream and rounds (LSMPQS_BATCH_GROUPS, default 4); any grouping gives
+    the single runs' results bit for bit (FP32 drainage on the contact-angle
+    path)."""
+    dt = np.float32
+    print(f"case: batch fp32 drain theta=45 groups={groups} boxes={len(SHAPES)}", flush=True)
+    monkeypatch.setenv("LSMPQS_BATCH_GROUPS", str(groups))
+    lib = hiplib.load(dt)
+    lib.lsmpqs_set_dt_slabs(4)
+    os.environ["LSMPQS_NO_STEP_FILES"] = "1"
+    single, batched = boxes(dt, 45.0), boxes(dt, 45.0)
+    run_single(lib, single, True)
+    n = len(batched)
+    opts = (C.c_void_p * n)(*[C.addressof(o) for _, o, _ in batched])
+    recs = (C.c_void_p * n)(*[C.addressof(A.rec) for _, _, A in batched])
+    grids = (C.c_void_p * n)(*[C.addressof(g) for g, _, _ in batched])
+    st = hiplib.BatchStats()
+    rc = lib.lsmpqs_batch_model3d(n, opts, recs, grids, None, 1, C.byref(st))
+    assert rc == 0, hiplib.last_error(lib)
+    for _, _, A in batched:
+        lib.lsmpqs_release(A.ptr())
+    assert st.boxes == n and st.box_steps >= st.batched_steps > 0
+    for (g1, o1, A1), (g2, o2, A2) in zip(single, batched):
+        for f in ("phi", "mask_w", "mask_nw"):
+            assert np.array_equal(A1[f], A2[f]), f
+
+
+@pytest.mark.gpu
 def test_sixty_four_subvolumes_of_a_768_pack():
     """The full batched case: 64 subvolumes (dims uniform in {48..192} per
     axis) of one seeded 768^3 pack (radius U[6,12], porosity 0.36, seed 1004),
