@@ -19,6 +19,7 @@
 #include <math.h>
 #include <stdio.h>
 #include <algorithm>
+#include <chrono>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -689,6 +690,17 @@ extern "C" int lsmpqs_adjoint_run(Options* o, QSS_DataArrays* p, Grid* g, const 
       throw std::runtime_error("lsmpqs_adjoint_run: options, arrays (phi, mask), grid, theta and args required");
     if (args->nsteps < 1 || args->checkpoint_every < 1)
       throw std::runtime_error("lsmpqs_adjoint_run: nsteps and checkpoint_every must be >= 1");
+    // LSMPQS_ADJOINT_TRACE=1: host-time breakdown on stderr
+    const char* te = getenv("LSMPQS_ADJOINT_TRACE");
+    const bool tr = te && *te && atoi(te) != 0;
+    const auto t_start = std::chrono::steady_clock::now();
+    std::vector<std::pair<const char*, double>> marks;
+    auto mark = [&](const char* what, hipStream_t st) {
+      if (!tr) return;
+      (void)hipStreamSynchronize(st);
+      marks.emplace_back(what, std::chrono::duration<double>(std::chrono::steady_clock::now() -
+                                                              t_start).count());
+    };
     Session* s = session_for(p, g);
     ensure_var(s);
     ensure_grad(s);
@@ -755,6 +767,7 @@ extern "C" int lsmpqs_adjoint_run(Options* o, QSS_DataArrays* p, Grid* g, const 
     A.cap = args->phi_cap > 0 ? (real)args->phi_cap : real(6) * g->dx[0];
     args->cap_used = (double)A.cap;
     ACHECK(hipMemcpyAsync(s->dt, &dt, sizeof(real), hipMemcpyHostToDevice, s->st));
+    mark("set-up", s->st);
     // forward with checkpoints every k steps
     const int N = args->nsteps, K = std::min(args->checkpoint_every, args->nsteps);
     const int nseg = (N + K - 1) / K;
@@ -795,6 +808,7 @@ extern "C" int lsmpqs_adjoint_run(Options* o, QSS_DataArrays* p, Grid* g, const 
       ACHECK(hipStreamSynchronize(s->st));
       return 0;
     }
+    mark("forward and loss", s->st);
     // reverse
     A.zero(A.lam);
     hipLaunchKernelGGL(adj_loss_kernel, dim3(kBlocks), dim3(256), 0, s->st, bx, A.lam, u,
@@ -841,6 +855,7 @@ extern "C" int lsmpqs_adjoint_run(Options* o, QSS_DataArrays* p, Grid* g, const 
       }
     }
     A.u1 = own[0]; A.u2 = own[1]; A.u3 = own[2]; A.pre = own[3];
+    mark("reverse", s->st);
     // theta and b gradients
     double* part = nullptr;
     ACHECK(hipMalloc((void**)&part, sizeof(double) * kBlocks));
@@ -858,18 +873,28 @@ extern "C" int lsmpqs_adjoint_run(Options* o, QSS_DataArrays* p, Grid* g, const 
     for (double v : h) db += v;
     args->dloss_db = db;
     {
-      // ghost entries carry no gradient
+      // ghost entries carry no gradient: zero them, copy the fill rows
+      const int nx = g->grid_dims_ghostbox[0], ny = g->grid_dims_ghostbox[1];
+      const int nz = g->grid_dims_ghostbox[2];
       std::vector<real> tmp(n);
       ACHECK(hipMemcpy(tmp.data(), d_theta, bytes, hipMemcpyDeviceToHost));
-      const int nx = g->grid_dims_ghostbox[0], ny = g->grid_dims_ghostbox[1];
-      for (int64_t l = 0; l < n; ++l) {
-        int i = (int)(l % nx), j = (int)((l / nx) % ny), k = (int)(l / ((int64_t)nx * ny));
-        bool fillv = i >= g->ilo_fb && i <= g->ihi_fb && j >= g->jlo_fb && j <= g->jhi_fb &&
-                     k >= g->klo_fb && k <= g->khi_fb;
-        dloss_dtheta[l] = fillv ? tmp[l] : (QSSLIB_REAL)0;
-      }
+      std::fill(dloss_dtheta, dloss_dtheta + n, (QSSLIB_REAL)0);
+      for (int k = std::max(0, g->klo_fb); k <= std::min(nz - 1, g->khi_fb); ++k)
+        for (int j = std::max(0, g->jlo_fb); j <= std::min(ny - 1, g->jhi_fb); ++j) {
+          const int64_t row = ((int64_t)k * ny + j) * nx;
+          for (int i = std::max(0, g->ilo_fb); i <= std::min(nx - 1, g->ihi_fb); ++i)
+            dloss_dtheta[row + i] = tmp[row + i];
+        }
     }
     ACHECK(hipStreamSynchronize(s->st));
+    mark("gradients to the host", s->st);
+    if (tr) {
+      double prev = 0;
+      for (const auto& m : marks) {
+        fprintf(stderr, "lsmpqs adjoint trace: %s %.4f s\n", m.first, m.second - prev);
+        prev = m.second;
+      }
+    }
   } catch (const std::exception& e) {
     set_error(e.what());
     fprintf(stderr, "lsmpqs_hip: %s\n", e.what());
