@@ -425,29 +425,34 @@ __global__ void adj_bc_kernel(Box bx, real* lam, const real* src, const real* fl
 
 // ---------------------------------------------------------------------------
 // Elementwise pieces over the fill box.
-__global__ void adj_axpy_kernel(Box bx, real* dst, const real* src, real alpha) {
-  FillIter it(bx);
-  for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < it.n;
-       f += (int64_t)gridDim.x * blockDim.x) {
-    int i, j, k;
-    it.ijk(f, i, j, k);
-    const int64_t idx = bx.at(i, j, k);
-    dst[idx] += alpha * src[idx];
-  }
+// One RK stage's combination transposed, over the whole ghost box row by
+// row: a += alpha lk and b += beta lk on the fill box, with "=" instead of
+// "+=" for an accumulator's first term of the step (its ghost entries are
+// then zeroed too), which replaces zeroing it first; a may be null.
+__global__ void adj_combine_kernel(Box bx, real* a, real alpha, int a_assign, real* b, real beta,
+                                   int b_assign, const real* lk) {
+  for_rows<false>(bx, [&](int64_t idx, int i, int j, int k) {
+    const real v = bx.in_fill(i, j, k) ? lk[idx] : real(0);
+    if (a) {
+      const real x = alpha * v;
+      a[idx] = a_assign ? x : a[idx] + x;
+    }
+    const real y = beta * v;
+    b[idx] = b_assign ? y : b[idx] + y;
+  });
 }
-// lam = 0 where the imposition took the floor (result equal to the floor)
-__global__ void adj_impose_kernel(Box bx, real* lam, const real* out, const real* floor) {
-  FillIter it(bx);
-  for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < it.n;
-       f += (int64_t)gridDim.x * blockDim.x) {
-    int i, j, k;
-    it.ijk(f, i, j, k);
-    const int64_t idx = bx.at(i, j, k);
-    if (out[idx] == floor[idx]) lam[idx] = 0;
-  }
+// The last stage's cap and imposition transposed, over the whole ghost box
+// row by row: lam = 0 where the cap was taken, and on the fill box where
+// the imposition took the floor (result equal to the floor).
+__global__ void adj_cap_impose_kernel(Box bx, real* lam, const real* out, real cap,
+                                      const real* floor) {
+  for_rows<false>(bx, [&](int64_t idx, int i, int j, int k) {
+    const real o = out[idx];
+    if (o == cap || (bx.in_fill(i, j, k) && o == floor[idx])) lam[idx] = real(0);
+  });
 }
-// Cap after every step, u = min(u, cap) over the ghost box, and its
-// transpose (lam = 0 where the cap was taken).  Without reinitialization
+// Cap after every step, u = min(u, cap) over the ghost box (its transpose,
+// lam = 0 where the cap was taken, is in adj_cap_impose_kernel).  Without reinitialization
 // between steps, the signed extrapolation at a face where the mask's
 // advection velocity points into the box feeds a downwind difference, and
 // phi grows exponentially there (in solid, away from the interface).  A cap
@@ -459,11 +464,6 @@ __global__ void adj_cap_kernel(real* u, real cap, int64_t n) {
     const real v = u[l];
     u[l] = v > cap ? cap : v;
   }
-}
-__global__ void adj_cap_rev_kernel(real* lam, const real* out, real cap, int64_t n) {
-  for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n;
-       l += (int64_t)gridDim.x * blockDim.x)
-    if (out[l] == cap) lam[l] = 0;
 }
 // d loss / d phi on the fill box: 2 (S_w - target) dV H'(phi) / V_pore, with
 // H the smoothed volume fraction of the volume reduction
@@ -641,10 +641,6 @@ struct Adjoint {
     else hipLaunchKernelGGL(adj_bc_kernel<false>, dim3(kBlocks), dim3(256), 0, s->st, s->bx, l, src, s->floor);
     ACHECK(hipGetLastError());
   }
-  void axpy(real* d, const real* src, real alpha) {
-    hipLaunchKernelGGL(adj_axpy_kernel, dim3(kBlocks), dim3(256), 0, s->st, s->bx, d, src, alpha);
-    ACHECK(hipGetLastError());
-  }
 
   // lam: adjoint of u_{n+1} in, adjoint of u_n out (u: state u_n).  The
   // step's stage values (u1, u2, u3 and the pre-imposition face layers) are
@@ -658,18 +654,25 @@ struct Adjoint {
     const real* alpha = nstage == 3 ? alpha3 : alpha2;
     const real* beta = nstage == 3 ? beta3 : beta2;
     const real* ins[3] = {u, u1, u2};
-    zero(lamn);
-    for (real* l : lams) zero(l);
-    // imposition of the last stage, then its ghost values
-    hipLaunchKernelGGL(adj_cap_rev_kernel, dim3(kBlocks), dim3(256), 0, s->st, lam, u3, cap, s->n);
-    hipLaunchKernelGGL(adj_impose_kernel, dim3(kBlocks), dim3(256), 0, s->st, s->bx, lam, u3, s->floor);
+    const dim3 rg(row_blocks<false>(s->bx, 16384)), rb(ROW_TX, ROW_TY);  // for_rows' launch shape
+    // cap and imposition of the last stage, then its ghost values
+    hipLaunchKernelGGL(adj_cap_impose_kernel, rg, rb, 0, s->st, s->bx, lam, u3, cap, s->floor);
     ACHECK(hipGetLastError());
     bc_transpose(lam, pre, true);
+    // lamn and lams[] start from their first term of this step
+    bool set_n = false, set_l[2] = {false, false};
     for (int k = nstage - 1; k >= 0; --k) {
       real* lk = k == nstage - 1 ? lam : lams[k];
       real* prev = k == 0 ? lamn : lams[k - 1];
-      if (alpha[k] != real(0)) axpy(lamn, lk, alpha[k]);
-      axpy(prev, lk, beta[k]);
+      real* a = alpha[k] != real(0) ? lamn : nullptr;  // never prev: alpha[0] = 0
+      const bool a_assign = a && !set_n;
+      if (a) set_n = true;
+      bool& pset = k == 0 ? set_n : set_l[k - 1];
+      const bool b_assign = !pset;
+      pset = true;
+      hipLaunchKernelGGL(adj_combine_kernel, rg, rb, 0, s->st, s->bx, a, alpha[k], a_assign ? 1 : 0,
+                         prev, beta[k], b_assign ? 1 : 0, (const real*)lk);
+      ACHECK(hipGetLastError());
       rhs_transpose(ins[k], lk, beta[k] * dt, prev);
       if (k > 0) bc_transpose(prev, ins[k], false);
     }
