@@ -111,6 +111,38 @@ def test_box_groups_do_not_change_results(groups, monkeypatch):
 
 
 @pytest.mark.gpu
+def test_box_with_invalid_input_fails_the_batch_without_stalling_it():
+    """A box below the drivers' minimum size fails on the host before any
+    device work; the other boxes still run to their end (their groups' rounds
+    continue without it) and the batch returns -2 with that box's message."""
+    from lsmpqs_amd.abi import make_grid
+    dt = np.float32
+    print("case: batch fp32 with one undersized box", flush=True)
+    lib = hiplib.load(dt)
+    lib.lsmpqs_set_dt_slabs(4)
+    os.environ["LSMPQS_NO_STEP_FILES"] = "1"
+    items = boxes(dt)
+    g_bad = make_grid((3, 3, 3), dtype=dt)
+    A_bad = Arrays(g_bad, dt)
+    shape = tuple(g_bad.grid_dims_ghostbox[i] for i in (2, 1, 0))
+    A_bad["mask"] = -np.ones(shape, dtype=dt)
+    A_bad["phi"] = np.ones(shape, dtype=dt)
+    o_bad = options(dt, space=3, time=3)
+    items.insert(2, (g_bad, o_bad, A_bad))
+    n = len(items)
+    opts = (C.c_void_p * n)(*[C.addressof(o) for _, o, _ in items])
+    recs = (C.c_void_p * n)(*[C.addressof(A.rec) for _, _, A in items])
+    grids = (C.c_void_p * n)(*[C.addressof(g) for g, _, _ in items])
+    st = hiplib.BatchStats()
+    rc = lib.lsmpqs_batch_model3d(n, opts, recs, grids, None, 1, C.byref(st))
+    msg = hiplib.last_error(lib)
+    for _, _, A in items:
+        lib.lsmpqs_release(A.ptr())
+    assert rc == -2 and "box 2" in msg and "4 fill voxels" in msg, msg
+    assert st.box_steps > 0  # the valid boxes ran
+
+
+@pytest.mark.gpu
 def test_sixty_four_subvolumes_of_a_768_pack():
     """The full batched case: 64 subvolumes (dims uniform in {48..192} per
     axis) of one seeded 768^3 pack (radius U[6,12], porosity 0.36, seed 1004),
