@@ -819,8 +819,9 @@ def main():
                          "profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(z-slab sharding is described in DESIGN.md, not built)", file=sys.stderr)
+        print("bench.py runs on one GPU only: the library has no multi-GPU path "
+              "(a z-slab split with 3-plane halos over xGMI is the design, not built)",
+              file=sys.stderr)
         sys.exit(2)
     if args.calib_copy:
         calibration_copy(args.calib_copy)

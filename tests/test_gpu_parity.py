@@ -148,8 +148,8 @@ FP32_CASES = [
 @pytest.mark.parametrize("name,opts", FP32_CASES, ids=[c[0] for c in FP32_CASES])
 def test_fp32_fast_path_against_fp64_oracle(name, opts):
     """The FP32 library (fast WENO5 arithmetic, FP64 reductions) against the
-    FP64 oracle: saturation within 1e-4, errors reported as the north star
-    asks (saturation, phi L-infinity in the |phi| < 3 band, interface voxels)."""
+    FP64 oracle: saturation within 1e-4, errors reported for saturation, phi
+    L-infinity in the |phi| < 3 band and interface voxels."""
     print(f"case fp32 fast {name}", flush=True)
     A64, _, g = run_model("oracle", np.float64, 32, opts)
     Ah, _, g32 = run_model("hip", np.float32, 32, opts)
