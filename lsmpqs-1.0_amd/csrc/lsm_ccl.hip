@@ -419,7 +419,9 @@ __device__ __forceinline__ void trap_nw_bin_body(const TrapArgs& a, int lb, int 
       real v = p[u];
       if (im && m[u] > v) v = m[u];  // phi = max(phi, mask) first (the var path's imposition)
       if (mn[u] > real(0)) v = -mn[u];
-      if (im || mn[u] > real(0)) phi[l] = v;
+      // stored only where it changed: most voxels keep phi, and waves with
+      // no change then write nothing
+      if ((im || mn[u] > real(0)) && v != p[u]) phi[l] = v;
       bin[l] = v < real(0) ? 1 : 0;
     }
   }
