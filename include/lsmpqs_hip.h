@@ -327,7 +327,10 @@ int lsmpqs_batch_model3d(int n, Options **options, QSS_DataArrays **p, Grid **g,
    loss only, no reverse pass, dloss_db = 0).  The
    reverse pass transposes the linearisation of every stage with the
    forward's discrete choices (ENO stencils, upwind sides, Godunov branch,
-   extrapolation signs, imposition) held fixed; states are stored every
+   extrapolation signs, imposition) held fixed.  With a gradient requested
+   and every step's state and three stage values in half the free device
+   memory, the forward keeps them all and nothing is recomputed; otherwise
+   (or with LSMPQS_ADJOINT_STORE=segment, or =none) states are stored every
    checkpoint_every steps and recomputed in between.  Returns 0, or -2 with
    the message in lsmpqs_hip_last_error(). */
 typedef struct {

@@ -14,8 +14,10 @@
 // max.  WENO weights are differentiated exactly.  Local Jacobians of the
 // scheme come from the forward's own templated functions evaluated on dual
 // numbers (7 partials per axis line); curvature partials are written out.
-// Transposed stencils scatter with atomics.  States are checkpointed every
-// k steps and recomputed between checkpoints.
+// Transposed stencils scatter with atomics.  The forward keeps every step's
+// state and stage values when they fit in half the free device memory;
+// otherwise states are checkpointed every k steps and recomputed between
+// checkpoints.
 #include <math.h>
 #include <stdio.h>
 #include <algorithm>
@@ -693,6 +695,11 @@ extern "C" int lsmpqs_adjoint_run(Options* o, QSS_DataArrays* p, Grid* g, const 
       throw std::runtime_error("lsmpqs_adjoint_run: options, arrays (phi, mask), grid, theta and args required");
     if (args->nsteps < 1 || args->checkpoint_every < 1)
       throw std::runtime_error("lsmpqs_adjoint_run: nsteps and checkpoint_every must be >= 1");
+    // what the reverse pass keeps: all, segment or none (described at the forward)
+    const char* sm = getenv("LSMPQS_ADJOINT_STORE");
+    const std::string store_mode = sm && *sm ? sm : "all";
+    if (store_mode != "all" && store_mode != "segment" && store_mode != "none")
+      throw std::runtime_error("lsmpqs_adjoint_run: LSMPQS_ADJOINT_STORE must be all, segment or none");
     // LSMPQS_ADJOINT_TRACE=1: host-time breakdown on stderr
     const char* te = getenv("LSMPQS_ADJOINT_TRACE");
     const bool tr = te && *te && atoi(te) != 0;
@@ -771,21 +778,57 @@ extern "C" int lsmpqs_adjoint_run(Options* o, QSS_DataArrays* p, Grid* g, const 
     args->cap_used = (double)A.cap;
     ACHECK(hipMemcpyAsync(s->dt, &dt, sizeof(real), hipMemcpyHostToDevice, s->st));
     mark("set-up", s->st);
-    // forward with checkpoints every k steps
+    // What the reverse pass keeps (LSMPQS_ADJOINT_STORE):
+    //   all (default): with a gradient requested and the whole trajectory
+    //     (4 fields per step: the state after it and its three stage values)
+    //     in half the free device memory, the forward keeps every step and the
+    //     reverse pass recomputes nothing (checkpoint_every is then unused);
+    //   segment, or all when the trajectory does not fit: states every
+    //     checkpoint_every steps, each segment recomputed once, keeping its
+    //     stage values when they fit;
+    //   none: states every checkpoint_every steps, each segment recomputed,
+    //     and every step once more in the reverse pass.
+    // The kernels are deterministic, so every mode replays the same values.
     const int N = args->nsteps, K = std::min(args->checkpoint_every, args->nsteps);
-    const int nseg = (N + K - 1) / K;
-    std::vector<real*> ck(nseg), seg(K);
+    size_t mem_free = 0, mem_total = 0;
+    const bool whole = dloss_dtheta && store_mode == "all" &&
+                       hipMemGetInfo(&mem_free, &mem_total) == hipSuccess &&
+                       (size_t)4 * (size_t)N * bytes < mem_free / 2;
+    // whole trajectory: state q in wst[q] (wst[0] = u), stage values of step q
+    // in wu1[q], wu2[q], wpre[q]
+    std::vector<real*> wst, wu1, wu2, wpre;
+    const int nseg = whole ? 0 : (N + K - 1) / K;
+    std::vector<real*> ck(nseg), seg(whole ? 0 : K);
     real* un = nullptr;
-    {
+    if (whole) {
+      const std::vector<real*> f = buf.slab(4 * N, n);
+      wst.push_back(u);
+      wst.insert(wst.end(), f.begin(), f.begin() + N);
+      wu1.assign(f.begin() + N, f.begin() + 2 * N);
+      wu2.assign(f.begin() + 2 * N, f.begin() + 3 * N);
+      wpre.assign(f.begin() + 3 * N, f.end());
+    } else {
       const std::vector<real*> f = buf.slab(nseg + K + 1, n);
       for (int m = 0; m < nseg; ++m) ck[m] = f[m];
       for (int q = 0; q < K; ++q) seg[q] = f[nseg + q];
       un = f[nseg + K];
     }
-    for (int step = 0; step < N; ++step) {
-      if (step % K == 0) A.copy(ck[step / K], u);
-      A.forward_step(u, un);
-      std::swap(u, un);
+    real* const own[4] = {A.u1, A.u2, A.u3, A.pre};
+    auto use_whole = [&](int q) {
+      A.u1 = wu1[q]; A.u2 = wu2[q]; A.pre = wpre[q]; A.u3 = wst[q + 1];
+    };
+    if (whole) {
+      for (int q = 0; q < N; ++q) {
+        use_whole(q);
+        A.forward_step(wst[q], wst[q + 1]);
+      }
+      u = wst[N];
+    } else {
+      for (int step = 0; step < N; ++step) {
+        if (step % K == 0) A.copy(ck[step / K], u);
+        A.forward_step(u, un);
+        std::swap(u, un);
+      }
     }
     // loss
     const real eps = o->eps_coefficient * dx;
@@ -823,18 +866,21 @@ extern "C" int lsmpqs_adjoint_run(Options* o, QSS_DataArrays* p, Grid* g, const 
     // forward then runs twice in all instead of three times.  The stage
     // kernels are deterministic, so the stored values are the recomputed
     // ones bit for bit.
-    size_t mem_free = 0, mem_total = 0;
-    const bool store = hipMemGetInfo(&mem_free, &mem_total) == hipSuccess &&
+    const bool store = !whole && store_mode != "none" &&
+                       hipMemGetInfo(&mem_free, &mem_total) == hipSuccess &&
                        (size_t)(3 * K + 1) * bytes < mem_free / 2;
     std::vector<real*> su1, su2, spre;
     real* slast = nullptr;
-    real* const own[4] = {A.u1, A.u2, A.u3, A.pre};
     if (store) {
       const std::vector<real*> f = buf.slab(3 * K + 1, n);
       su1.assign(f.begin(), f.begin() + K);
       su2.assign(f.begin() + K, f.begin() + 2 * K);
       spre.assign(f.begin() + 2 * K, f.begin() + 3 * K);
       slast = f[3 * K];
+    }
+    for (int q = whole ? N - 1 : -1; q >= 0; --q) {
+      use_whole(q);
+      A.reverse_step(wst[q], false);
     }
     for (int m = nseg - 1; m >= 0; --m) {
       const int s0 = m * K, s1 = std::min(N, s0 + K), len = s1 - s0;

@@ -475,19 +475,33 @@ hipError_t launch_reinit_first_step_reduce(const Box& bx, real* next, const real
 }
 
 // merge and imposition floor in one pass: phi = -mask_nw where mask_nw > 0
-// (merge_kernel) and floor = max(mask - overlap, mask_w, mask_nw) (floor_kernel)
+// (merge_kernel) and floor = max(mask - overlap, mask_w, mask_nw) (floor_kernel).
+// Four voxels per thread and iteration (a stride apart), all their loads
+// issued before any value is used.
 __device__ __forceinline__ void merge_floor_body(real* phi, real* f, const real* m, const real* mw,
                                                  const real* mnw, real ov, int64_t n, int lb,
                                                  int lg) {
-  for (int64_t l = (int64_t)lb * blockDim.x + threadIdx.x; l < n;
-       l += (int64_t)lg * blockDim.x) {
-    const real b = mnw[l];
-    if (b > real(0)) phi[l] = -b;
-    real v = m[l] - ov;
-    real a = mw[l];
-    v = (a > v) ? a : v;
-    v = (b > v) ? b : v;
-    f[l] = v;
+  constexpr int U = 4;
+  const int64_t stride = (int64_t)lg * blockDim.x;
+  for (int64_t l0 = (int64_t)lb * blockDim.x + threadIdx.x; l0 < n; l0 += U * stride) {
+    real b[U], mm[U], a[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t l = min(l0 + u * stride, n - 1);
+      b[u] = mnw[l];
+      mm[u] = m[l];
+      a[u] = mw[l];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t l = l0 + u * stride;
+      if (l >= n) break;
+      if (b[u] > real(0)) phi[l] = -b[u];
+      real v = mm[u] - ov;
+      v = (a[u] > v) ? a[u] : v;
+      v = (b[u] > v) ? b[u] : v;
+      f[l] = v;
+    }
   }
 }
 __global__ void merge_floor_kernel(real* phi, real* f, const real* m, const real* mw,

@@ -108,11 +108,18 @@ def test_gradient_matches_finite_differences_fp64(space, time):
 
 
 @pytest.mark.gpu
-def test_checkpoint_interval_does_not_change_the_gradient():
-    print("case: adjoint checkpoints fp64 WENO5 RK3", flush=True)
+@pytest.mark.parametrize("store", ["all", "segment", "none"])
+def test_checkpoint_interval_does_not_change_the_gradient(store, monkeypatch):
+    """Whole trajectory kept, checkpointed segments with their stage values
+    kept, and segments recomputed twice: the same loss and final state bit
+    for bit, the same gradient up to the atomics' summation order."""
+    print(f"case: adjoint checkpoints fp64 WENO5 RK3, store {store}", flush=True)
     g, o, phi0, mask, theta = problem(np.float64, 5, 3)
-    rs = [saturation_loss(phi0, mask, theta, g, o, STEPS, A0, B, TARGET, checkpoint_every=k)
-          for k in (1, 5, STEPS)]
+    monkeypatch.delenv("LSMPQS_ADJOINT_STORE", raising=False)
+    ref = saturation_loss(phi0, mask, theta, g, o, STEPS, A0, B, TARGET, checkpoint_every=STEPS)
+    monkeypatch.setenv("LSMPQS_ADJOINT_STORE", store)
+    rs = [ref] + [saturation_loss(phi0, mask, theta, g, o, STEPS, A0, B, TARGET, checkpoint_every=k)
+                  for k in (1, 5, STEPS)]
     for r in rs[1:]:
         assert r.loss == rs[0].loss
         assert np.array_equal(r.phi, rs[0].phi)

@@ -2,7 +2,8 @@
 lsmpqs_hip_last_error (and the return code where the entry point has one)
 before any device call, so these run without a GPU: a box below the drivers'
 minimum (3 ghost layers and 4 fill voxels per axis), a batch without boxes or
-with one record given to two boxes, an adjoint run without steps."""
+with one record given to two boxes, an adjoint run without steps or with an
+unknown LSMPQS_ADJOINT_STORE mode."""
 import ctypes as C
 
 import numpy as np
@@ -59,3 +60,17 @@ def test_adjoint_without_steps_is_rejected():
     rc = lib.lsmpqs_adjoint_run(C.addressof(o), C.addressof(A.rec), C.addressof(g),
                                 theta.ctypes.data, None, C.byref(a))
     assert rc == -2 and "nsteps and checkpoint_every" in hiplib.last_error(lib)
+
+
+def test_adjoint_with_unknown_store_mode_is_rejected(monkeypatch):
+    dt = np.float64
+    lib = adjoint_lib(dt)
+    g, A = small_box(dt, 8)
+    o = options(dt)
+    theta = np.zeros(g.num_gridpts, dtype=dt)
+    a = AdjointArgs()
+    a.nsteps, a.checkpoint_every = 4, 2
+    monkeypatch.setenv("LSMPQS_ADJOINT_STORE", "everything")
+    rc = lib.lsmpqs_adjoint_run(C.addressof(o), C.addressof(A.rec), C.addressof(g),
+                                theta.ctypes.data, None, C.byref(a))
+    assert rc == -2 and "LSMPQS_ADJOINT_STORE must be" in hiplib.last_error(lib)
