@@ -28,6 +28,7 @@
 #include "lsm_adjoint.h"
 #include "lsm_batch.h"
 #include "lsm_coeff.h"
+#include "lsm_xfer.h"
 #include "lsmpqs_hip.h"
 
 namespace lsm {
@@ -564,9 +565,11 @@ struct Buffers {
     for (int q = 0; q < count; ++q) f[q] = p + stride * q;
     return f;
   }
-  ~Buffers() {
+  void release() {
     for (real* p : all) (void)hipFree(p);
+    all.clear();
   }
+  ~Buffers() { release(); }
 };
 
 struct Adjoint {
@@ -908,6 +911,7 @@ extern "C" int lsmpqs_adjoint_run(Options* o, QSS_DataArrays* p, Grid* g, const 
     // theta and b gradients
     double* part = nullptr;
     ACHECK(hipMalloc((void**)&part, sizeof(double) * kBlocks));
+    A.zero(d_theta);  // ghost entries carry no gradient (theta was staged here)
     hipLaunchKernelGGL(adj_finish_kernel, dim3(kBlocks), dim3(256), 0, s->st, bx, d_theta, A.g_vn,
                        A.dvn, A.g_b, s->mask, s->gx, s->gy, s->gz, dx, part);
     hipError_t le = hipGetLastError();
@@ -921,22 +925,10 @@ extern "C" int lsmpqs_adjoint_run(Options* o, QSS_DataArrays* p, Grid* g, const 
     double db = 0;
     for (double v : h) db += v;
     args->dloss_db = db;
-    {
-      // ghost entries carry no gradient: zero them, copy the fill rows
-      const int nx = g->grid_dims_ghostbox[0], ny = g->grid_dims_ghostbox[1];
-      const int nz = g->grid_dims_ghostbox[2];
-      std::vector<real> tmp(n);
-      ACHECK(hipMemcpy(tmp.data(), d_theta, bytes, hipMemcpyDeviceToHost));
-      std::fill(dloss_dtheta, dloss_dtheta + n, (QSSLIB_REAL)0);
-      for (int k = std::max(0, g->klo_fb); k <= std::min(nz - 1, g->khi_fb); ++k)
-        for (int j = std::max(0, g->jlo_fb); j <= std::min(ny - 1, g->jhi_fb); ++j) {
-          const int64_t row = ((int64_t)k * ny + j) * nx;
-          for (int i = std::max(0, g->ilo_fb); i <= std::min(nx - 1, g->ihi_fb); ++i)
-            dloss_dtheta[row + i] = tmp[row + i];
-        }
-    }
-    ACHECK(hipStreamSynchronize(s->st));
+    ACHECK(copy_to_host(dloss_dtheta, d_theta, bytes, s->st));
     mark("gradients to the host", s->st);
+    buf.release();
+    mark("device buffers freed", s->st);
     if (tr) {
       double prev = 0;
       for (const auto& m : marks) {
