@@ -344,7 +344,8 @@ def adjoint_leg(dtype, n=128, steps=200, checkpoint_every=20):
     field (40 deg + Gaussian random field, sigma 5 deg, correlation length
     10; n^3 pack of radius 8, seed 1005) and b, through `steps` WENO5 TVD-RK3
     steps at a0/b = 0.15, b = 0.04: host wall time of the forward map alone
-    and of forward + checkpointed reverse pass (lsmpqs_adjoint_run)."""
+    and of forward + reverse pass (lsmpqs_adjoint_run; the forward keeps every
+    step when it fits in device memory, checkpoint_every applies otherwise)."""
     from lsmpqs_amd.adjoint import saturation_loss
     from lsmpqs_amd.inputs import contact_angle_case
     g, mask, phi0, theta = contact_angle_case(n, dtype=dtype)
