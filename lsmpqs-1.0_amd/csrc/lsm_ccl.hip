@@ -283,6 +283,18 @@ __device__ __forceinline__ void border_job(const Box& bx, int* L, const int* T, 
   }
 }
 
+// Threads per border block by axis; each thread takes face voxels a block
+// stride apart.  A block's time is mostly the latency of its loads and
+// union-find hops: for the x and y faces 256-thread blocks (eight tiles in
+// flight per CU, 16 KB of LDS each) finish sooner than 1024-thread ones (two
+// tiles per CU; 512^3 pack, y faces 131 -> 93 us), but the z faces, where
+// the largest components meet, took 325 -> 584 us with them (more tiles
+// linking the same few roots at once).
+template <int AX>
+constexpr int border_threads() {
+  return AX == 2 ? 1024 : 256;
+}
+
 template <int AX>
 __device__ __forceinline__ void ccl_border_tile(const Box& bx, int* L, const int* T, int conn,
                                                 int cx, int cy, int cz) {
@@ -360,13 +372,16 @@ hipError_t launch_label_components(const Box& bx, const uint8_t* bin, int* L, in
   if (roots) hipLaunchKernelGGL(ccl_reset_kernel, dim3(1), dim3(1), 0, st, roots);
   hipLaunchKernelGGL(ccl_local_kernel, tiles, dim3(CX, CY, 1), 0, st, bx, bin, L, T, conn, roots,
                      cap, full);
-  hipLaunchKernelGGL(ccl_border_kernel<0>, tiles, dim3(256), 0, st, bx, L, (const int*)T, conn);
+  hipLaunchKernelGGL(ccl_border_kernel<0>, tiles, dim3(border_threads<0>()), 0, st, bx, L,
+                     (const int*)T, conn);
   hipLaunchKernelGGL(ccl_jump_kernel, dim3(gj), dim3(256), 0, st, L, (const int*)T, (int)n,
                      (const int*)roots, cap);
-  hipLaunchKernelGGL(ccl_border_kernel<1>, tiles, dim3(1024), 0, st, bx, L, (const int*)T, conn);
+  hipLaunchKernelGGL(ccl_border_kernel<1>, tiles, dim3(border_threads<1>()), 0, st, bx, L,
+                     (const int*)T, conn);
   hipLaunchKernelGGL(ccl_jump_kernel, dim3(gj), dim3(256), 0, st, L, (const int*)T, (int)n,
                      (const int*)roots, cap);
-  hipLaunchKernelGGL(ccl_border_kernel<2>, tiles, dim3(1024), 0, st, bx, L, (const int*)T, conn);
+  hipLaunchKernelGGL(ccl_border_kernel<2>, tiles, dim3(border_threads<2>()), 0, st, bx, L,
+                     (const int*)T, conn);
   hipLaunchKernelGGL(ccl_jump_kernel, dim3(gj), dim3(256), 0, st, L, (const int*)T, (int)n,
                      (const int*)roots, cap);
   if (full) hipLaunchKernelGGL(ccl_resolve_kernel, dim3(grid_for(n)), dim3(256), 0, st, L,
@@ -671,11 +686,14 @@ static void label_multi(const OuterJob* J, const Multi& one, const Multi& tiles,
                         int conn, hipStream_t st) {
   hipLaunchKernelGGL(ccl_reset_multi, dim3(one.total), dim3(64), 0, st, J, one);
   hipLaunchKernelGGL(ccl_local_multi, dim3(tiles.total), dim3(CX, CY, 1), 0, st, J, tiles, conn);
-  hipLaunchKernelGGL(ccl_border_multi<0>, dim3(tiles.total), dim3(256), 0, st, J, tiles, conn);
+  hipLaunchKernelGGL(ccl_border_multi<0>, dim3(tiles.total), dim3(border_threads<0>()), 0, st,
+                     J, tiles, conn);
   hipLaunchKernelGGL(ccl_jump_multi, dim3(jump.total), dim3(256), 0, st, J, jump);
-  hipLaunchKernelGGL(ccl_border_multi<1>, dim3(tiles.total), dim3(1024), 0, st, J, tiles, conn);
+  hipLaunchKernelGGL(ccl_border_multi<1>, dim3(tiles.total), dim3(border_threads<1>()), 0, st,
+                     J, tiles, conn);
   hipLaunchKernelGGL(ccl_jump_multi, dim3(jump.total), dim3(256), 0, st, J, jump);
-  hipLaunchKernelGGL(ccl_border_multi<2>, dim3(tiles.total), dim3(1024), 0, st, J, tiles, conn);
+  hipLaunchKernelGGL(ccl_border_multi<2>, dim3(tiles.total), dim3(border_threads<2>()), 0, st,
+                     J, tiles, conn);
   hipLaunchKernelGGL(ccl_jump_multi, dim3(jump.total), dim3(256), 0, st, J, jump);
 }
 
