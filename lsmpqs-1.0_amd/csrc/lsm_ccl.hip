@@ -288,8 +288,8 @@ __device__ __forceinline__ void border_job(const Box& bx, int* L, const int* T, 
 // union-find hops: for the x and y faces 256-thread blocks (eight tiles in
 // flight per CU, 16 KB of LDS each) finish sooner than 1024-thread ones (two
 // tiles per CU; 512^3 pack, y faces 131 -> 93 us), but the z faces, where
-// the largest components meet, took 325 -> 584 us with them (more tiles
-// linking the same few roots at once).
+// the largest components meet, took 320 / 454 / 584 us with 1024- / 512- /
+// 256-thread blocks (more tiles linking the same few roots at once).
 template <int AX>
 constexpr int border_threads() {
   return AX == 2 ? 1024 : 256;
