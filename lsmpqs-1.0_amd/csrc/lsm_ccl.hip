@@ -273,13 +273,27 @@ __device__ __forceinline__ void border_job(const Box& bx, int* L, const int* T, 
     if (i2 < 0 || i2 >= bx.nx || j2 < 0 || j2 >= bx.ny || k2 < 0 || k2 >= bx.nz) continue;
     nb[q] = T[i2 + bx.nx * j2 + plane * k2];
   }
+  // z faces: unite from the roots the tile roots pointed at after the last
+  // root-jump pass (each an ancestor of its tile root, so a valid start for
+  // the union).  The pairs of one tile face then collapse in the pair set to
+  // the few pairs of components that meet there, instead of one union per
+  // pair of tile-local components, and equal roots need no union: on the
+  // last pass, where the largest components meet, every tile of a z layer
+  // otherwise walks the same few root entries (512^3 pack: 320 -> 128 us;
+  // the x and y passes, with mostly distinct roots, take longer with the
+  // extra loads).
+  const int ra = AX == 2 ? ld_dev(&L[a0]) : a0;
+  int rb[26];
+#pragma unroll
+  for (int q = 0; q < 26; ++q)
+    rb[q] = (nb[q] >= 0 && nb[q] != a0) ? (AX == 2 ? ld_dev(&L[nb[q]]) : nb[q]) : -1;
   int last = -1;
 #pragma unroll
   for (int q = 0; q < 26; ++q) {
-    const int b0 = nb[q];
-    if (b0 < 0 || b0 == a0 || b0 == last) continue;
-    last = b0;
-    if (pair_insert(set, a0, b0)) uf_unite(L, a0, b0);
+    const int b = rb[q];
+    if (b < 0 || b == ra || b == last) continue;
+    last = b;
+    if (pair_insert(set, ra, b)) uf_unite(L, ra, b);
   }
 }
 
