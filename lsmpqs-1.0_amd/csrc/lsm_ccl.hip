@@ -477,9 +477,15 @@ __device__ __forceinline__ void trap_w_bin_body(const TrapArgs& a, int lb, int l
     for (int u = 0; u < UNR; ++u) {
       const int64_t l = l0 + u * stride;
       const bool in = l < n;
-      t[u] = in ? T[l] : -1;
       p[u] = in ? phi[l] : real(0);
       m[u] = in ? mask[l] : real(0);
+    }
+    // the label is needed only inside the non-wetting phase (phi < 0), a
+    // small part of the box for most of a drainage: elsewhere T is not read
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      const int64_t l = l0 + u * stride;
+      t[u] = (l < n && p[u] < real(0)) ? T[l] : -1;
     }
     int cv[UNR];
 #pragma unroll
