@@ -595,9 +595,14 @@ __device__ __forceinline__ void trap_w_mask_body(const TrapArgs& a, int lb, int 
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
       const int64_t l = l0 + u * stride;
-      const bool in = l < n;
-      t[u] = in ? T[l] : -1;
-      v[u] = in ? sw[l] : real(0);
+      v[u] = l < n ? sw[l] : real(0);
+    }
+    // the label is needed only inside the wetting phase (s < 0): the solid
+    // and the non-wetting phase do not read T
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      const int64_t l = l0 + u * stride;
+      t[u] = (l < n && v[u] < real(0)) ? T[l] : -1;
     }
     int cv[UNR];
 #pragma unroll
